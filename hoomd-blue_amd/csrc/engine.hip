@@ -1,0 +1,1287 @@
+// engine.hip -- device-resident particle system that runs whole timesteps natively.
+//
+// Host structure (what the reference spreads over System, IntegratorTwoStep, NeighborList,
+// CellList, PotentialPair, PotentialBond, TwoStepConstantVolume / TwoStepLangevin, the Bussi
+// thermostat and SFCPackTuner) collapses into one object that owns every per-particle array in
+// HBM. A timestep is a fixed launch sequence
+//     step-begin (timestep, Bussi factor) -> step one (+ distance check, clear cell counts)
+//     -> cell build* -> neighbour list build* -> pair/bond forces -> step two (+ KE partials)
+// where * runs only when the device-side distance check (or a forced update) requested a
+// rebuild for this step. Nothing in the sequence reads back to the host, so chunks of steps are
+// captured once as a hipGraph and replayed. Conditions (overflow, NaN, out-of-box, bond stretch)
+// are read once per chunk; an overflow restores the chunk's starting state, grows the buffer
+// and replays the chunk, which gives the reference's regrow-and-repeat semantics
+// (NeighborList.cc:411-427, CellList.cc:183-196) without a per-step host sync.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "capi_util.h"
+#include "kernels.h"
+
+using namespace hm;
+
+namespace
+{
+struct HmError
+{
+    int code;
+    std::string what;
+};
+
+inline void hchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw HmError{HM_ERR_LAUNCH, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template<class T> T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess)
+        throw HmError{HM_ERR_ALLOC, "device allocation failed"};
+    return (T*)p;
+}
+
+template<class T> void dfree(T*& p)
+{
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template<class T> T* halloc(size_t n)
+{
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sizeof(T) * (n ? n : 1), hipHostMallocDefault) != hipSuccess)
+        throw HmError{HM_ERR_ALLOC, "pinned host allocation failed"};
+    memset(p, 0, sizeof(T) * (n ? n : 1));
+    return (T*)p;
+}
+
+struct ForceSlot
+{
+    int kind = 0; // 0 LJ pair, 1 FENE bond
+    float4* f = nullptr;
+    float* v = nullptr;
+    // LJ
+    double rcut[HM_MAX_TYPES * HM_MAX_TYPES] = {};
+    double ron[HM_MAX_TYPES * HM_MAX_TYPES] = {};
+    hm_lj_params p[HM_MAX_TYPES * HM_MAX_TYPES] = {};
+    int mode = 0;
+    // FENE, CSR by tag
+    unsigned int* row = nullptr;
+    unsigned int* other = nullptr;
+    unsigned int* btype = nullptr;
+    unsigned int nbt = 1;
+    hm_fene_params fp[HM_MAX_TYPES] = {};
+    unsigned int* flag = nullptr;
+};
+} // namespace
+
+struct hm_system
+{
+    unsigned int N = 0, ntypes = 1;
+    hm_box hbox{};
+    DevBox box{};
+    uint16_t seed = 0;
+    hipStream_t stream = nullptr;
+
+    double4 *pos = nullptr, *vel = nullptr;
+    d3* accel = nullptr;
+    i3* image = nullptr;
+    unsigned int *tag = nullptr, *rtag = nullptr;
+    d3* last = nullptr;
+
+    std::vector<ForceSlot> forces;
+
+    // neighbour list
+    double r_buff = 0.4;
+    int exclude_bonds = 1;
+    unsigned int dim[3] = {0, 0, 0};
+    unsigned int ncell = 0, n_adj = 0, cmax = 0, nmax = 0;
+    unsigned int *cell_size = nullptr, *cell_adj = nullptr, *nlist = nullptr, *n_neigh = nullptr;
+    hm_cell_entry* entries = nullptr;
+    double rcut_mat[HM_MAX_TYPES * HM_MAX_TYPES] = {};
+    double rcut_max = 0.0;
+    unsigned int max_excl = 0;
+    unsigned int *n_excl_tag = nullptr, *excl_tag = nullptr, *n_excl = nullptr, *excl = nullptr;
+
+    // device state
+    StepState* st = nullptr;
+    Conditions* cond = nullptr;
+    StepState* h_st = nullptr;
+    Conditions* h_cond = nullptr;
+    double* ke_partial = nullptr;
+    unsigned int n_partial = 0;
+    double* thermo_partial = nullptr;
+    double* thermo_out = nullptr;
+    double* h_thermo = nullptr;
+
+    // method
+    int method = -1;
+    double dt = 0.005, kT = 1.0, tau = 0.0;
+    double gamma[HM_MAX_TYPES] = {1.0, 1.0, 1.0, 1.0};
+    double* d_gamma = nullptr;
+
+    // sorter
+    unsigned int sort_period = 0, sort_grid = 256;
+    unsigned long long *keys_a = nullptr, *keys_b = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    unsigned int* order = nullptr;
+    void* scratch = nullptr; // N * 32 bytes
+    uint64_t last_sort_step = ~0ull;
+
+    // run control
+    uint64_t timestep = 0;
+    bool prepared = false;
+    int compute_virial = 0;
+    PairLaunch pair_cfg{4, 256};
+    unsigned int chunk = 100;
+    bool use_graph = true;
+    std::map<unsigned int, hipGraphExec_t> graphs;
+
+    // rollback snapshot
+    double4 *s_pos = nullptr, *s_vel = nullptr;
+    d3 *s_accel = nullptr, *s_last = nullptr;
+    i3* s_image = nullptr;
+    unsigned int* s_tag = nullptr;
+    double* s_ke = nullptr;
+    StepState s_state{};
+
+    uint64_t n_sorts = 0, steps_run = 0, rollbacks = 0;
+
+    ~hm_system() { release(); }
+
+    void release()
+    {
+        clear_graphs();
+        dfree(pos);
+        dfree(vel);
+        dfree(accel);
+        dfree(image);
+        dfree(tag);
+        dfree(rtag);
+        dfree(last);
+        for (auto& f : forces)
+        {
+            dfree(f.f);
+            dfree(f.v);
+            dfree(f.row);
+            dfree(f.other);
+            dfree(f.btype);
+            dfree(f.flag);
+        }
+        forces.clear();
+        dfree(cell_size);
+        dfree(cell_adj);
+        dfree(nlist);
+        dfree(n_neigh);
+        dfree(entries);
+        dfree(n_excl_tag);
+        dfree(excl_tag);
+        dfree(n_excl);
+        dfree(excl);
+        dfree(st);
+        dfree(cond);
+        if (h_st)
+            (void)hipHostFree(h_st);
+        if (h_cond)
+            (void)hipHostFree(h_cond);
+        if (h_thermo)
+            (void)hipHostFree(h_thermo);
+        h_st = nullptr;
+        h_cond = nullptr;
+        h_thermo = nullptr;
+        dfree(ke_partial);
+        dfree(thermo_partial);
+        dfree(thermo_out);
+        dfree(d_gamma);
+        dfree(keys_a);
+        dfree(keys_b);
+        if (sort_tmp)
+            (void)hipFree(sort_tmp);
+        sort_tmp = nullptr;
+        dfree(order);
+        if (scratch)
+            (void)hipFree(scratch);
+        scratch = nullptr;
+        dfree(s_pos);
+        dfree(s_vel);
+        dfree(s_accel);
+        dfree(s_last);
+        dfree(s_image);
+        dfree(s_tag);
+        dfree(s_ke);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    void clear_graphs()
+    {
+        for (auto& kv : graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
+    // ------------------------------------------------------------------ configuration
+    void configure_nlist()
+    {
+        // NeighborList::updateRList: r_cut matrix = max over the pair forces (NeighborList.cc:452-532)
+        for (auto& v : rcut_mat)
+            v = 0.0;
+        rcut_max = 0.0;
+        for (auto& f : forces)
+            if (f.kind == 0)
+                for (unsigned int q = 0; q < ntypes * ntypes; q++)
+                    rcut_mat[q] = std::max(rcut_mat[q], f.rcut[q]);
+        for (unsigned int q = 0; q < ntypes * ntypes; q++)
+            rcut_max = std::max(rcut_max, rcut_mat[q]);
+        if (rcut_max <= 0.0)
+            return;
+        // NeighborList::checkBoxSize (NeighborList.cc:538-561)
+        double rmax = rcut_max + r_buff;
+        for (int d = 0; d < 3; d++)
+            if (hbox.L[d] <= rmax * 2.0)
+                throw HmError{HM_ERR_BOXSMALL, "nlist: simulation box is too small for r_cut + buffer"};
+        unsigned int nd[3];
+        hm_cell_dims(&hbox, rmax, nd);
+        unsigned int nc = nd[0] * nd[1] * nd[2];
+        if (nc != ncell || !cell_adj)
+        {
+            dim[0] = nd[0];
+            dim[1] = nd[1];
+            dim[2] = nd[2];
+            ncell = nc;
+            n_adj = hm_cell_n_adj(dim);
+            dfree(cell_size);
+            dfree(cell_adj);
+            dfree(entries);
+            cell_size = dalloc<unsigned int>(ncell);
+            cell_adj = dalloc<unsigned int>((size_t)ncell * n_adj);
+            hchk(launch_cell_adj(dim, cell_adj, stream), "cell adjacency");
+            cmax = 0;
+        }
+        double vol = hbox.L[0] * hbox.L[1] * hbox.L[2];
+        double rho = N / vol;
+        double w3 = (hbox.L[0] / dim[0]) * (hbox.L[1] / dim[1]) * (hbox.L[2] / dim[2]);
+        if (cmax == 0)
+        {
+            cmax = (unsigned int)std::min(256.0, std::ceil(rho * w3 * 2.0 + 16.0));
+            dfree(entries);
+            entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
+        }
+        if (nmax == 0)
+        {
+            double expect = rho * 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
+            nmax = (unsigned int)std::ceil(expect * 1.3 + 16.0);
+            nmax = (nmax + 3) & ~3u;
+            dfree(nlist);
+            nlist = dalloc<unsigned int>((size_t)N * nmax);
+            hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+        }
+        if (!n_neigh)
+        {
+            n_neigh = dalloc<unsigned int>(N);
+            hchk(hipMemsetAsync(n_neigh, 0, sizeof(unsigned int) * N, stream), "clear nlist");
+        }
+        clear_graphs();
+    }
+
+    void grow_cmax(unsigned int need)
+    {
+        unsigned int c = std::min(256u, (unsigned int)std::ceil(need * 1.3 + 4.0));
+        if (need > 256)
+            throw HmError{HM_ERR_ARGS, "cell list: more than 256 particles in one cell"};
+        if (c <= cmax)
+            c = std::min(256u, cmax + 8);
+        cmax = c;
+        dfree(entries);
+        entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
+        clear_graphs();
+    }
+
+    void grow_nmax(unsigned int need)
+    {
+        // Nmax rounds up to a multiple of 4 (NeighborList.cc:1601-1617); extra headroom avoids
+        // repeated replays when the maximum drifts upward by one or two.
+        unsigned int m = (unsigned int)std::ceil(need * 1.15 + 8.0);
+        m = (m + 3) & ~3u;
+        if (m <= nmax)
+            m = nmax + 8;
+        nmax = m;
+        dfree(nlist);
+        nlist = dalloc<unsigned int>((size_t)N * nmax);
+        hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+        clear_graphs();
+    }
+
+    void update_exclusion_index()
+    {
+        if (max_excl && exclude_bonds)
+            hchk(launch_excl_index(N, tag, rtag, n_excl_tag, excl_tag, max_excl, n_excl, excl,
+                                   stream),
+                 "exclusion index");
+    }
+
+    void fill_nlist_args(NlistKernelArgs& k, bool gated)
+    {
+        memset(&k, 0, sizeof(k));
+        k.N = N;
+        k.box = box;
+        k.ncell = ncell;
+        k.cmax = cmax;
+        k.cell_size = cell_size;
+        k.entries = entries;
+        k.cell_adj = cell_adj;
+        k.n_adj = n_adj;
+        k.ntypes = ntypes;
+        for (unsigned int q = 0; q < ntypes * ntypes; q++)
+        {
+            double rc = rcut_mat[q];
+            double rl = rc > 0.0 ? rc + r_buff : 0.0;
+            k.rlistsq[q] = rl * rl;
+            k.enabled[q] = rc > 0.0 ? 1u : 0u;
+        }
+        bool ex = max_excl && exclude_bonds;
+        k.n_excl = ex ? n_excl : nullptr;
+        k.excl = ex ? excl : nullptr;
+        k.max_excl = ex ? max_excl : 0;
+        k.nmax = nmax;
+        k.nlist = nlist;
+        k.n_neigh = n_neigh;
+        k.cond = &cond->nlist_need;
+        k.gate = gated ? st : nullptr;
+        k.counter = st;
+    }
+
+    // ------------------------------------------------------------------ kernels of one step
+    void launch_forces(int with_virial)
+    {
+        for (auto& f : forces)
+        {
+            if (f.kind == 0)
+            {
+                PairKernelArgs k;
+                fill_pair_args(k, N, pos, hbox, nlist, nmax, n_neigh, ntypes, f.rcut, f.ron,
+                               f.mode, with_virial, f.f, f.v);
+                hchk(launch_pair_lj(k, f.p, pair_cfg, stream), "pair force");
+            }
+            else
+            {
+                BondKernelArgs k;
+                memset(&k, 0, sizeof(k));
+                k.N = N;
+                k.pos = pos;
+                k.box = box;
+                k.tag = tag;
+                k.rtag = rtag;
+                k.row = f.row;
+                k.other_tag = f.other;
+                k.btype = f.btype;
+                k.nbond_types = f.nbt;
+                k.compute_virial = with_virial;
+                k.force = f.f;
+                k.virial = f.v;
+                k.flag = f.flag;
+                k.st = st;
+                hchk(launch_bond_fene(k, f.fp, stream), "bond force");
+            }
+        }
+    }
+
+    unsigned int force_ptrs(const float4** fp) const
+    {
+        unsigned int n = 0;
+        for (auto& f : forces)
+            if (n < 4)
+                fp[n++] = f.f;
+        return n;
+    }
+
+    double ndof() const
+    {
+        // IntegratorTwoStep::getTranslationalDOF: one momentum-conserving method over all
+        // particles removes D degrees of freedom (IntegratorTwoStep.cc:154-188)
+        return method == 2 ? 3.0 * N : 3.0 * N - 3.0;
+    }
+
+    void launch_step()
+    {
+        hchk(launch_step_begin(st, ke_partial, n_partial, method == 1 ? 1 : 0, ndof(), dt, tau,
+                               kT, seed, stream),
+             "step begin");
+        StepOneArgs a1;
+        memset(&a1, 0, sizeof(a1));
+        a1.N = N;
+        a1.pos = pos;
+        a1.vel = vel;
+        a1.accel = accel;
+        a1.image = image;
+        a1.box = box;
+        a1.dt = dt;
+        a1.alpha = 1.0;
+        a1.st = st;
+        a1.use_state_alpha = method == 1 ? 1 : 0;
+        bool have_nl = rcut_max > 0.0;
+        if (have_nl)
+        {
+            a1.last = last;
+            a1.maxsq = r_buff < 1e-6 ? -1.0 : distance_check_maxsq(rcut_max, r_buff);
+            a1.flag_state = st;
+            a1.cell_size = cell_size;
+            a1.ncell = ncell;
+        }
+        hchk(launch_step_one(a1, stream), "step one");
+        if (have_nl)
+        {
+            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
+                                   &cond->cell_need, st, stream),
+                 "cell build");
+            NlistKernelArgs k;
+            fill_nlist_args(k, true);
+            hchk(launch_nlist_build(k, stream), "nlist build");
+        }
+        launch_forces(compute_virial);
+        StepTwoArgs a2;
+        memset(&a2, 0, sizeof(a2));
+        a2.N = N;
+        a2.vel = vel;
+        a2.accel = accel;
+        a2.nforces = force_ptrs(a2.f);
+        a2.dt = dt;
+        a2.alpha = 1.0;
+        if (method == 2)
+        {
+            a2.gamma_by_type = d_gamma;
+            a2.pos = pos;
+            a2.tag = tag;
+            a2.kT = kT;
+            a2.seed = seed;
+        }
+        a2.st = st;
+        a2.ke_partial = ke_partial;
+        hchk(launch_step_two(a2, stream, nullptr), "step two");
+    }
+
+    // ------------------------------------------------------------------ prep / forces
+    void build_nlist_now()
+    {
+        // CellList::compute + NeighborList::compute loops with host-side regrow (the reference's
+        // behaviour on the first build, NeighborList.cc:409-427)
+        for (int attempt = 0; attempt < 64; attempt++)
+        {
+            hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
+            hchk(hipMemsetAsync(cell_size, 0, sizeof(unsigned int) * ncell, stream), "clear cells");
+            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
+                                   &cond->cell_need, nullptr, stream),
+                 "cell build");
+            hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
+                 "read conditions");
+            hchk(hipStreamSynchronize(stream), "sync");
+            if (h_cond->nan_idx)
+                throw HmError{HM_ERR_NAN, "particle position is NaN"};
+            if (h_cond->out_idx)
+                throw HmError{HM_ERR_OUTSIDE, "particle is no longer in the simulation box"};
+            if (h_cond->cell_need > cmax)
+            {
+                grow_cmax(h_cond->cell_need);
+                continue;
+            }
+            NlistKernelArgs k;
+            fill_nlist_args(k, false);
+            k.counter = nullptr;
+            hchk(launch_nlist_build(k, stream), "nlist build");
+            hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
+                 "read conditions");
+            hchk(hipStreamSynchronize(stream), "sync");
+            if (h_cond->nlist_need > nmax)
+            {
+                grow_nmax(h_cond->nlist_need);
+                continue;
+            }
+            return;
+        }
+        throw HmError{HM_ERR_LAUNCH, "neighbour list did not converge"};
+    }
+
+    void prep()
+    {
+        if (method < 0)
+            throw HmError{HM_ERR_STATE, "no integration method set"};
+        configure_nlist();
+        if (!st)
+        {
+            st = dalloc<StepState>(1);
+            cond = dalloc<Conditions>(1);
+            h_st = halloc<StepState>(1);
+            h_cond = halloc<Conditions>(1);
+        }
+        StepState s0{};
+        s0.next = timestep;
+        s0.timestep = timestep;
+        s0.rebuild_at = 0;
+        s0.alpha = 1.0;
+        hchk(hipMemcpyAsync(st, &s0, sizeof(StepState), hipMemcpyHostToDevice, stream), "state");
+        update_exclusion_index();
+        if (rcut_max > 0.0)
+            build_nlist_now();
+        launch_forces(compute_virial);
+        const float4* fp[4];
+        unsigned int nf = force_ptrs(fp);
+        hchk(launch_accel_init(N, vel, accel, fp, nf, stream), "accelerations");
+        hchk(launch_ke_partial(N, vel, ke_partial, &n_partial, stream), "kinetic energy");
+        hchk(hipStreamSynchronize(stream), "prep sync");
+        check_errors_after(false);
+        prepared = true;
+    }
+
+    void check_errors_after(bool chunk)
+    {
+        hchk(hipMemcpyAsync(h_st, st, sizeof(StepState), hipMemcpyDeviceToHost, stream), "state");
+        hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
+             "conditions");
+        hchk(hipStreamSynchronize(stream), "sync");
+        (void)chunk;
+    }
+
+    // ------------------------------------------------------------------ sort
+    void do_sort()
+    {
+        // SFCPackTuner::update (hoomd/SFCPackTuner.cc:81-285): order, permute, rtag, force update
+        int end_bit = 32 + 3 * ilog2(sort_grid);
+        if (!keys_a)
+        {
+            keys_a = dalloc<unsigned long long>(N);
+            keys_b = dalloc<unsigned long long>(N);
+            order = dalloc<unsigned int>(N);
+            hchk(sort_keys64(nullptr, nullptr, N, end_bit, nullptr, &sort_tmp_bytes, stream),
+                 "sort size");
+            hchk(hipMalloc(&sort_tmp, sort_tmp_bytes ? sort_tmp_bytes : 16), "sort scratch");
+        }
+        if (!scratch)
+            hchk(hipMalloc(&scratch, (size_t)N * 32), "sort scratch");
+        hchk(launch_sfc_keys(N, pos, box, sort_grid, keys_a, stream), "sfc keys");
+        size_t tb = sort_tmp_bytes;
+        hchk(sort_keys64(keys_a, keys_b, N, end_bit, sort_tmp, &tb, stream), "sfc sort");
+        hchk(launch_keys_to_order(N, keys_b, order, stream), "sfc order");
+        auto permute = [&](void* arr, unsigned int bytes) {
+            hchk(gather_bytes(arr, scratch, order, N, bytes, stream), "permute");
+            hchk(hipMemcpyAsync(arr, scratch, (size_t)N * bytes, hipMemcpyDeviceToDevice, stream),
+                 "permute copy");
+        };
+        permute(pos, 32);
+        permute(vel, 32);
+        permute(accel, 24);
+        permute(image, 12);
+        permute(tag, 4);
+        permute(last, 24);
+        hchk(launch_rtag(N, tag, rtag, stream), "rtag");
+        update_exclusion_index();
+        hchk(launch_set_rebuild(st, stream), "force rebuild");
+        n_sorts++;
+        last_sort_step = timestep;
+    }
+
+    // ------------------------------------------------------------------ snapshot / rollback
+    void snapshot()
+    {
+        if (!s_pos)
+        {
+            s_pos = dalloc<double4>(N);
+            s_vel = dalloc<double4>(N);
+            s_accel = dalloc<d3>(N);
+            s_last = dalloc<d3>(N);
+            s_image = dalloc<i3>(N);
+            s_tag = dalloc<unsigned int>(N);
+            s_ke = dalloc<double>(n_partial);
+        }
+        auto cp = [&](void* d, const void* s, size_t b) {
+            hchk(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream), "snapshot");
+        };
+        cp(s_pos, pos, (size_t)N * 32);
+        cp(s_vel, vel, (size_t)N * 32);
+        cp(s_accel, accel, (size_t)N * 24);
+        cp(s_last, last, (size_t)N * 24);
+        cp(s_image, image, (size_t)N * 12);
+        cp(s_tag, tag, (size_t)N * 4);
+        cp(s_ke, ke_partial, (size_t)n_partial * 8);
+        hchk(hipMemcpyAsync(&s_state, st, sizeof(StepState), hipMemcpyDeviceToHost, stream),
+             "snapshot state");
+    }
+
+    void restore()
+    {
+        auto cp = [&](void* d, const void* s, size_t b) {
+            hchk(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream), "restore");
+        };
+        cp(pos, s_pos, (size_t)N * 32);
+        cp(vel, s_vel, (size_t)N * 32);
+        cp(accel, s_accel, (size_t)N * 24);
+        cp(last, s_last, (size_t)N * 24);
+        cp(image, s_image, (size_t)N * 12);
+        cp(tag, s_tag, (size_t)N * 4);
+        cp(ke_partial, s_ke, (size_t)n_partial * 8);
+        StepState s = s_state;
+        s.error = 0;
+        hchk(hipMemcpyAsync(st, &s, sizeof(StepState), hipMemcpyHostToDevice, stream), "restore");
+        hchk(launch_rtag(N, tag, rtag, stream), "rtag");
+        update_exclusion_index();
+        hchk(hipStreamSynchronize(stream), "restore sync");
+    }
+
+    // ------------------------------------------------------------------ run
+    void launch_chunk(unsigned int K)
+    {
+        if (!use_graph)
+        {
+            for (unsigned int k = 0; k < K; k++)
+                launch_step();
+            return;
+        }
+        auto it = graphs.find(K);
+        if (it == graphs.end())
+        {
+            hipGraph_t g;
+            hchk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            try
+            {
+                for (unsigned int k = 0; k < K; k++)
+                    launch_step();
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(stream, &g);
+                throw;
+            }
+            hchk(hipStreamEndCapture(stream, &g), "end capture");
+            hipGraphExec_t ex;
+            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hchk(e, "graph instantiate");
+            it = graphs.emplace(K, ex).first;
+        }
+        hchk(hipGraphLaunch(it->second, stream), "graph launch");
+    }
+
+    int run(uint64_t nsteps)
+    {
+        if (!prepared)
+            prep();
+        uint64_t end = timestep + nsteps;
+        int retries = 0;
+        while (timestep < end)
+        {
+            if (sort_period && timestep % sort_period == 0 && last_sort_step != timestep)
+                do_sort();
+            uint64_t K = std::min<uint64_t>(chunk ? chunk : 1, end - timestep);
+            if (sort_period)
+            {
+                uint64_t next_sort = (timestep / sort_period + 1) * sort_period;
+                K = std::min<uint64_t>(K, next_sort - timestep);
+            }
+            hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
+            snapshot();
+            launch_chunk((unsigned int)K);
+            check_errors_after(true);
+            const Conditions c = *h_cond;
+            const StepState s = *h_st;
+            if (c.cell_need > cmax || c.nlist_need > nmax)
+            {
+                restore();
+                if (c.cell_need > cmax)
+                    grow_cmax(c.cell_need);
+                if (c.nlist_need > nmax)
+                    grow_nmax(c.nlist_need);
+                hchk(launch_set_rebuild(st, stream), "force rebuild");
+                rollbacks++;
+                if (++retries > 32)
+                    throw HmError{HM_ERR_LAUNCH, "neighbour list kept overflowing"};
+                continue;
+            }
+            if (c.nan_idx || c.out_idx || (s.error & (ERR_BOND | ERR_ZERO_KE)))
+            {
+                restore();
+                if (c.nan_idx)
+                    throw HmError{HM_ERR_NAN, "particle position is NaN"};
+                if (c.out_idx)
+                    throw HmError{HM_ERR_OUTSIDE, "particle is no longer in the simulation box"};
+                if (s.error & ERR_BOND)
+                    throw HmError{HM_ERR_BOND, "bond.fene: bond out of bounds"};
+                throw HmError{HM_ERR_THERMO, "Bussi thermostat requires non-zero initial momenta"};
+            }
+            retries = 0;
+            timestep += K;
+            steps_run += K;
+        }
+        return HM_OK;
+    }
+};
+
+// ====================================================================== C ABI
+namespace
+{
+int guard_fail(const HmError& e)
+{
+    return fail(e.code, e.what.c_str());
+}
+
+template<class F> int guarded(F&& f)
+{
+    try
+    {
+        return f();
+    }
+    catch (const HmError& e)
+    {
+        return guard_fail(e);
+    }
+    catch (const std::bad_alloc&)
+    {
+        return fail(HM_ERR_ALLOC, "host allocation failed");
+    }
+    catch (const std::exception& e)
+    {
+        return fail(HM_ERR_LAUNCH, e.what());
+    }
+}
+} // namespace
+
+extern "C"
+{
+
+hm_system* hm_sys_create(unsigned int N, hm_box box, unsigned int ntypes, const double* pos3,
+                         const double* vel3, const unsigned int* type_id, const double* mass,
+                         const int* image3, uint16_t seed)
+{
+    if (N == 0 || !pos3 || ntypes == 0 || ntypes > HM_MAX_TYPES)
+    {
+        fail(HM_ERR_ARGS, "hm_sys_create: invalid arguments");
+        return nullptr;
+    }
+    hm_system* s = new (std::nothrow) hm_system();
+    if (!s)
+        return nullptr;
+    int r = guarded([&]() {
+        s->N = N;
+        s->ntypes = ntypes;
+        s->hbox = box;
+        s->box = make_devbox(box);
+        s->seed = seed;
+        hchk(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream");
+        std::vector<double4> p(N), v(N);
+        std::vector<i3> im(N);
+        std::vector<unsigned int> tg(N);
+        for (unsigned int i = 0; i < N; i++)
+        {
+            unsigned int t = type_id ? type_id[i] : 0u;
+            if (t >= ntypes)
+                throw HmError{HM_ERR_ARGS, "type id out of range"};
+            p[i] = make_double4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], (double)t);
+            v[i] = make_double4(vel3 ? vel3[3 * i] : 0.0, vel3 ? vel3[3 * i + 1] : 0.0,
+                                vel3 ? vel3[3 * i + 2] : 0.0, mass ? mass[i] : 1.0);
+            im[i] = image3 ? i3{image3[3 * i], image3[3 * i + 1], image3[3 * i + 2]} : i3{0, 0, 0};
+            tg[i] = i;
+        }
+        s->pos = dalloc<double4>(N);
+        s->vel = dalloc<double4>(N);
+        s->accel = dalloc<d3>(N);
+        s->image = dalloc<i3>(N);
+        s->tag = dalloc<unsigned int>(N);
+        s->rtag = dalloc<unsigned int>(N);
+        s->last = dalloc<d3>(N);
+        s->ke_partial = dalloc<double>((N + 255) / 256);
+        s->thermo_partial = dalloc<double>(9 * THERMO_BLOCKS);
+        s->thermo_out = dalloc<double>(9);
+        s->h_thermo = halloc<double>(9);
+        s->d_gamma = dalloc<double>(HM_MAX_TYPES);
+        hchk(hipMemcpy(s->pos, p.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
+        hchk(hipMemcpy(s->vel, v.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
+        hchk(hipMemcpy(s->image, im.data(), sizeof(i3) * N, hipMemcpyHostToDevice), "upload");
+        hchk(hipMemcpy(s->tag, tg.data(), sizeof(unsigned int) * N, hipMemcpyHostToDevice),
+             "upload");
+        hchk(hipMemcpy(s->rtag, tg.data(), sizeof(unsigned int) * N, hipMemcpyHostToDevice),
+             "upload");
+        hchk(hipMemset(s->accel, 0, sizeof(d3) * N), "clear");
+        hchk(hipMemset(s->last, 0, sizeof(d3) * N), "clear");
+        return HM_OK;
+    });
+    if (r != HM_OK)
+    {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void hm_sys_destroy(hm_system* s)
+{
+    if (s)
+    {
+        (void)hipStreamSynchronize(s->stream);
+        delete s;
+    }
+}
+
+int hm_sys_set_nlist(hm_system* s, double r_buff, int exclude_bonds)
+{
+    if (!s || r_buff < 0.0)
+        return fail(HM_ERR_ARGS, "hm_sys_set_nlist: invalid arguments");
+    s->r_buff = r_buff;
+    s->exclude_bonds = exclude_bonds;
+    s->prepared = false;
+    s->nmax = 0;
+    s->clear_graphs();
+    return HM_OK;
+}
+
+int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
+                       const hm_lj_params* params, int mode)
+{
+    if (!s || !r_cut || !params || mode < 0 || mode > 2 || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_pair_lj: invalid arguments");
+    return guarded([&]() {
+        ForceSlot f;
+        f.kind = 0;
+        unsigned int nt = s->ntypes;
+        for (unsigned int q = 0; q < nt * nt; q++)
+        {
+            f.rcut[q] = r_cut[q];
+            f.ron[q] = r_on ? r_on[q] : 0.0;
+            f.p[q] = params[q];
+        }
+        f.mode = mode;
+        f.f = dalloc<float4>(s->N);
+        f.v = dalloc<float>((size_t)6 * s->N);
+        hchk(hipMemset(f.f, 0, sizeof(float4) * s->N), "clear");
+        hchk(hipMemset(f.v, 0, sizeof(float) * 6 * s->N), "clear");
+        s->forces.push_back(f);
+        s->prepared = false;
+        s->nmax = 0;
+        s->clear_graphs();
+        return (int)s->forces.size() - 1;
+    });
+}
+
+int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                         const unsigned int* bond_typeid, unsigned int nbond_types,
+                         const hm_fene_params* params)
+{
+    if (!s || (nb && !bond_tags) || !params || nbond_types == 0 || nbond_types > HM_MAX_TYPES
+        || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_bond_fene: invalid arguments");
+    return guarded([&]() {
+        unsigned int N = s->N;
+        std::vector<unsigned int> deg(N + 1, 0);
+        for (unsigned int b = 0; b < nb; b++)
+        {
+            unsigned int a = bond_tags[2 * b], c = bond_tags[2 * b + 1];
+            if (a >= N || c >= N || a == c)
+                throw HmError{HM_ERR_ARGS, "bond tag out of range"};
+            if (bond_typeid && bond_typeid[b] >= nbond_types)
+                throw HmError{HM_ERR_ARGS, "bond type out of range"};
+            deg[a]++;
+            deg[c]++;
+        }
+        std::vector<unsigned int> row(N + 1, 0);
+        for (unsigned int i = 0; i < N; i++)
+            row[i + 1] = row[i] + deg[i];
+        std::vector<unsigned int> other(row[N] ? row[N] : 1), btype(row[N] ? row[N] : 1);
+        std::vector<unsigned int> fill(row.begin(), row.end() - 1);
+        for (unsigned int b = 0; b < nb; b++)
+        {
+            unsigned int a = bond_tags[2 * b], c = bond_tags[2 * b + 1];
+            unsigned int t = bond_typeid ? bond_typeid[b] : 0u;
+            other[fill[a]] = c;
+            btype[fill[a]++] = t;
+            other[fill[c]] = a;
+            btype[fill[c]++] = t;
+        }
+        ForceSlot f;
+        f.kind = 1;
+        f.nbt = nbond_types;
+        for (unsigned int q = 0; q < nbond_types; q++)
+            f.fp[q] = params[q];
+        f.row = dalloc<unsigned int>(N + 1);
+        f.other = dalloc<unsigned int>(other.size());
+        f.btype = dalloc<unsigned int>(btype.size());
+        f.flag = dalloc<unsigned int>(1);
+        f.f = dalloc<float4>(N);
+        f.v = dalloc<float>((size_t)6 * N);
+        hchk(hipMemcpy(f.row, row.data(), sizeof(unsigned int) * (N + 1), hipMemcpyHostToDevice),
+             "upload");
+        hchk(hipMemcpy(f.other, other.data(), sizeof(unsigned int) * other.size(),
+                       hipMemcpyHostToDevice),
+             "upload");
+        hchk(hipMemcpy(f.btype, btype.data(), sizeof(unsigned int) * btype.size(),
+                       hipMemcpyHostToDevice),
+             "upload");
+        hchk(hipMemset(f.flag, 0, sizeof(unsigned int)), "clear");
+        hchk(hipMemset(f.f, 0, sizeof(float4) * N), "clear");
+        hchk(hipMemset(f.v, 0, sizeof(float) * 6 * N), "clear");
+        // bond exclusions by tag (NeighborList::addExclusionsFromBonds, NeighborList.cc:813-846)
+        unsigned int mx = 0;
+        for (unsigned int i = 0; i < N; i++)
+            mx = std::max(mx, deg[i]);
+        if (mx > 4)
+            throw HmError{HM_ERR_ARGS, "at most 4 bonds per particle are supported"};
+        if (mx > 0)
+        {
+            std::vector<unsigned int> ne(N, 0), ex((size_t)N * mx, 0xffffffffu);
+            for (unsigned int i = 0; i < N; i++)
+                for (unsigned int b = row[i]; b < row[i + 1]; b++)
+                {
+                    unsigned int o = other[b];
+                    bool dup = false;
+                    for (unsigned int q = 0; q < ne[i]; q++)
+                        dup = dup || ex[(size_t)i * mx + q] == o;
+                    if (!dup)
+                        ex[(size_t)i * mx + ne[i]++] = o;
+                }
+            dfree(s->n_excl_tag);
+            dfree(s->excl_tag);
+            dfree(s->n_excl);
+            dfree(s->excl);
+            s->max_excl = mx;
+            s->n_excl_tag = dalloc<unsigned int>(N);
+            s->excl_tag = dalloc<unsigned int>((size_t)N * mx);
+            s->n_excl = dalloc<unsigned int>(N);
+            s->excl = dalloc<unsigned int>((size_t)N * mx);
+            hchk(hipMemcpy(s->n_excl_tag, ne.data(), sizeof(unsigned int) * N,
+                           hipMemcpyHostToDevice),
+                 "upload");
+            hchk(hipMemcpy(s->excl_tag, ex.data(), sizeof(unsigned int) * ex.size(),
+                           hipMemcpyHostToDevice),
+                 "upload");
+        }
+        s->forces.push_back(f);
+        s->prepared = false;
+        s->clear_graphs();
+        return (int)s->forces.size() - 1;
+    });
+}
+
+int hm_sys_set_method(hm_system* s, int method, double dt, double kT, const double* param)
+{
+    if (!s || method < 0 || method > 2 || !(dt >= 0.0))
+        return fail(HM_ERR_ARGS, "hm_sys_set_method: invalid arguments");
+    return guarded([&]() {
+        s->method = method;
+        s->dt = dt;
+        s->kT = kT;
+        s->tau = 0.0;
+        if (method == 1 && param)
+            s->tau = param[0];
+        if (method == 2)
+        {
+            for (unsigned int q = 0; q < s->ntypes; q++)
+                s->gamma[q] = param ? param[q] : 1.0;
+            hchk(hipMemcpy(s->d_gamma, s->gamma, sizeof(double) * HM_MAX_TYPES,
+                           hipMemcpyHostToDevice),
+                 "upload");
+        }
+        s->clear_graphs();
+        return HM_OK;
+    });
+}
+
+int hm_sys_set_sorter(hm_system* s, unsigned int period, unsigned int grid)
+{
+    if (!s || (period && (grid == 0 || grid > 1024 || (grid & (grid - 1)))))
+        return fail(HM_ERR_ARGS, "hm_sys_set_sorter: invalid arguments");
+    s->sort_period = period;
+    s->sort_grid = grid;
+    return HM_OK;
+}
+
+int hm_sys_set_compute_virial(hm_system* s, int on)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_compute_virial: NULL system");
+    s->compute_virial = on ? 1 : 0;
+    s->clear_graphs();
+    return HM_OK;
+}
+
+int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_size)
+{
+    if (!s || (tpp && tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
+        || (block_size && (block_size % 64 || block_size > 1024)))
+        return fail(HM_ERR_ARGS, "hm_sys_set_pair_launch: invalid arguments");
+    if (tpp)
+        s->pair_cfg.tpp = tpp;
+    if (block_size)
+        s->pair_cfg.block = block_size;
+    s->clear_graphs();
+    return HM_OK;
+}
+
+int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_chunk: NULL system");
+    s->chunk = steps ? steps : 1;
+    s->use_graph = use_graph != 0;
+    s->clear_graphs();
+    return HM_OK;
+}
+
+int hm_sys_run(hm_system* s, uint64_t nsteps)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_run: NULL system");
+    return guarded([&]() { return s->run(nsteps); });
+}
+
+uint64_t hm_sys_timestep(const hm_system* s)
+{
+    return s ? s->timestep : 0;
+}
+
+int hm_sys_get_state(hm_system* s, double* pos3, double* vel3, int* image3, double* accel3)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_get_state: NULL system");
+    return guarded([&]() {
+        unsigned int N = s->N;
+        std::vector<double4> p(N), v(N);
+        std::vector<i3> im(N);
+        std::vector<d3> ac(N);
+        std::vector<unsigned int> tg(N);
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpy(p.data(), s->pos, sizeof(double4) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(v.data(), s->vel, sizeof(double4) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(im.data(), s->image, sizeof(i3) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(ac.data(), s->accel, sizeof(d3) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "download");
+        for (unsigned int n = 0; n < N; n++)
+        {
+            size_t t = tg[n];
+            if (pos3)
+            {
+                pos3[3 * t] = p[n].x;
+                pos3[3 * t + 1] = p[n].y;
+                pos3[3 * t + 2] = p[n].z;
+            }
+            if (vel3)
+            {
+                vel3[3 * t] = v[n].x;
+                vel3[3 * t + 1] = v[n].y;
+                vel3[3 * t + 2] = v[n].z;
+            }
+            if (image3)
+            {
+                image3[3 * t] = im[n].x;
+                image3[3 * t + 1] = im[n].y;
+                image3[3 * t + 2] = im[n].z;
+            }
+            if (accel3)
+            {
+                accel3[3 * t] = ac[n].x;
+                accel3[3 * t + 1] = ac[n].y;
+                accel3[3 * t + 2] = ac[n].z;
+            }
+        }
+        return HM_OK;
+    });
+}
+
+int hm_sys_set_state(hm_system* s, const double* pos3, const double* vel3, const int* image3)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_state: NULL system");
+    return guarded([&]() {
+        unsigned int N = s->N;
+        std::vector<double4> p(N), v(N);
+        std::vector<i3> im(N);
+        std::vector<unsigned int> tg(N);
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpy(p.data(), s->pos, sizeof(double4) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(v.data(), s->vel, sizeof(double4) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(im.data(), s->image, sizeof(i3) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "download");
+        for (unsigned int n = 0; n < N; n++)
+        {
+            size_t t = tg[n];
+            if (pos3)
+            {
+                p[n].x = pos3[3 * t];
+                p[n].y = pos3[3 * t + 1];
+                p[n].z = pos3[3 * t + 2];
+            }
+            if (vel3)
+            {
+                v[n].x = vel3[3 * t];
+                v[n].y = vel3[3 * t + 1];
+                v[n].z = vel3[3 * t + 2];
+            }
+            if (image3)
+                im[n] = i3{image3[3 * t], image3[3 * t + 1], image3[3 * t + 2]};
+        }
+        hchk(hipMemcpy(s->pos, p.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
+        hchk(hipMemcpy(s->vel, v.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
+        hchk(hipMemcpy(s->image, im.data(), sizeof(i3) * N, hipMemcpyHostToDevice), "upload");
+        s->prepared = false;
+        return HM_OK;
+    });
+}
+
+int hm_sys_get_forces(hm_system* s, int force_id, double* force4, double* virial6)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size())
+        return fail(HM_ERR_ARGS, "hm_sys_get_forces: invalid force id");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        unsigned int N = s->N;
+        auto& f = s->forces[force_id];
+        std::vector<float4> fh(N);
+        std::vector<float> vh((size_t)6 * N);
+        std::vector<unsigned int> tg(N);
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        if (virial6 && !s->compute_virial)
+        {
+            s->launch_forces(1);
+            hchk(hipStreamSynchronize(s->stream), "sync");
+        }
+        hchk(hipMemcpy(fh.data(), f.f, sizeof(float4) * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(vh.data(), f.v, sizeof(float) * 6 * N, hipMemcpyDeviceToHost), "download");
+        hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "download");
+        for (unsigned int n = 0; n < N; n++)
+        {
+            size_t t = tg[n];
+            if (force4)
+            {
+                force4[4 * t] = fh[n].x;
+                force4[4 * t + 1] = fh[n].y;
+                force4[4 * t + 2] = fh[n].z;
+                force4[4 * t + 3] = fh[n].w;
+            }
+            if (virial6)
+                for (int m = 0; m < 6; m++)
+                    virial6[(size_t)m * N + t] = vh[(size_t)m * N + n];
+        }
+        return HM_OK;
+    });
+}
+
+long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pairs)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_get_pairs: NULL system");
+    long long out = 0;
+    int r = guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        unsigned int N = s->N;
+        if (!s->nlist)
+            return HM_OK;
+        std::vector<unsigned int> nl((size_t)N * s->nmax), nn(N), tg(N);
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpy(nl.data(), s->nlist, sizeof(unsigned int) * nl.size(),
+                       hipMemcpyDeviceToHost),
+             "download");
+        hchk(hipMemcpy(nn.data(), s->n_neigh, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "download");
+        hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "download");
+        for (unsigned int i = 0; i < N; i++)
+            for (unsigned int k = 0; k < nn[i] && k < s->nmax; k++)
+            {
+                unsigned int a = tg[i], b = tg[nl[(size_t)i * s->nmax + k]];
+                if (a < b)
+                {
+                    if (pairs && out < max_pairs)
+                    {
+                        pairs[2 * out] = a;
+                        pairs[2 * out + 1] = b;
+                    }
+                    out++;
+                }
+            }
+        return HM_OK;
+    });
+    return r == HM_OK ? out : r;
+}
+
+int hm_sys_thermo(hm_system* s, double* out9)
+{
+    if (!s || !out9)
+        return fail(HM_ERR_ARGS, "hm_sys_thermo: invalid arguments");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        if (!s->compute_virial)
+            s->launch_forces(1);
+        const float4* fp[4];
+        const float* vp[4] = {nullptr, nullptr, nullptr, nullptr};
+        unsigned int nf = s->force_ptrs(fp);
+        for (unsigned int q = 0; q < nf; q++)
+            vp[q] = s->forces[q].v;
+        hchk(launch_thermo(s->N, s->vel, fp, vp, nf, s->thermo_partial, s->thermo_out, s->stream),
+             "thermo");
+        hchk(hipMemcpyAsync(s->h_thermo, s->thermo_out, sizeof(double) * 9, hipMemcpyDeviceToHost,
+                            s->stream),
+             "thermo");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        for (int k = 0; k < 9; k++)
+            out9[k] = s->h_thermo[k];
+        return HM_OK;
+    });
+}
+
+int hm_sys_stats(hm_system* s, uint64_t* out6)
+{
+    if (!s || !out6)
+        return fail(HM_ERR_ARGS, "hm_sys_stats: invalid arguments");
+    return guarded([&]() {
+        uint64_t builds = 0;
+        if (s->st)
+        {
+            StepState h;
+            hchk(hipStreamSynchronize(s->stream), "sync");
+            hchk(hipMemcpy(&h, s->st, sizeof(StepState), hipMemcpyDeviceToHost), "download");
+            builds = h.n_builds;
+        }
+        out6[0] = builds;
+        out6[1] = s->nmax;
+        out6[2] = s->cmax;
+        out6[3] = s->n_sorts;
+        out6[4] = s->steps_run;
+        out6[5] = s->rollbacks;
+        return HM_OK;
+    });
+}
+
+int hm_sys_thermalize(hm_system* s, double kT)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_thermalize: NULL system");
+    return guarded([&]() {
+        hchk(launch_thermalize(s->N, s->vel, s->tag, kT, s->timestep, s->seed, s->thermo_partial,
+                               s->stream),
+             "thermalize");
+        unsigned int np = 0;
+        hchk(launch_ke_partial(s->N, s->vel, s->ke_partial, &np, s->stream), "kinetic energy");
+        s->n_partial = np;
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        return HM_OK;
+    });
+}
+
+hm_stream_t hm_sys_stream(hm_system* s)
+{
+    return s ? (hm_stream_t)s->stream : nullptr;
+}
+
+} // extern "C"

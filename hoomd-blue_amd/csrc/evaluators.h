@@ -1,0 +1,144 @@
+// evaluators.h -- pair and bond evaluators (the PotentialPair / PotentialBond plugin concept).
+//
+// A pair evaluator has the shape the reference defines in hoomd/md/EvaluatorPairLJ.h:35-94:
+//   * a POD param_type built on the host from user parameters,
+//   * a constructor (rsq, rcutsq, const param_type&),
+//   * needsCharge() / setCharge(qi, qj),
+//   * bool evalForceAndEnergy(Real& force_divr, Real& energy, bool energy_shift),
+//   * getName().
+// The pair kernel template (pair_kernel.h) is instantiated once per evaluator, as
+// PotentialPairGPUKernel.cu.inc:19-21 does per evaluator name. Real is the arithmetic type
+// (float on the device hot path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace hm
+{
+
+template<typename Real> struct EvaluatorPairLJ
+{
+    // EvaluatorPairLJ::param_type (hoomd/md/EvaluatorPairLJ.h:99-155): (sigma^6, 4 epsilon)
+    struct param_type
+    {
+        Real sigma_6;
+        Real epsilon_x_4;
+
+        static param_type make(double sigma, double epsilon)
+        {
+            param_type p;
+            double s6 = sigma * sigma * sigma * sigma * sigma * sigma;
+            p.sigma_6 = Real(s6);
+            p.epsilon_x_4 = Real(4.0 * epsilon);
+            return p;
+        }
+    };
+
+    __host__ __device__ __forceinline__ EvaluatorPairLJ(Real _rsq, Real _rcutsq,
+                                                         const param_type& p)
+        : rsq(_rsq), rcutsq(_rcutsq), lj1(p.epsilon_x_4 * p.sigma_6 * p.sigma_6),
+          lj2(p.epsilon_x_4 * p.sigma_6)
+    {
+    }
+
+    __host__ __device__ static constexpr bool needsCharge()
+    {
+        return false;
+    }
+    __host__ __device__ void setCharge(Real, Real) { }
+
+    // EvaluatorPairLJ::evalForceAndEnergy (hoomd/md/EvaluatorPairLJ.h:190-211); the caller has
+    // already applied the exact (FP64) cutoff test, lj1 == 0 disables the pair.
+    __host__ __device__ __forceinline__ bool evalForceAndEnergy(Real& force_divr, Real& pair_eng,
+                                                                bool energy_shift) const
+    {
+        if (lj1 == Real(0))
+            return false;
+        Real r2inv = Real(1) / rsq;
+        Real r6inv = r2inv * r2inv * r2inv;
+        force_divr = r2inv * r6inv * (Real(12) * lj1 * r6inv - Real(6) * lj2);
+        pair_eng = r6inv * (lj1 * r6inv - lj2);
+        if (energy_shift)
+        {
+            Real rc2inv = Real(1) / rcutsq;
+            Real rc6inv = rc2inv * rc2inv * rc2inv;
+            pair_eng -= rc6inv * (lj1 * rc6inv - lj2);
+        }
+        return true;
+    }
+
+    static const char* getName()
+    {
+        return "lj";
+    }
+
+    Real rsq, rcutsq, lj1, lj2;
+};
+
+// FENE + WCA bond evaluator (hoomd/md/EvaluatorBondFENE.h:30-156).
+template<typename Real> struct EvaluatorBondFENE
+{
+    struct param_type
+    {
+        Real k, r_0, epsilon_x_4, sigma_6, delta;
+
+        static param_type make(double k, double r0, double epsilon, double sigma, double delta)
+        {
+            param_type p;
+            double s6 = sigma * sigma * sigma * sigma * sigma * sigma;
+            p.k = Real(k);
+            p.r_0 = Real(r0);
+            p.epsilon_x_4 = Real(4.0 * epsilon);
+            p.sigma_6 = Real(s6);
+            p.delta = Real(delta);
+            return p;
+        }
+    };
+
+    __host__ __device__ __forceinline__ EvaluatorBondFENE(Real _rsq, const param_type& p)
+        : rsq(_rsq), K(p.k), r_0(p.r_0), lj1(p.epsilon_x_4 * p.sigma_6 * p.sigma_6),
+          lj2(p.epsilon_x_4 * p.sigma_6), delta(p.delta)
+    {
+    }
+
+    __host__ __device__ __forceinline__ bool evalForceAndEnergy(Real& force_divr, Real& bond_eng)
+    {
+        Real r = sqrt(rsq);
+        Real rtemp = r - delta;
+        Real rmdoverr = rtemp / r;
+        Real rsq2 = rtemp * rtemp;
+        Real r2inv = Real(1) / rsq2;
+        Real r6inv = r2inv * r2inv * r2inv;
+        Real wca = Real(0);
+        Real pair_eng = Real(0);
+        Real sigma6inv = lj2 / lj1;
+        Real epsilon = lj2 * lj2 / Real(4) / lj1;
+        if (lj1 != Real(0) && r6inv > sigma6inv / Real(2))
+        {
+            wca = r2inv * r6inv * (Real(12) * lj1 * r6inv - Real(6) * lj2);
+            pair_eng = r6inv * (lj1 * r6inv - lj2) + epsilon;
+        }
+        force_divr = wca * rmdoverr;
+        bond_eng = pair_eng;
+        if (K != Real(0))
+        {
+            Real r0sq = r_0 * r_0;
+            if (rsq2 >= r0sq)
+                return false;
+            force_divr += -K / (Real(1) - rsq2 / r0sq) * rmdoverr;
+            bond_eng += -Real(0.5) * K * r0sq * log(Real(1) - rsq2 / r0sq);
+        }
+        return true;
+    }
+
+    static const char* getName()
+    {
+        return "fene";
+    }
+
+    Real rsq, K, r_0, lj1, lj2, delta;
+};
+
+} // namespace hm

@@ -1,0 +1,221 @@
+// kernels.h -- launch interfaces of the gfx950 kernels (used by the C-ABI drivers and the engine).
+#pragma once
+
+#include "hm_device.h"
+
+namespace hm
+{
+
+struct d3
+{
+    double x, y, z;
+};
+struct i3
+{
+    int x, y, z;
+};
+
+// Device-resident step state: every per-step decision lives here, so a whole chunk of steps can
+// be captured once in a hipGraph and replayed.
+struct StepState
+{
+    unsigned long long next;        // step that the next step-begin kernel starts
+    unsigned long long timestep;    // current step t (set by the step-begin kernel)
+    unsigned long long rebuild_at;  // neighbour list must be rebuilt before forces of this step
+    unsigned long long n_builds;    // rebuild counter
+    double alpha;                   // Bussi translational factor of the current step
+    unsigned int error;             // sticky error bits (see ERR_*)
+    unsigned int pad;
+};
+
+enum : unsigned int
+{
+    ERR_CELL_OVERFLOW = 1u,
+    ERR_NLIST_OVERFLOW = 2u,
+    ERR_NAN = 4u,
+    ERR_OUTSIDE = 8u,
+    ERR_BOND = 16u,
+    ERR_ZERO_KE = 32u,
+};
+
+// device-side conditions words (atomicMax'd)
+struct Conditions
+{
+    unsigned int cell_need;   // required cmax
+    unsigned int nan_idx;     // particle + 1
+    unsigned int out_idx;     // particle + 1
+    unsigned int nlist_need;  // required nmax
+    unsigned int bond_flag;
+    unsigned int pad[3];
+};
+
+struct PairLaunch
+{
+    unsigned int tpp;
+    unsigned int block;
+};
+
+// ---------------------------------------------------------------- cell list / nlist
+hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStream_t s);
+
+hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
+                             const unsigned int dim[3], unsigned int cmax,
+                             unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
+                             unsigned int* cond3, const StepState* gate, hipStream_t s);
+
+struct NlistKernelArgs
+{
+    unsigned int N;
+    DevBox box;
+    unsigned int ncell;
+    unsigned int cmax;
+    const unsigned int* cell_size;
+    const hm_cell_entry* entries;
+    const unsigned int* cell_adj;
+    unsigned int n_adj;
+    unsigned int ntypes;
+    double rlistsq[HM_MAX_TYPES * HM_MAX_TYPES];  // 0 for disabled pairs -> never included
+    unsigned int enabled[HM_MAX_TYPES * HM_MAX_TYPES];
+    const unsigned int* n_excl;
+    const unsigned int* excl;
+    unsigned int max_excl;
+    unsigned int nmax;
+    unsigned int* nlist;
+    unsigned int* n_neigh;
+    unsigned int* cond;       // needed nmax
+    const StepState* gate;
+    StepState* counter;       // n_builds incremented by block 0 when non-NULL
+};
+hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
+
+hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,
+                              const DevBox& box, double maxsq, unsigned int* flag, hipStream_t s);
+
+// ---------------------------------------------------------------- pair
+struct PairKernelArgs
+{
+    unsigned int N;
+    const double4* pos;
+    DevBox box;
+    const unsigned int* nlist;
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double rcutsq[HM_MAX_TYPES * HM_MAX_TYPES];
+    double ronsq[HM_MAX_TYPES * HM_MAX_TYPES];
+    float rcutsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float ronsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;
+    int compute_virial;
+    float4* force;
+    float* virial;
+};
+hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params,
+                          PairLaunch cfg, hipStream_t s);
+
+// ---------------------------------------------------------------- bond
+struct BondKernelArgs
+{
+    unsigned int N;
+    const double4* pos;
+    DevBox box;
+    const unsigned int* tag;        // index -> tag
+    const unsigned int* rtag;       // tag -> index
+    const unsigned int* row;        // [N+1] CSR by tag
+    const unsigned int* other_tag;  // partner tag
+    const unsigned int* btype;      // bond type (may be NULL)
+    unsigned int nbond_types;
+    int compute_virial;
+    float4* force;
+    float* virial;
+    unsigned int* flag;
+    StepState* st;                  // error bit on stretched bonds (may be NULL)
+};
+hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* params, hipStream_t s);
+
+// ---------------------------------------------------------------- integration
+struct StepOneArgs
+{
+    unsigned int N;
+    double4* pos;
+    double4* vel;
+    const d3* accel;
+    i3* image;
+    DevBox box;
+    double dt;
+    double alpha;              // used when st == NULL or !use_state_alpha
+    const StepState* st;
+    int use_state_alpha;
+    // fused distance check (NeighborList::distanceCheck for step t + 1)
+    const d3* last;
+    double maxsq;
+    StepState* flag_state;     // rebuild_at = t + 1 when a particle moved too far
+    // fused clearing of the cell occupancy counters
+    unsigned int* cell_size;
+    unsigned int ncell;
+};
+hipError_t launch_step_one(const StepOneArgs& a, hipStream_t s);
+
+struct StepTwoArgs
+{
+    unsigned int N;
+    double4* vel;
+    d3* accel;
+    const float4* f[4];
+    unsigned int nforces;
+    double dt;
+    double alpha;
+    // Langevin (gamma_by_type != NULL)
+    const double* gamma_by_type;
+    const double4* pos;
+    const unsigned int* tag;
+    double kT;
+    uint16_t seed;
+    uint64_t timestep;       // used when st == NULL
+    const StepState* st;
+    // FP64 kinetic energy partial sums per block (may be NULL)
+    double* ke_partial;
+};
+hipError_t launch_step_two(const StepTwoArgs& a, hipStream_t s, unsigned int* nblocks_out);
+
+// step-begin kernel: advances the device timestep; Bussi factor from KE partial sums
+hipError_t launch_step_begin(StepState* st, const double* ke_partial, unsigned int npartial,
+                             int bussi, double ndof, double dt, double tau, double kT,
+                             uint16_t seed, hipStream_t s);
+hipError_t launch_set_rebuild(StepState* st, hipStream_t s);  // rebuild before next forces
+
+// initial accelerations a = F/m (IntegratorTwoStep::prepRun -> computeAccelerations)
+hipError_t launch_accel_init(unsigned int N, const double4* vel, d3* accel, const float4* const* f,
+                             unsigned int nforces, hipStream_t s);
+
+// thermo sums: out[0] KE, out[1] PE, out[2] W, out[3..8] virial tensor; partial scratch of
+// THERMO_BLOCKS * 9 doubles
+constexpr unsigned int THERMO_BLOCKS = 512;
+hipError_t launch_thermo(unsigned int N, const double4* vel, const float4* const* f,
+                         const float* const* vir, unsigned int nforces, double* partial,
+                         double* out, hipStream_t s);
+hipError_t launch_ke_partial(unsigned int N, const double4* vel, double* partial,
+                             unsigned int* nblocks_out, hipStream_t s);
+
+hipError_t launch_thermalize(unsigned int N, double4* vel, const unsigned int* tag, double kT,
+                             uint64_t timestep, uint16_t seed, double* scratch3, hipStream_t s);
+
+// ---------------------------------------------------------------- Hilbert sort
+hipError_t launch_sfc_keys(unsigned int N, const double4* pos, const DevBox& box,
+                           unsigned int grid, unsigned long long* keys, hipStream_t s);
+hipError_t sort_keys64(unsigned long long* keys_in, unsigned long long* keys_out, unsigned int N,
+                       int end_bit, void* tmp, size_t* tmp_bytes, hipStream_t s);
+hipError_t launch_keys_to_order(unsigned int N, const unsigned long long* keys,
+                                unsigned int* order, hipStream_t s);
+hipError_t launch_traversal(unsigned int grid, unsigned int* traversal, hipStream_t s);
+
+// gather helpers for applying a sort order
+hipError_t gather_bytes(const void* src, void* dst, const unsigned int* order, unsigned int N,
+                        unsigned int elem_bytes, hipStream_t s);
+hipError_t launch_rtag(unsigned int N, const unsigned int* tag, unsigned int* rtag, hipStream_t s);
+hipError_t launch_excl_index(unsigned int N, const unsigned int* tag, const unsigned int* rtag,
+                             const unsigned int* n_excl_tag, const unsigned int* excl_tag,
+                             unsigned int max_excl, unsigned int* n_excl, unsigned int* excl,
+                             hipStream_t s);
+
+} // namespace hm

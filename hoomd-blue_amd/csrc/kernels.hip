@@ -1,0 +1,1249 @@
+// kernels.hip -- gfx950 kernels of the short-range MD timestep.
+//
+// Positions, velocities and the integrator are FP64; neighbour-list inclusion and the pair
+// cutoff test are exact FP64 comparisons; pair and bond arithmetic is FP32 with FP32
+// per-particle sums. Wavefronts are 64 lanes; particle-parallel gathers use an XCD-aware block
+// remap so spatially close (Hilbert-sorted) particles share an XCD's L2.
+#include "evaluators.h"
+#include "kernels.h"
+
+namespace hm
+{
+
+// ======================================================================== cell adjacency
+// CellList::initializeCellAdj (hoomd/CellList.cc:346-402): the 27 wrapped neighbours of each
+// cell, sorted, duplicates removed (fewer than 3 cells along an axis).
+__global__ void cell_adj_kernel(unsigned int dx, unsigned int dy, unsigned int dz,
+                                unsigned int n_adj, unsigned int* adj)
+{
+    unsigned int c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int ncell = dx * dy * dz;
+    if (c >= ncell)
+        return;
+    int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
+    unsigned int tmp[27];
+    int n = 0;
+    for (int nk = k - 1; nk <= k + 1; nk++)
+        for (int nj = j - 1; nj <= j + 1; nj++)
+            for (int ni = i - 1; ni <= i + 1; ni++)
+            {
+                int wi = ((ni % (int)dx) + (int)dx) % (int)dx;
+                int wj = ((nj % (int)dy) + (int)dy) % (int)dy;
+                int wk = ((nk % (int)dz) + (int)dz) % (int)dz;
+                tmp[n++] = (unsigned int)((wk * (int)dy + wj) * (int)dx + wi);
+            }
+    for (int a = 1; a < 27; a++) // insertion sort
+    {
+        unsigned int v = tmp[a];
+        int b = a - 1;
+        while (b >= 0 && tmp[b] > v)
+        {
+            tmp[b + 1] = tmp[b];
+            b--;
+        }
+        tmp[b + 1] = v;
+    }
+    unsigned int m = 0;
+    for (int a = 0; a < 27; a++)
+        if (a == 0 || tmp[a] != tmp[a - 1])
+        {
+            if (m < n_adj)
+                adj[(size_t)c * n_adj + m] = tmp[a];
+            m++;
+        }
+}
+
+hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStream_t s)
+{
+    unsigned int ncell = dim[0] * dim[1] * dim[2];
+    unsigned int n_adj = 1;
+    for (int d = 0; d < 3; d++)
+        n_adj *= dim[d] > 3 ? 3 : dim[d];
+    hipLaunchKernelGGL(cell_adj_kernel, dim3((ncell + 255) / 256), dim3(256), 0, s, dim[0],
+                       dim[1], dim[2], n_adj, adj);
+    return hipGetLastError();
+}
+
+// ======================================================================== cell build
+// CellList::computeCellList (hoomd/CellList.cc:404-547) / gpu_compute_cell_list_kernel
+// (hoomd/CellListGPU.cu:42-171): FP64 makeFraction, truncation to the cell grid, wrap at dim,
+// atomic slot, conditions (overflow, NaN, out of box).
+__global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const double4* __restrict__ pos,
+                                                         DevBox box, unsigned int dx, unsigned int dy,
+                                                         unsigned int dz, unsigned int cmax,
+                                                         unsigned int* cell_size,
+                                                         hm_cell_entry* entries, d3* last,
+                                                         unsigned int* cond, const StepState* gate)
+{
+#pragma clang fp contract(off)
+    if (gate && gate->rebuild_at != gate->timestep + 1)
+        return;
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    double4 p = pos[n];
+    if (last)
+        last[n] = d3{p.x, p.y, p.z};
+    if (isnan(p.x) || isnan(p.y) || isnan(p.z))
+    {
+        atomicMax(&cond[1], n + 1);
+        return;
+    }
+    double fx = (p.x - box.lo[0]) * box.Linv[0];
+    double fy = (p.y - box.lo[1]) * box.Linv[1];
+    double fz = (p.z - box.lo[2]) * box.Linv[2];
+    int ib = (int)(fx * (double)dx);
+    int jb = (int)(fy * (double)dy);
+    int kb = (int)(fz * (double)dz);
+    if (fx < -0.00001 || fx >= 1.00001 || fy < -0.00001 || fy >= 1.00001 || fz < -0.00001
+        || fz >= 1.00001)
+    {
+        atomicMax(&cond[2], n + 1);
+        return;
+    }
+    if (ib == (int)dx)
+        ib = 0;
+    if (jb == (int)dy)
+        jb = 0;
+    if (kb == (int)dz)
+        kb = 0;
+    if (ib < 0 || ib >= (int)dx || jb < 0 || jb >= (int)dy || kb < 0 || kb >= (int)dz)
+    {
+        atomicMax(&cond[2], n + 1);
+        return;
+    }
+    unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
+    unsigned int slot = atomicAdd(&cell_size[bin], 1u);
+    if (slot < cmax)
+    {
+        hm_cell_entry e;
+        e.x = p.x;
+        e.y = p.y;
+        e.z = p.z;
+        e.idx = n;
+        e.type = (uint32_t)p.w;
+        entries[(size_t)bin * cmax + slot] = e;
+    }
+    else
+        atomicMax(&cond[0], slot + 1);
+}
+
+hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
+                             const unsigned int dim[3], unsigned int cmax,
+                             unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
+                             unsigned int* cond3, const StepState* gate, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(cell_build_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, box,
+                       dim[0], dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
+    return hipGetLastError();
+}
+
+// ======================================================================== neighbour list
+// NeighborListBinned::buildNlist (hoomd/md/NeighborListBinned.cc:36-179) in full storage,
+// restructured for CDNA: one workgroup per home cell stages the adjacent cells' particles in
+// LDS (each cell ordered by particle index), one wavefront per home particle sweeps the staged
+// candidates 64 at a time, and ballot + mbcnt compaction writes the hits contiguously into the
+// particle's row. Inclusion is dr^2 <= (r_cut + r_buff)^2 in FP64 with the same operation order
+// as the CPU path, so the rows equal the CPU full list entry for entry.
+constexpr int NLB = 256;
+constexpr int NLCAP = 1024;
+constexpr int NLHOME = 256;
+
+template<bool MT, bool EXCL>
+__global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double sx[NLCAP], sy[NLCAP], sz[NLCAP];
+    __shared__ unsigned int sj[NLCAP];
+    __shared__ unsigned int sraw[NLCAP];
+    __shared__ unsigned char stp[MT ? NLCAP : 1];
+    __shared__ unsigned int acell[27], asize[27], aoff[27];
+    __shared__ unsigned int hcnt[NLHOME];
+    __shared__ unsigned int pass_last, pass_total;
+
+    if (a.gate && a.gate->rebuild_at != a.gate->timestep + 1)
+        return;
+    const unsigned int cell = xcd_block(blockIdx.x, gridDim.x);
+    if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
+        a.counter->n_builds++;
+    if (cell >= a.ncell)
+        return;
+    const unsigned int nhome = min(a.cell_size[cell], a.cmax);
+    if (nhome == 0)
+        return;
+    const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid < a.n_adj)
+    {
+        unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
+        acell[tid] = c;
+        asize[tid] = min(a.cell_size[c], a.cmax);
+    }
+    for (unsigned int s = tid; s < nhome; s += NLB)
+        hcnt[s] = 0;
+    __syncthreads();
+
+    unsigned int first = 0;
+    while (first < a.n_adj)
+    {
+        if (tid == 0)
+        {
+            unsigned int tot = 0, l = first;
+            while (l < a.n_adj && tot + asize[l] <= NLCAP)
+            {
+                aoff[l] = tot;
+                tot += asize[l];
+                l++;
+            }
+            pass_last = l;
+            pass_total = tot;
+        }
+        __syncthreads();
+        const unsigned int last = pass_last, total = pass_total;
+
+        // stage candidates: raw load, then rank by particle index inside each cell
+        hm_cell_entry e[NLCAP / NLB];
+        unsigned int segb[NLCAP / NLB], segn[NLCAP / NLB];
+#pragma unroll
+        for (int r = 0; r < NLCAP / NLB; r++)
+        {
+            unsigned int k = tid + r * NLB;
+            segn[r] = 0;
+            if (k < total)
+            {
+                unsigned int q = first;
+                while (q + 1 < last && aoff[q + 1] <= k)
+                    q++;
+                e[r] = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
+                segb[r] = aoff[q];
+                segn[r] = asize[q];
+                sraw[k] = e[r].idx;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NLCAP / NLB; r++)
+        {
+            unsigned int k = tid + r * NLB;
+            if (k < total)
+            {
+                unsigned int rank = 0;
+                for (unsigned int m = 0; m < segn[r]; m++)
+                    rank += sraw[segb[r] + m] < e[r].idx ? 1u : 0u;
+                unsigned int p = segb[r] + rank;
+                sx[p] = e[r].x;
+                sy[p] = e[r].y;
+                sz[p] = e[r].z;
+                sj[p] = e[r].idx;
+                if (MT)
+                    stp[p] = (unsigned char)e[r].type;
+            }
+        }
+        __syncthreads();
+
+        // one wavefront per home particle
+        for (unsigned int s = wave; s < nhome; s += NLB / 64)
+        {
+            const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
+            const unsigned int i = h.idx;
+            const unsigned int ti = MT ? h.type : 0u;
+            unsigned int ex0 = 0xffffffffu, ex1 = 0xffffffffu, ex2 = 0xffffffffu,
+                         ex3 = 0xffffffffu;
+            if (EXCL)
+            {
+                unsigned int ne = min(a.n_excl[i], a.max_excl);
+                const unsigned int* er = a.excl + (size_t)i * a.max_excl;
+                if (ne > 0)
+                    ex0 = er[0];
+                if (ne > 1)
+                    ex1 = er[1];
+                if (ne > 2)
+                    ex2 = er[2];
+                if (ne > 3)
+                    ex3 = er[3];
+            }
+            unsigned int cnt = hcnt[s];
+            unsigned int* row = a.nlist + (size_t)i * a.nmax;
+            for (unsigned int base = 0; base < total; base += 64)
+            {
+                const unsigned int k = base + lane;
+                bool hit = false;
+                unsigned int j = 0;
+                if (k < total)
+                {
+                    j = sj[k];
+                    double dx = h.x - sx[k];
+                    double dy = h.y - sy[k];
+                    double dz = h.z - sz[k];
+                    min_image(a.box, dx, dy, dz);
+                    double dr2 = dx * dx + dy * dy + dz * dz;
+                    unsigned int tp = MT ? ti * a.ntypes + stp[k] : 0u;
+                    hit = (j != i) && a.enabled[tp] && (dr2 <= a.rlistsq[tp]);
+                    if (EXCL)
+                        hit = hit && j != ex0 && j != ex1 && j != ex2 && j != ex3;
+                }
+                const uint64_t m = __ballot(hit);
+                if (hit)
+                {
+                    unsigned int p = cnt + lanes_below(m);
+                    if (p < a.nmax)
+                        row[p] = j;
+                }
+                cnt += (unsigned int)__popcll(m);
+            }
+            if (lane == 0)
+                hcnt[s] = cnt;
+        }
+        __syncthreads();
+        first = last;
+    }
+    for (unsigned int s = tid; s < nhome; s += NLB)
+    {
+        unsigned int i = a.entries[(size_t)cell * a.cmax + s].idx;
+        unsigned int c = hcnt[s];
+        a.n_neigh[i] = c;
+        if (c > a.nmax)
+            atomicMax(a.cond, c);
+    }
+}
+
+hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s)
+{
+    if (a.ncell == 0)
+        return hipSuccess;
+    unsigned int grid = round_up(a.ncell, 8);
+    bool mt = a.ntypes > 1;
+    bool ex = a.n_excl != nullptr && a.max_excl > 0;
+    if (mt && ex)
+        hipLaunchKernelGGL((nlist_kernel<true, true>), dim3(grid), dim3(NLB), 0, s, a);
+    else if (mt)
+        hipLaunchKernelGGL((nlist_kernel<true, false>), dim3(grid), dim3(NLB), 0, s, a);
+    else if (ex)
+        hipLaunchKernelGGL((nlist_kernel<false, true>), dim3(grid), dim3(NLB), 0, s, a);
+    else
+        hipLaunchKernelGGL((nlist_kernel<false, false>), dim3(grid), dim3(NLB), 0, s, a);
+    return hipGetLastError();
+}
+
+// NeighborList::distanceCheck (hoomd/md/NeighborList.cc:1220-1289), standalone form.
+__global__ void nlist_check_kernel(unsigned int N, const double4* __restrict__ pos,
+                                   const d3* __restrict__ last, DevBox box, double maxsq,
+                                   unsigned int* flag)
+{
+#pragma clang fp contract(off)
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (n < N)
+    {
+        double4 p = pos[n];
+        d3 l = last[n];
+        double dx = p.x - 1.0 * l.x, dy = p.y - 1.0 * l.y, dz = p.z - 1.0 * l.z;
+        min_image(box, dx, dy, dz);
+        moved = dx * dx + dy * dy + dz * dz >= maxsq;
+    }
+    if (__ballot(moved) != 0 && (threadIdx.x & 63u) == 0)
+        atomicOr(flag, 1u);
+}
+
+hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,
+                              const DevBox& box, double maxsq, unsigned int* flag, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(nlist_check_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, last,
+                       box, maxsq, flag);
+    return hipGetLastError();
+}
+
+// ======================================================================== pair forces
+// PotentialPair::computeForces (hoomd/md/PotentialPair.h:587-794) with a full list, the
+// structure of gpu_compute_pair_forces_shared_kernel (hoomd/md/PotentialPairGPU.cuh:140-404):
+// TPP lanes per particle stride the particle's neighbour row four entries (16 B) at a time;
+// the partial sums are combined by xor-shuffles inside the TPP group.
+template<class Eval> struct ParamTable
+{
+    typename Eval::param_type p[HM_MAX_TYPES * HM_MAX_TYPES];
+};
+
+template<class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__global__ __launch_bounds__(1024) void pair_kernel(PairKernelArgs a, ParamTable<Eval> tab)
+{
+    const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned int gt = lb * blockDim.x + threadIdx.x;
+    const unsigned int i = gt / TPP;
+    const unsigned int sub = gt % TPP;
+    const bool active = i < a.N;
+    const unsigned int ii = active ? i : a.N - 1;
+    const double4 pi = a.pos[ii];
+    // a row never holds more than nmax entries (n_neigh reports the full count on overflow)
+    const unsigned int n = active ? min(a.n_neigh[ii], a.nmax) : 0u;
+    const unsigned int ti = MT ? (unsigned int)pi.w : 0u;
+    const unsigned int jmax = a.N - 1;
+
+    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+    float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
+    const unsigned int* row = a.nlist + (size_t)ii * a.nmax;
+
+    for (unsigned int k0 = sub * 4; k0 < n; k0 += TPP * 4)
+    {
+        const uint4 jj = *reinterpret_cast<const uint4*>(row + k0);
+        const unsigned int js[4] = {jj.x, jj.y, jj.z, jj.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+        {
+            if (k0 + q < n)
+            {
+                const double4 pj = a.pos[min(js[q], jmax)];
+                double dx, dy, dz, rsq;
+                {
+#pragma clang fp contract(off)
+                    dx = pi.x - pj.x;
+                    dy = pi.y - pj.y;
+                    dz = pi.z - pj.z;
+                    min_image(a.box, dx, dy, dz);
+                    rsq = dx * dx + dy * dy + dz * dz;
+                }
+                const unsigned int tp = MT ? ti * a.ntypes + (unsigned int)pj.w : 0u;
+                if (rsq < a.rcutsq[tp])
+                {
+                    const float rsqf = (float)rsq;
+                    const float rcf = a.rcutsq_f[tp];
+                    Eval ev(rsqf, rcf, tab.p[tp]);
+                    float fdr = 0.f, e = 0.f;
+                    bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
+                    if (ev.evalForceAndEnergy(fdr, e, shift))
+                    {
+                        if (SHIFT == 2)
+                        {
+                            // XPLOR smoothing (PotentialPair.h:706-731)
+                            const float ron = a.ronsq_f[tp];
+                            if (rsqf >= ron && rsqf < rcf)
+                            {
+                                float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
+                                float rmc = rsqf - rcf;
+                                float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
+                                float ds = 12.f * (rsqf - ron) * rmc * den;
+                                float e0 = e;
+                                e = e0 * sw;
+                                fdr = sw * fdr - ds * e0;
+                            }
+                        }
+                        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+                        fx += dxf * fdr;
+                        fy += dyf * fdr;
+                        fz += dzf * fdr;
+                        pe += 0.5f * e;
+                        if (VIRIAL)
+                        {
+                            const float f2 = 0.5f * fdr;
+                            vxx += f2 * dxf * dxf;
+                            vxy += f2 * dxf * dyf;
+                            vxz += f2 * dxf * dzf;
+                            vyy += f2 * dyf * dyf;
+                            vyz += f2 * dyf * dzf;
+                            vzz += f2 * dzf * dzf;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (TPP > 1)
+    {
+#pragma unroll
+        for (int off = TPP / 2; off > 0; off >>= 1)
+        {
+            fx += __shfl_xor(fx, off, TPP);
+            fy += __shfl_xor(fy, off, TPP);
+            fz += __shfl_xor(fz, off, TPP);
+            pe += __shfl_xor(pe, off, TPP);
+            if (VIRIAL)
+            {
+                vxx += __shfl_xor(vxx, off, TPP);
+                vxy += __shfl_xor(vxy, off, TPP);
+                vxz += __shfl_xor(vxz, off, TPP);
+                vyy += __shfl_xor(vyy, off, TPP);
+                vyz += __shfl_xor(vyz, off, TPP);
+                vzz += __shfl_xor(vzz, off, TPP);
+            }
+        }
+    }
+    if (active && sub == 0)
+    {
+        a.force[i] = make_float4(fx, fy, fz, pe);
+        if (VIRIAL)
+        {
+            const size_t P = a.N;
+            a.virial[0 * P + i] = vxx;
+            a.virial[1 * P + i] = vxy;
+            a.virial[2 * P + i] = vxz;
+            a.virial[3 * P + i] = vyy;
+            a.virial[4 * P + i] = vyz;
+            a.virial[5 * P + i] = vzz;
+        }
+    }
+}
+
+template<class Eval, int TPP, bool VIRIAL, bool MT>
+static hipError_t pair_dispatch_shift(const PairKernelArgs& a, const ParamTable<Eval>& tab,
+                                      dim3 grid, dim3 block, hipStream_t s)
+{
+    switch (a.shift_mode)
+    {
+    case 0:
+        hipLaunchKernelGGL((pair_kernel<Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a, tab);
+        break;
+    case 1:
+        hipLaunchKernelGGL((pair_kernel<Eval, TPP, VIRIAL, MT, 1>), grid, block, 0, s, a, tab);
+        break;
+    default:
+        hipLaunchKernelGGL((pair_kernel<Eval, TPP, VIRIAL, MT, 2>), grid, block, 0, s, a, tab);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template<class Eval, int TPP>
+static hipError_t pair_dispatch_tpp(const PairKernelArgs& a, const ParamTable<Eval>& tab,
+                                    unsigned int block, hipStream_t s)
+{
+    unsigned int threads = a.N * TPP;
+    unsigned int nblocks = round_up((threads + block - 1) / block, 8);
+    dim3 g(nblocks), b(block);
+    bool mt = a.ntypes > 1;
+    if (a.compute_virial)
+        return mt ? pair_dispatch_shift<Eval, TPP, true, true>(a, tab, g, b, s)
+                  : pair_dispatch_shift<Eval, TPP, true, false>(a, tab, g, b, s);
+    return mt ? pair_dispatch_shift<Eval, TPP, false, true>(a, tab, g, b, s)
+              : pair_dispatch_shift<Eval, TPP, false, false>(a, tab, g, b, s);
+}
+
+template<class Eval>
+static hipError_t pair_dispatch(const PairKernelArgs& a, const ParamTable<Eval>& tab,
+                                PairLaunch cfg, hipStream_t s)
+{
+    if (a.N == 0)
+        return hipSuccess;
+    unsigned int block = cfg.block ? cfg.block : 256;
+    switch (cfg.tpp ? cfg.tpp : 4)
+    {
+    case 1:
+        return pair_dispatch_tpp<Eval, 1>(a, tab, block, s);
+    case 2:
+        return pair_dispatch_tpp<Eval, 2>(a, tab, block, s);
+    case 8:
+        return pair_dispatch_tpp<Eval, 8>(a, tab, block, s);
+    case 16:
+        return pair_dispatch_tpp<Eval, 16>(a, tab, block, s);
+    default:
+        return pair_dispatch_tpp<Eval, 4>(a, tab, block, s);
+    }
+}
+
+hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params, PairLaunch cfg,
+                          hipStream_t s)
+{
+    using E = EvaluatorPairLJ<float>;
+    ParamTable<E> tab;
+    unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+    return pair_dispatch<E>(a, tab, cfg, s);
+}
+
+// ======================================================================== FENE bonds
+// PotentialBond::computeForces (hoomd/md/PotentialBond.h:167-315) in the per-particle form of
+// gpu_compute_bond_forces_kernel (hoomd/md/PotentialBondGPU.cuh:98-239). Bonds are held by tag
+// (CSR), so a Hilbert sort only changes the tag->index map.
+template<class Eval> struct BondTable
+{
+    typename Eval::param_type p[HM_MAX_TYPES];
+};
+
+__global__ __launch_bounds__(256) void bond_fene_kernel(BondKernelArgs a,
+                                                        BondTable<EvaluatorBondFENE<float>> tab)
+{
+    unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+    const unsigned int tg = a.tag[i];
+    const unsigned int b0 = a.row[tg], b1 = a.row[tg + 1];
+    const double4 pi = a.pos[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool bad = false;
+    for (unsigned int b = b0; b < b1; b++)
+    {
+        const unsigned int j = a.rtag[a.other_tag[b]];
+        const double4 pj = a.pos[j];
+        double dx, dy, dz, rsq;
+        {
+#pragma clang fp contract(off)
+            dx = pi.x - pj.x;
+            dy = pi.y - pj.y;
+            dz = pi.z - pj.z;
+            min_image(a.box, dx, dy, dz);
+            rsq = dx * dx + dy * dy + dz * dz;
+        }
+        const unsigned int bt = a.btype ? a.btype[b] : 0u;
+        EvaluatorBondFENE<float> ev((float)rsq, tab.p[bt]);
+        float fdr = 0.f, e = 0.f;
+        if (!ev.evalForceAndEnergy(fdr, e))
+        {
+            bad = true;
+            continue;
+        }
+        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+        fx += fdr * dxf;
+        fy += fdr * dyf;
+        fz += fdr * dzf;
+        pe += 0.5f * e;
+        if (a.compute_virial)
+        {
+            float f2 = 0.5f * fdr;
+            v[0] += dxf * dxf * f2;
+            v[1] += dxf * dyf * f2;
+            v[2] += dxf * dzf * f2;
+            v[3] += dyf * dyf * f2;
+            v[4] += dyf * dzf * f2;
+            v[5] += dzf * dzf * f2;
+        }
+    }
+    if (bad)
+    {
+        if (a.flag)
+            atomicOr(a.flag, 1u);
+        if (a.st)
+            atomicOr(&a.st->error, (unsigned int)ERR_BOND);
+    }
+    a.force[i] = make_float4(fx, fy, fz, pe);
+    if (a.compute_virial)
+        for (int m = 0; m < 6; m++)
+            a.virial[(size_t)m * a.N + i] = v[m];
+}
+
+hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* params, hipStream_t s)
+{
+    if (a.N == 0)
+        return hipSuccess;
+    using E = EvaluatorBondFENE<float>;
+    BondTable<E> tab;
+    unsigned int nt = a.nbond_types ? a.nbond_types : 1;
+    for (unsigned int q = 0; q < nt; q++)
+        tab.p[q] = E::param_type::make(params[q].k, params[q].r0, params[q].epsilon,
+                                       params[q].sigma, params[q].delta);
+    hipLaunchKernelGGL(bond_fene_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a, tab);
+    return hipGetLastError();
+}
+
+// ======================================================================== integration
+// TwoStepConstantVolume::integrateStepOne (hoomd/md/TwoStepConstantVolume.cc:7-80), fused with
+// NeighborList::distanceCheck for step t + 1 (NeighborList.cc:1220-1289) and with clearing the
+// cell occupancy counters for a possible rebuild.
+__global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
+{
+#pragma clang fp contract(off)
+    const unsigned int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.cell_size)
+        for (unsigned int c = gtid; c < a.ncell; c += gridDim.x * blockDim.x)
+            a.cell_size[c] = 0;
+    const unsigned int n = gtid;
+    bool moved = false;
+    if (n < a.N)
+    {
+        const double alpha = (a.st && a.use_state_alpha) ? a.st->alpha : a.alpha;
+        const double dt = a.dt;
+        double4 v = a.vel[n];
+        double4 p = a.pos[n];
+        const d3 acc = a.accel[n];
+        i3 img = a.image[n];
+        double vx = v.x + (1.0 / 2.0) * acc.x * dt;
+        double vy = v.y + (1.0 / 2.0) * acc.y * dt;
+        double vz = v.z + (1.0 / 2.0) * acc.z * dt;
+        vx *= alpha;
+        vy *= alpha;
+        vz *= alpha;
+        double px = p.x + dt * vx;
+        double py = p.y + dt * vy;
+        double pz = p.z + dt * vz;
+        wrap_axis(px, img.x, a.box.lo[0], a.box.hi[0], a.box.L[0]);
+        wrap_axis(py, img.y, a.box.lo[1], a.box.hi[1], a.box.L[1]);
+        wrap_axis(pz, img.z, a.box.lo[2], a.box.hi[2], a.box.L[2]);
+        v.x = vx;
+        v.y = vy;
+        v.z = vz;
+        p.x = px;
+        p.y = py;
+        p.z = pz;
+        a.vel[n] = v;
+        a.pos[n] = p;
+        a.image[n] = img;
+        if (a.last)
+        {
+            const d3 l = a.last[n];
+            double dx = px - 1.0 * l.x, dy = py - 1.0 * l.y, dz = pz - 1.0 * l.z;
+            min_image(a.box, dx, dy, dz);
+            moved = dx * dx + dy * dy + dz * dz >= a.maxsq;
+        }
+    }
+    if (a.flag_state && __ballot(moved) != 0 && (threadIdx.x & 63u) == 0)
+        atomicMax(&a.flag_state->rebuild_at, a.flag_state->timestep + 1);
+}
+
+hipError_t launch_step_one(const StepOneArgs& a, hipStream_t s)
+{
+    unsigned int work = a.N > a.ncell ? a.N : a.ncell;
+    if (work == 0)
+        return hipSuccess;
+    unsigned int blocks = (a.N + 255) / 256;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(step_one_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// FP64 block sum over 256 threads in a fixed order (deterministic)
+__device__ __forceinline__ double block_sum_256(double v, double* sh)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    const unsigned int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0)
+        sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    return t;
+}
+
+// TwoStepConstantVolume::integrateStepTwo (TwoStepConstantVolume.cc:213-260) or
+// TwoStepLangevin::integrateStepTwo (hoomd/md/TwoStepLangevin.cc:206-275), with the net force
+// summed from up to four force arrays and the FP64 kinetic-energy partial of each block.
+template<bool LANGEVIN>
+__global__ __launch_bounds__(256) void step_two_kernel(StepTwoArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    const unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (n < a.N)
+    {
+        double4 v = a.vel[n];
+        const double m = v.w;
+        const double minv = 1.0 / m;
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        for (unsigned int q = 0; q < a.nforces; q++)
+        {
+            const float4 f = a.f[q][n];
+            Fx += (double)f.x;
+            Fy += (double)f.y;
+            Fz += (double)f.z;
+        }
+        double ax, ay, az;
+        if (LANGEVIN)
+        {
+            const uint64_t t = a.st ? a.st->timestep : a.timestep;
+            Philox g(25, t, a.seed, a.tag[n]);
+            const double rx = uniform(g, -1.0, 1.0);
+            const double ry = uniform(g, -1.0, 1.0);
+            const double rz = uniform(g, -1.0, 1.0);
+            const double gm = a.gamma_by_type[(unsigned int)a.pos[n].w];
+            const double coeff = sqrt(6.0 * gm * a.kT / a.dt);
+            const double bx = rx * coeff - gm * v.x;
+            const double by = ry * coeff - gm * v.y;
+            const double bz = rz * coeff - gm * v.z;
+            ax = (Fx + bx) * minv;
+            ay = (Fy + by) * minv;
+            az = (Fz + bz) * minv;
+            v.x += (1.0 / 2.0) * ax * a.dt;
+            v.y += (1.0 / 2.0) * ay * a.dt;
+            v.z += (1.0 / 2.0) * az * a.dt;
+        }
+        else
+        {
+            ax = Fx * minv;
+            ay = Fy * minv;
+            az = Fz * minv;
+            v.x *= a.alpha;
+            v.y *= a.alpha;
+            v.z *= a.alpha;
+            v.x += (1.0 / 2.0) * a.dt * ax;
+            v.y += (1.0 / 2.0) * a.dt * ay;
+            v.z += (1.0 / 2.0) * a.dt * az;
+        }
+        a.vel[n] = v;
+        a.accel[n] = d3{ax, ay, az};
+        ke = m * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    if (a.ke_partial)
+    {
+        double t = block_sum_256(ke, sh);
+        if (threadIdx.x == 0)
+            a.ke_partial[blockIdx.x] = t;
+    }
+}
+
+hipError_t launch_step_two(const StepTwoArgs& a, hipStream_t s, unsigned int* nblocks_out)
+{
+    unsigned int blocks = (a.N + 255) / 256;
+    if (nblocks_out)
+        *nblocks_out = blocks;
+    if (blocks == 0)
+        return hipSuccess;
+    if (a.gamma_by_type)
+        hipLaunchKernelGGL(step_two_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(step_two_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void ke_partial_kernel(unsigned int N, const double4* vel,
+                                                         double* partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    const unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (n < N)
+    {
+        double4 v = vel[n];
+        ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    double t = block_sum_256(ke, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = t;
+}
+
+hipError_t launch_ke_partial(unsigned int N, const double4* vel, double* partial,
+                             unsigned int* nblocks_out, hipStream_t s)
+{
+    unsigned int blocks = (N + 255) / 256;
+    *nblocks_out = blocks;
+    if (blocks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(ke_partial_kernel, dim3(blocks), dim3(256), 0, s, N, vel, partial);
+    return hipGetLastError();
+}
+
+// Step-begin: advance the device timestep and draw the Bussi factor from the kinetic energy
+// of the current velocities (BussiThermostat::getRescalingFactorsOne, Thermostat.h:357-391).
+__global__ __launch_bounds__(256) void step_begin_kernel(StepState* st, const double* ke_partial,
+                                                         unsigned int npartial, int bussi,
+                                                         double ndof, double dt, double tau,
+                                                         double kT, uint16_t seed)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    double acc = 0.0;
+    if (bussi)
+        for (unsigned int q = threadIdx.x; q < npartial; q += 256)
+            acc += ke_partial[q];
+    double tot = block_sum_256(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        const unsigned long long t = st->next;
+        st->timestep = t;
+        st->next = t + 1;
+        double alpha = 1.0;
+        if (bussi)
+        {
+            const double K = tot * 0.5;
+            if (K == 0.0)
+                st->error |= ERR_ZERO_KE;
+            else
+                alpha = bussi_alpha(K, ndof, dt, tau, kT, t, seed, 0u);
+        }
+        st->alpha = alpha;
+    }
+}
+
+hipError_t launch_step_begin(StepState* st, const double* ke_partial, unsigned int npartial,
+                             int bussi, double ndof, double dt, double tau, double kT,
+                             uint16_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(256), 0, s, st, ke_partial, npartial,
+                       bussi, ndof, dt, tau, kT, seed);
+    return hipGetLastError();
+}
+
+__global__ void set_rebuild_kernel(StepState* st)
+{
+    st->rebuild_at = st->next + 1;
+}
+
+hipError_t launch_set_rebuild(StepState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, s, st);
+    return hipGetLastError();
+}
+
+// Integrator::computeAccelerations: a = F_net / m
+__global__ void accel_init_kernel(unsigned int N, const double4* vel, d3* accel,
+                                  const float4* f0, const float4* f1, const float4* f2,
+                                  const float4* f3, unsigned int nforces)
+{
+#pragma clang fp contract(off)
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    const float4* fs[4] = {f0, f1, f2, f3};
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    for (unsigned int q = 0; q < nforces; q++)
+    {
+        float4 f = fs[q][n];
+        Fx += (double)f.x;
+        Fy += (double)f.y;
+        Fz += (double)f.z;
+    }
+    const double minv = 1.0 / vel[n].w;
+    accel[n] = d3{Fx * minv, Fy * minv, Fz * minv};
+}
+
+hipError_t launch_accel_init(unsigned int N, const double4* vel, d3* accel, const float4* const* f,
+                             unsigned int nforces, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    const float4* fs[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (unsigned int q = 0; q < nforces && q < 4; q++)
+        fs[q] = f[q];
+    hipLaunchKernelGGL(accel_init_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, vel, accel,
+                       fs[0], fs[1], fs[2], fs[3], nforces);
+    return hipGetLastError();
+}
+
+// ======================================================================== thermo
+// ComputeThermo::computeProperties (hoomd/md/ComputeThermo.cc:76-292) sums in FP64; two passes
+// with a fixed block count so the summation order does not depend on timing.
+struct ForcePtrs
+{
+    const float4* f[4];
+    const float* v[4];
+    unsigned int n;
+};
+
+__global__ __launch_bounds__(256) void thermo_partial_kernel(unsigned int N, const double4* vel,
+                                                             ForcePtrs fp, double* partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned int n = blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += gridDim.x * blockDim.x)
+    {
+        double4 v = vel[n];
+        acc[0] += v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+        for (unsigned int q = 0; q < fp.n; q++)
+        {
+            acc[1] += (double)fp.f[q][n].w;
+            if (fp.v[q])
+                for (int m = 0; m < 6; m++)
+                    acc[3 + m] += (double)fp.v[q][(size_t)m * N + n];
+        }
+    }
+    for (int k = 0; k < 9; k++)
+    {
+        double t = block_sum_256(acc[k], sh);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x * 9 + k] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void thermo_final_kernel(const double* partial,
+                                                           unsigned int nblocks, double* out)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    for (int k = 0; k < 9; k++)
+    {
+        double acc = 0.0;
+        for (unsigned int b = threadIdx.x; b < nblocks; b += 256)
+            acc += partial[b * 9 + k];
+        double t = block_sum_256(acc, sh);
+        if (threadIdx.x == 0)
+        {
+            if (k == 0)
+                out[0] = 0.5 * t;
+            else if (k == 1)
+                out[1] = t;
+            else if (k >= 3)
+                out[k] = t;
+        }
+    }
+    if (threadIdx.x == 0)
+        out[2] = (1.0 / 3.0) * (out[3] + out[6] + out[8]);
+}
+
+hipError_t launch_thermo(unsigned int N, const double4* vel, const float4* const* f,
+                         const float* const* vir, unsigned int nforces, double* partial,
+                         double* out, hipStream_t s)
+{
+    ForcePtrs fp;
+    fp.n = nforces > 4 ? 4 : nforces;
+    for (int q = 0; q < 4; q++)
+    {
+        fp.f[q] = q < (int)fp.n ? f[q] : nullptr;
+        fp.v[q] = (q < (int)fp.n && vir) ? vir[q] : nullptr;
+    }
+    hipLaunchKernelGGL(thermo_partial_kernel, dim3(THERMO_BLOCKS), dim3(256), 0, s, N, vel, fp,
+                       partial);
+    hipLaunchKernelGGL(thermo_final_kernel, dim3(1), dim3(256), 0, s, partial, THERMO_BLOCKS, out);
+    return hipGetLastError();
+}
+
+// ======================================================================== thermalize
+// ParticleGroup::thermalizeParticleMomenta (hoomd/ParticleGroup.cc:690-806): Philox
+// Seed(35, timestep, seed), Counter(tag); three normals of width sqrt(kT/m); COM momentum
+// removed afterwards.
+__global__ __launch_bounds__(256) void thermalize_kernel(unsigned int N, double4* vel,
+                                                         const unsigned int* tag, double kT,
+                                                         uint64_t timestep, uint16_t seed,
+                                                         double* partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    double p[3] = {0, 0, 0};
+    for (unsigned int n = blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += gridDim.x * blockDim.x)
+    {
+        double4 v = vel[n];
+        Philox g(35, timestep, seed, tag[n]);
+        double sigma = sqrt(kT / v.w);
+        v.x = normal(g, sigma, 0.0);
+        v.y = normal(g, sigma, 0.0);
+        v.z = normal(g, sigma, 0.0);
+        vel[n] = v;
+        p[0] += v.w * v.x;
+        p[1] += v.w * v.y;
+        p[2] += v.w * v.z;
+    }
+    for (int k = 0; k < 3; k++)
+    {
+        double t = block_sum_256(p[k], sh);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x * 3 + k] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void remove_com_kernel(unsigned int N, double4* vel,
+                                                         const double* partial,
+                                                         unsigned int nblocks)
+{
+#pragma clang fp contract(off)
+    double tot[3] = {0, 0, 0};
+    for (unsigned int b = 0; b < nblocks; b++)
+        for (int k = 0; k < 3; k++)
+            tot[k] += partial[b * 3 + k];
+    const double cx = tot[0] / (double)N, cy = tot[1] / (double)N, cz = tot[2] / (double)N;
+    for (unsigned int n = blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += gridDim.x * blockDim.x)
+    {
+        double4 v = vel[n];
+        v.x = v.x - cx / v.w;
+        v.y = v.y - cy / v.w;
+        v.z = v.z - cz / v.w;
+        vel[n] = v;
+    }
+}
+
+hipError_t launch_thermalize(unsigned int N, double4* vel, const unsigned int* tag, double kT,
+                             uint64_t timestep, uint16_t seed, double* scratch3, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    const unsigned int nb = THERMO_BLOCKS;
+    hipLaunchKernelGGL(thermalize_kernel, dim3(nb), dim3(256), 0, s, N, vel, tag, kT, timestep,
+                       seed, scratch3);
+    hipLaunchKernelGGL(remove_com_kernel, dim3(nb), dim3(256), 0, s, N, vel, scratch3, nb);
+    return hipGetLastError();
+}
+
+// ======================================================================== Hilbert keys
+// SFCPackTuner::generateTraversalOrder (hoomd/SFCPackTuner.cc:290-487) visits the eight
+// children of each sub-box in an order that is permuted per child by eight fixed rules. The
+// rank of a bin is therefore a base-8 number whose digit at each level is the position of the
+// bin's child in the current order; it is computed directly instead of storing the grid^3
+// table. Walking tables and rules are the reference's (they define the curve).
+__constant__ unsigned char c_child_of[8] = {0, 1, 3, 2, 7, 6, 4, 5}; // (di*4+dj*2+dk) -> child
+__constant__ unsigned char c_istep[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+__constant__ unsigned char c_jstep[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+__constant__ unsigned char c_kstep[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ unsigned char c_rule[8][8] = {
+    {0, 3, 4, 7, 6, 5, 2, 1}, {0, 7, 6, 1, 2, 5, 4, 3}, {0, 7, 6, 1, 2, 5, 4, 3},
+    {2, 3, 0, 1, 6, 7, 4, 5}, {2, 3, 0, 1, 6, 7, 4, 5}, {4, 3, 2, 5, 6, 1, 0, 7},
+    {4, 3, 2, 5, 6, 1, 0, 7}, {6, 5, 2, 1, 0, 3, 4, 7}};
+
+__device__ unsigned int hilbert_rank(unsigned int ib, unsigned int jb, unsigned int kb,
+                                     unsigned int grid)
+{
+    unsigned char order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    unsigned int rank = 0;
+    unsigned int i0 = 0, j0 = 0, k0 = 0;
+    for (unsigned int w = grid >> 1; w >= 1; w >>= 1)
+    {
+        unsigned int di = (ib - i0) >= w, dj = (jb - j0) >= w, dk = (kb - k0) >= w;
+        unsigned char child = c_child_of[di * 4 + dj * 2 + dk];
+        unsigned int m = 0;
+        while (order[m] != child)
+            m++;
+        rank = rank * 8 + m;
+        i0 += w * c_istep[child];
+        j0 += w * c_jstep[child];
+        k0 += w * c_kstep[child];
+        unsigned char nxt[8];
+        for (int q = 0; q < 8; q++)
+            nxt[q] = order[c_rule[m][q]];
+        for (int q = 0; q < 8; q++)
+            order[q] = nxt[q];
+        if (w == 1)
+            break;
+    }
+    return rank;
+}
+
+// gpu_sfc_bin_particles_kernel (hoomd/SFCPackTunerGPU.cu:26-79): bin = (uint)(f * grid) % grid,
+// clamped; key = (rank << 32) | index so a key sort is the stable (rank, index) sort.
+__global__ void sfc_keys_kernel(unsigned int N, const double4* pos, DevBox box, unsigned int grid,
+                                unsigned long long* keys)
+{
+#pragma clang fp contract(off)
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    double4 p = pos[n];
+    double f[3] = {(p.x - box.lo[0]) * box.Linv[0], (p.y - box.lo[1]) * box.Linv[1],
+                   (p.z - box.lo[2]) * box.Linv[2]};
+    int b[3];
+    for (int d = 0; d < 3; d++)
+    {
+        int v = (int)((unsigned int)(f[d] * (double)grid) % grid);
+        if (v < 0)
+            v = 0;
+        if (v >= (int)grid)
+            v = (int)grid - 1;
+        b[d] = v;
+    }
+    unsigned int rank = hilbert_rank((unsigned)b[0], (unsigned)b[1], (unsigned)b[2], grid);
+    keys[n] = ((unsigned long long)rank << 32) | n;
+}
+
+hipError_t launch_sfc_keys(unsigned int N, const double4* pos, const DevBox& box,
+                           unsigned int grid, unsigned long long* keys, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(sfc_keys_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, box, grid,
+                       keys);
+    return hipGetLastError();
+}
+
+__global__ void keys_to_order_kernel(unsigned int N, const unsigned long long* keys,
+                                     unsigned int* order)
+{
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N)
+        order[n] = (unsigned int)(keys[n] & 0xffffffffull);
+}
+
+hipError_t launch_keys_to_order(unsigned int N, const unsigned long long* keys,
+                                unsigned int* order, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(keys_to_order_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, keys,
+                       order);
+    return hipGetLastError();
+}
+
+__global__ void traversal_kernel(unsigned int grid, unsigned int* traversal)
+{
+    unsigned int bin = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int nb = grid * grid * grid;
+    if (bin >= nb)
+        return;
+    unsigned int i = bin / (grid * grid), j = (bin / grid) % grid, k = bin % grid;
+    traversal[bin] = hilbert_rank(i, j, k, grid);
+}
+
+hipError_t launch_traversal(unsigned int grid, unsigned int* traversal, hipStream_t s)
+{
+    unsigned int nb = grid * grid * grid;
+    hipLaunchKernelGGL(traversal_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, grid, traversal);
+    return hipGetLastError();
+}
+
+// ======================================================================== permutation helpers
+// SFCPackTuner::applySortOrder (hoomd/SFCPackTuner.cc:119-285): dst[n] = src[order[n]]
+__global__ void gather_words_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                    const unsigned int* __restrict__ order, unsigned int N,
+                                    unsigned int words)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t total = (size_t)N * words;
+    if (t >= total)
+        return;
+    unsigned int n = (unsigned int)(t / words), w = (unsigned int)(t % words);
+    dst[t] = src[(size_t)order[n] * words + w];
+}
+
+hipError_t gather_bytes(const void* src, void* dst, const unsigned int* order, unsigned int N,
+                        unsigned int elem_bytes, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    unsigned int words = elem_bytes / 4;
+    size_t total = (size_t)N * words;
+    hipLaunchKernelGGL(gather_words_kernel, dim3((unsigned int)((total + 255) / 256)), dim3(256), 0,
+                       s, (const uint32_t*)src, (uint32_t*)dst, order, N, words);
+    return hipGetLastError();
+}
+
+__global__ void rtag_kernel(unsigned int N, const unsigned int* tag, unsigned int* rtag)
+{
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N)
+        rtag[tag[n]] = n;
+}
+
+hipError_t launch_rtag(unsigned int N, const unsigned int* tag, unsigned int* rtag, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(rtag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, tag, rtag);
+    return hipGetLastError();
+}
+
+// NeighborList::updateExListIdx (hoomd/md/NeighborList.cc): exclusions held by tag, mapped to
+// indices after every sort.
+__global__ void excl_index_kernel(unsigned int N, const unsigned int* tag, const unsigned int* rtag,
+                                  const unsigned int* n_excl_tag, const unsigned int* excl_tag,
+                                  unsigned int max_excl, unsigned int* n_excl, unsigned int* excl)
+{
+    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    unsigned int tg = tag[n];
+    unsigned int ne = n_excl_tag[tg];
+    for (unsigned int q = 0; q < max_excl; q++)
+        excl[(size_t)n * max_excl + q] =
+            q < ne ? rtag[excl_tag[(size_t)tg * max_excl + q]] : 0xffffffffu;
+    n_excl[n] = ne;
+}
+
+hipError_t launch_excl_index(unsigned int N, const unsigned int* tag, const unsigned int* rtag,
+                             const unsigned int* n_excl_tag, const unsigned int* excl_tag,
+                             unsigned int max_excl, unsigned int* n_excl, unsigned int* excl,
+                             hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(excl_index_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, tag, rtag,
+                       n_excl_tag, excl_tag, max_excl, n_excl, excl);
+    return hipGetLastError();
+}
+
+} // namespace hm

@@ -1,0 +1,313 @@
+/*
+ * hoomd_amd.h -- C ABI of the MI355X (gfx950) short-range MD timestep library.
+ *
+ * Two layers share this header:
+ *
+ *  1. Kernel drivers (hm_*): one call per GPU pass of the reference's hot path. Each replaces a
+ *     HOOMD-blue "driver function" (the C-style layer between the *GPU C++ classes and the
+ *     kernels, ARCHITECTURE.md:198-208) and takes plain device pointers, sizes and a stream.
+ *     They are asynchronous; data errors (overflow, NaN, out-of-box, stretched bond) come back
+ *     through device condition words, as in the reference (CellList.cc:549-607,
+ *     NeighborList.cc:1601-1617).
+ *
+ *  2. The system engine (hm_sys_*): a device-resident particle system that runs whole timesteps
+ *     natively (cell list, neighbour list, pair and bond forces, integration, thermostats and
+ *     the Hilbert sort) with every rebuild and thermostat decision taken on the device. It is
+ *     what the hoomd_amd Python package (the hoomd.md surface) drives.
+ *
+ * Memory layout in HBM (structure of arrays, index order = current sorted order):
+ *   pos      double[4N]  x, y, z, type (type stored as an exact double)
+ *   vel      double[4N]  vx, vy, vz, mass
+ *   accel    double[3N]
+ *   image    int[3N]
+ *   tag      uint32[N], rtag uint32[N]
+ *   force    float[4N]   fx, fy, fz, energy (per force compute, FP32 accumulation)
+ *   virial   float[6N]   xx, xy, xz, yy, yz, zz rows of pitch N
+ *   nlist    uint32[N*nmax] row-major (row i holds the neighbours of particle i), n_neigh uint32[N]
+ *   cell     cell_size uint32[ncell]; cell entries hm_cell_entry[ncell*cmax]
+ *
+ * Return codes: HM_OK (0) or a negative HM_ERR_* value. No call throws.
+ * Boxes are orthorhombic and periodic in x, y and z (every workload here is cubic).
+ */
+#ifndef HOOMD_AMD_H
+#define HOOMD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_OK 0
+#define HM_ERR_LAUNCH (-1)   /* a kernel launch or HIP runtime call failed */
+#define HM_ERR_ARGS (-2)     /* invalid arguments (sizes, NULL pointers, limits) */
+#define HM_ERR_ALLOC (-3)    /* device allocation failed */
+#define HM_ERR_NAN (-4)      /* a particle position is NaN */
+#define HM_ERR_OUTSIDE (-5)  /* a particle left the box */
+#define HM_ERR_BOND (-6)     /* a FENE bond is stretched beyond r0 */
+#define HM_ERR_THERMO (-7)   /* Bussi thermostat with zero kinetic energy */
+#define HM_ERR_STATE (-8)    /* engine used in the wrong order */
+#define HM_ERR_BOXSMALL (-9) /* r_cut + buffer exceeds half the box */
+
+#define HM_MAX_TYPES 4
+
+typedef void* hm_stream_t; /* a hipStream_t; NULL is the null stream */
+
+/* Orthorhombic periodic box; lo = -L/2, Linv = 1/L are derived as BoxDim::setL does. */
+typedef struct hm_box
+{
+    double L[3];
+} hm_box;
+
+/* One cell-list slot (32 bytes). */
+typedef struct hm_cell_entry
+{
+    double x, y, z;
+    uint32_t idx;
+    uint32_t type;
+} hm_cell_entry;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Library                                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Version string of this library. */
+const char* hm_version(void);
+/* Number of visible GPUs (0 when none); does not initialise a context on failure. */
+int hm_device_count(void);
+/* Text of the last error recorded on this thread. */
+const char* hm_last_error(void);
+/* Device memory helpers so callers without a HIP runtime binding can stage buffers. */
+int hm_malloc(void** ptr, size_t bytes);
+int hm_free(void* ptr);
+int hm_memcpy_htod(void* dst, const void* src, size_t bytes);
+int hm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int hm_memset(void* dst, int value, size_t bytes);
+int hm_synchronize(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Kernel drivers                                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Cell grid for a nominal width: dim = max(1, floor(L / width)) per axis.
+ * Replaces CellList::computeDimensions (hoomd/CellList.cc:83-131). Host-only. */
+int hm_cell_dims(const hm_box* box, double nominal_width, unsigned int dim[3]);
+/* Number of unique adjacent cells: prod(min(dim, 3)) (hoomd/CellList.cc:258-275). Host-only. */
+unsigned int hm_cell_n_adj(const unsigned int dim[3]);
+/* Adjacency table adj[cell*n_adj + k], sorted and unique per cell, built on the device.
+ * Replaces CellList::initializeCellAdj (hoomd/CellList.cc:346-402). */
+int hm_cell_adj(const unsigned int dim[3], unsigned int* d_adj, hm_stream_t stream);
+
+typedef struct hm_cell_args
+{
+    unsigned int N;
+    const double* pos;          /* [4N] */
+    hm_box box;
+    unsigned int dim[3];
+    unsigned int cmax;          /* slots per cell */
+    unsigned int* cell_size;    /* [ncell]; cleared by this call */
+    hm_cell_entry* entries;     /* [ncell*cmax] */
+    double* last_pos;           /* [3N] or NULL: receives the binned positions */
+    unsigned int* conditions;   /* [3] atomicMax'd: needed cmax, NaN particle + 1, outside + 1 */
+} hm_cell_args;
+/* Bin every particle. Slot order inside a cell is unspecified (atomics); the neighbour-list
+ * build orders each cell by particle index. Replaces gpu_compute_cell_list
+ * (hoomd/CellListGPU.cuh, kernel CellListGPU.cu:42-171) driven by CellListGPU::computeCellList
+ * (hoomd/CellListGPU.cc:39-232). */
+int hm_cell_list_build(const hm_cell_args* args, hm_stream_t stream);
+
+typedef struct hm_nlist_args
+{
+    unsigned int N;
+    const double* pos;          /* [4N] */
+    hm_box box;
+    unsigned int dim[3];
+    unsigned int cmax;
+    const unsigned int* cell_size;
+    const hm_cell_entry* entries;
+    const unsigned int* cell_adj; /* [ncell*n_adj] */
+    unsigned int n_adj;
+    unsigned int ntypes;        /* <= HM_MAX_TYPES */
+    double r_cut[HM_MAX_TYPES * HM_MAX_TYPES]; /* r_cut[ti*ntypes+tj]; <= 0 disables the pair */
+    double r_buff;
+    const unsigned int* n_excl; /* [N] or NULL: excluded neighbours (indices) per particle */
+    const unsigned int* excl;   /* [N*max_excl] */
+    unsigned int max_excl;
+    unsigned int nmax;          /* row length of nlist (multiple of 4) */
+    unsigned int* nlist;        /* [N*nmax] */
+    unsigned int* n_neigh;      /* [N] */
+    unsigned int* conditions;   /* [1] atomicMax'd: largest n_neigh when it exceeds nmax */
+} hm_nlist_args;
+/* Full neighbour list: for each i every j != i with |r_ij|^2 <= (r_cut + r_buff)^2 in FP64,
+ * candidates in adjacency-cell order and particle-index order within a cell.
+ * Replaces gpu_compute_nlist_binned (hoomd/md/NeighborListGPUBinned.cuh, kernel
+ * NeighborListGPUBinned.cu:52-298) together with gpu_nlist_filter (NeighborListGPU.cu:158-223). */
+int hm_nlist_build(const hm_nlist_args* args, hm_stream_t stream);
+
+/* Distance check: flag[0] = 1 when any particle moved by >= ((r_cut_max + r_buff) - r_cut_max)/2
+ * since last_pos. Replaces gpu_nlist_needs_update_check_new (hoomd/md/NeighborListGPU.cuh,
+ * kernel NeighborListGPU.cu:42-84). flag must be cleared by the caller. */
+int hm_nlist_check(unsigned int N, const double* pos, const double* last_pos, hm_box box,
+                   double r_cut_max, double r_buff, unsigned int* flag, hm_stream_t stream);
+
+/* LJ parameters for one type pair, built as EvaluatorPairLJ::param_type(sigma, epsilon). */
+typedef struct hm_lj_params
+{
+    double sigma;
+    double epsilon;
+} hm_lj_params;
+
+typedef struct hm_pair_args
+{
+    unsigned int N;
+    const double* pos;
+    hm_box box;
+    const unsigned int* nlist;
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double r_cut[HM_MAX_TYPES * HM_MAX_TYPES];
+    double r_on[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;             /* 0 none, 1 shift, 2 xplor */
+    int compute_virial;
+    float* force;               /* [4N] */
+    float* virial;              /* [6N] pitch N (unused when compute_virial == 0) */
+    unsigned int tpp;           /* lanes per particle: 1, 2, 4, 8 or 16; 0 = tuned default */
+    unsigned int block_size;    /* 64..1024 step 64; 0 = tuned default */
+} hm_pair_args;
+/* Pair force, energy and virial with a full neighbour list: FP64 positions and minimum image,
+ * FP64 cutoff test (rsq < r_cut^2), FP32 evaluator arithmetic and FP32 per-particle sums;
+ * energy and virial carry the 1/2 of each pair. Replaces
+ * gpu_compute_pair_forces<EvaluatorPairLJ> (hoomd/md/PotentialPairGPU.cuh:579-656, kernel
+ * :140-404) as instantiated by PotentialPairGPUKernel.cu.inc:19-21. */
+int hm_pair_lj(const hm_pair_args* args, const hm_lj_params* params, hm_stream_t stream);
+
+typedef struct hm_fene_params
+{
+    double k, r0, epsilon, sigma, delta;
+} hm_fene_params;
+
+typedef struct hm_bond_args
+{
+    unsigned int N;
+    const double* pos;
+    hm_box box;
+    const unsigned int* tag;        /* [N] index -> tag */
+    const unsigned int* rtag;       /* [N] tag -> index */
+    const unsigned int* bond_row;   /* [N+1] CSR offsets by TAG: bonds of the particle with tag t */
+    const unsigned int* bond_other; /* partner TAG of each row entry */
+    const unsigned int* bond_type;  /* type of each row entry (may be NULL: type 0) */
+    unsigned int nbond_types;       /* <= HM_MAX_TYPES */
+    int compute_virial;
+    float* force;                   /* [4N] */
+    float* virial;                  /* [6N] pitch N */
+    unsigned int* flag;             /* [1] set to 1 when a bond is stretched beyond r0 */
+} hm_bond_args;
+/* FENE+WCA bond force, one thread per particle over its bond row (both endpoints hold the
+ * bond; energy and virial carry 1/2 per endpoint). Bonds are stored by tag so a particle sort
+ * needs no table rebuild. Replaces gpu_compute_bond_forces<EvaluatorBondFENE>
+ * (hoomd/md/PotentialBondGPU.cuh:98-239). */
+int hm_bond_fene(const hm_bond_args* args, const hm_fene_params* params, hm_stream_t stream);
+
+/* Velocity-Verlet step one with velocity rescale alpha (1 for NVE): v = (v + a dt/2) * alpha;
+ * r += v dt; wrap. Replaces gpu_nvt_rescale_step_one (hoomd/md/TwoStepConstantVolumeGPU.cu:
+ * 35-100) and gpu_nve_step_one (hoomd/md/TwoStepNVEGPU.cu:46-126). */
+int hm_nvt_step_one(unsigned int N, double* pos, double* vel, const double* accel, int* image,
+                    hm_box box, double dt, double alpha, hm_stream_t stream);
+/* Step two: a = (sum of forces)/m; v = v*alpha + a dt/2. Up to 4 force arrays are summed
+ * (the gpu_integrator_sum_net_force pass, hoomd/Integrator.cu:68, fused). Replaces
+ * gpu_nvt_rescale_step_two (TwoStepConstantVolumeGPU.cu:166-213). */
+int hm_nvt_step_two(unsigned int N, double* vel, double* accel, const float* const* forces,
+                    unsigned int nforces, double dt, double alpha, hm_stream_t stream);
+/* Langevin step two with per-particle Philox noise Seed(25, timestep, seed), Counter(tag).
+ * Replaces gpu_langevin_step_two (hoomd/md/TwoStepLangevinGPU.cu:51-182). gamma[ntypes]. */
+int hm_langevin_step_two(unsigned int N, double* vel, double* accel, const double* pos,
+                         const unsigned int* tag, const float* const* forces,
+                         unsigned int nforces, const double* gamma_by_type, unsigned int ntypes,
+                         double kT, double dt, uint64_t timestep, uint16_t seed,
+                         hm_stream_t stream);
+/* Thermodynamic sums in FP64 (deterministic order): out[0] = KE, out[1] = PE, out[2] = trace of
+ * the virial / 3 (W), out[3..8] = virial tensor sums. Replaces gpu_compute_thermo_partial_sums +
+ * final_sums (hoomd/md/ComputeThermoGPU.cu:46,141). forces may hold 0..4 arrays. */
+int hm_thermo(unsigned int N, const double* vel, const float* const* forces,
+              const float* const* virials, unsigned int nforces, double* out, hm_stream_t stream);
+/* Bussi rescale factor for the translational DOF from the kinetic energy K, drawn on the host
+ * thread exactly as BussiThermostat::compute_rescale_factor (hoomd/md/Thermostat.h:357-446). */
+double hm_bussi_alpha_host(double K, double ndof, double dt, double tau, double kT,
+                           uint64_t timestep, uint16_t seed, uint32_t instance_tag);
+/* Maxwell-Boltzmann velocities from Philox Seed(35, timestep, seed), Counter(tag), then the COM
+ * momentum removed. Replaces ParticleGroup::thermalizeParticleMomenta (hoomd/ParticleGroup.cc:
+ * 690-806). */
+int hm_thermalize(unsigned int N, double* vel, const unsigned int* tag, double kT,
+                  uint64_t timestep, uint16_t seed, hm_stream_t stream);
+/* Hilbert-curve order: order[new] = old index, sorted by (traversal rank of the particle's
+ * bin on a grid^3 lattice, index). Replaces gpu_sfc_bin_particles + thrust sort
+ * (hoomd/SFCPackTunerGPU.cu:26-146, SFCPackTunerGPU.cc:65-148). grid is a power of two <= 1024. */
+int hm_sfc_order(unsigned int N, const double* pos, hm_box box, unsigned int grid,
+                 unsigned int* order, hm_stream_t stream);
+/* Hilbert traversal rank of every bin of a grid^3 lattice (diagnostic; SFCPackTuner.cc:456-487). */
+int hm_sfc_traversal(unsigned int grid, unsigned int* traversal, hm_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* System engine                                                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct hm_system hm_system;
+
+/* Create a device-resident system from host arrays given in TAG order. type_id may be NULL
+ * (all type 0); mass may be NULL (all 1). Positions must lie inside the box. */
+hm_system* hm_sys_create(unsigned int N, hm_box box, unsigned int ntypes, const double* pos3,
+                         const double* vel3, const unsigned int* type_id, const double* mass,
+                         const int* image3, uint16_t seed);
+void hm_sys_destroy(hm_system* s);
+
+/* Neighbour list (md.nlist.Cell): buffer, bond exclusions on/off. */
+int hm_sys_set_nlist(hm_system* s, double r_buff, int exclude_bonds);
+/* LJ pair force (md.pair.LJ): r_cut/params per type pair (ntypes^2, row-major, symmetric);
+ * r_cut <= 0 disables a pair. mode: 0 none, 1 shift, 2 xplor. Returns the force id (>= 0). */
+int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
+                       const hm_lj_params* params, int mode);
+/* FENE+WCA bonds (md.bond.FENEWCA): bonds by tag [2*nb], bond types [nb] (may be NULL),
+ * params per bond type. Returns the force id. */
+int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                         const unsigned int* bond_typeid, unsigned int nbond_types,
+                         const hm_fene_params* params);
+/* Integration method: 0 ConstantVolume (NVE), 1 ConstantVolume + Bussi(kT, tau),
+ * 2 Langevin(kT, gamma per type in param[0..ntypes-1]). */
+int hm_sys_set_method(hm_system* s, int method, double dt, double kT, const double* param);
+/* Hilbert sorter (tune.ParticleSorter): period in steps (0 disables), grid (power of 2). */
+int hm_sys_set_sorter(hm_system* s, unsigned int period, unsigned int grid);
+/* Request the pressure tensor (virial) on every step (default off; on demand for logging). */
+int hm_sys_set_compute_virial(hm_system* s, int on);
+/* Launch configuration of the pair kernel (0 = keep/tuned). */
+int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_size);
+/* Steps per captured hipGraph chunk (0 disables graphs; default 100). */
+int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
+/* Advance nsteps timesteps. Returns HM_OK or a negative error (state is left at the last good
+ * chunk boundary on data errors). */
+int hm_sys_run(hm_system* s, uint64_t nsteps);
+uint64_t hm_sys_timestep(const hm_system* s);
+/* Copy out state by tag (each pointer may be NULL): pos3, vel3, image3, accel3. */
+int hm_sys_get_state(hm_system* s, double* pos3, double* vel3, int* image3, double* accel3);
+/* Replace positions/velocities/images by tag (forces a neighbour-list rebuild). */
+int hm_sys_set_state(hm_system* s, const double* pos3, const double* vel3, const int* image3);
+/* Per-particle force (fx, fy, fz, energy) [4N] and virial [6N] of one force id, by tag. */
+int hm_sys_get_forces(hm_system* s, int force_id, double* force4, double* virial6);
+/* Neighbour list of the last build as (tag_i, tag_j) pairs with tag_i < tag_j; returns the
+ * number of pairs (<= max_pairs) or a negative error. */
+long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pairs);
+/* Thermo sums of the current state: see hm_thermo. Evaluates forces first if stale. */
+int hm_sys_thermo(hm_system* s, double* out9);
+/* Statistics: out[0] = nlist builds, out[1] = nmax, out[2] = cmax, out[3] = sorts, out[4] = steps
+ * run, out[5] = rollbacks after overflow. */
+int hm_sys_stats(hm_system* s, uint64_t* out6);
+/* Thermalize velocities of all particles (ParticleGroup::thermalizeParticleMomenta). */
+int hm_sys_thermalize(hm_system* s, double kT);
+/* Launch stream used by the engine (a hipStream_t). */
+hm_stream_t hm_sys_stream(hm_system* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
