@@ -1,0 +1,223 @@
+"""Benchmark: the 1M-particle LJ liquid timestep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 1000] [--n 1048576] [--no-cpu]
+
+Workload (the benchmark's LJ-at-scale configuration): N = 1,048,576 particles (FCC 64^3 x 4) in a
+cubic periodic box at number density 0.8442, LJ eps = sigma = 1, r_cut = 2.5 (unshifted), cell
+neighbour list with buffer 0.4, Maxwell-Boltzmann velocities at kT = 1.2 (Philox thermalize,
+COM momentum zeroed), ConstantVolume + Bussi(kT = 1.2, tau = 0), dt = 0.005, Hilbert sort every
+500 steps (grid 256). Synthetic input; timed steps run with all state resident in HBM.
+
+Prints ONE JSON line. "value" is timesteps/s of the whole job (one GPU); particle-steps/s, the
+pair kernel's roofline and the CPU baseline (the oracle's C restatement of the reference CPU path,
+timed on a bounded sample of the same workload on this host) are reported beside it.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hoomd-blue_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = ("timesteps/s + particle-steps/s (1M LJ; MLP-CG vs plain) at 1 GPU; % HBM peak")
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fcc(n, rho):
+    from lattices import fcc_lattice
+    return fcc_lattice(n, rho)
+
+
+def build_sim(hoomd, n_cells, rho, seed, sort_period=500):
+    pos, L = fcc(n_cells, rho)
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = len(pos)
+    snap.particles.types = ["A"]
+    snap.particles.position[:] = pos
+    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
+    sim.create_state_from_snapshot(snap)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.2)
+    nl = hoomd.md.nlist.Cell(buffer=0.4)
+    lj = hoomd.md.pair.LJ(nlist=nl, default_r_cut=2.5, mode="none")
+    lj.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0)
+    bussi = hoomd.md.methods.thermostats.Bussi(kT=1.2, tau=0.0)
+    method = hoomd.md.methods.ConstantVolume(filter=hoomd.filter.All(), thermostat=bussi)
+    sim.operations.integrator = hoomd.md.Integrator(dt=0.005, forces=[lj], methods=[method])
+    sim.operations.tuners.clear()
+    sim.operations.add(hoomd.tune.ParticleSorter(trigger=sort_period, grid=256))
+    return sim, lj, pos, L
+
+
+def pair_roofline(sim, lj, N):
+    """Average pair-kernel launch duration (HIP events on the engine stream) and the
+    algorithmic bytes of one launch: 4 B per neighbour-list entry + N * (4 n_neigh + 32 position
+    + 16 force) [+ 24 virial]."""
+    import ctypes as C
+
+    from hoomd_amd import _lib
+    ms = C.c_double()
+    ent = C.c_uint64()
+    _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, 200, C.byref(ms),
+                                           C.byref(ent)), "time pair")
+    B = 4.0 * ent.value + N * (4 + 32 + 16)
+    achieved = B / (ms.value * 1e-3) / 1e9
+    return ms.value, ent.value, B, achieved
+
+
+def load_traffic(N):
+    """HBM bytes per pair launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
+    by a copy-kernel calibration), committed under profiles/ for this workload."""
+    path = os.path.join(ROOT, "profiles", "pair_lj_1M_traffic.json")
+    if N != 1048576 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(n_cells, rho, seed, steps):
+    """Oracle (FP64 C restatement of the reference CPU path: half list, 3rd law, Bussi) on one
+    host core, on the same 1M-particle state for a bounded number of steps."""
+    import oracle as O
+    from lattices import fcc_lattice
+    pos, L = fcc_lattice(n_cells, rho)
+    N = len(pos)
+    vel = O.thermalize(np.ones(N), np.arange(N), 1.2, 0, seed)
+    s = O.System(pos, vel, L, r_cut=2.5, mode="none", buffer=0.4, dt=0.005, seed=seed)
+    s.set_bussi(1.2, 0.0)
+    s.set_sorter(500, 256)
+    s.run(1)  # initial build, first step and sort outside the timed sample
+    b0 = s.n_builds
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    return steps / dt, dt, int(s.n_builds - b0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--n", type=int, default=1048576, help="particles (4 * n_cells^3)")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--pair-only", type=int, default=0,
+                    help="profiling mode: after warmup launch only the pair kernel this many "
+                         "times (for rocprofv3 counter passes) and exit")
+    ap.add_argument("--pair-tpp", type=int, default=0, help="pair kernel lanes per particle")
+    ap.add_argument("--pair-block", type=int, default=0, help="pair kernel block size")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="profiling mode: device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE) and exit")
+    a = ap.parse_args()
+    if a.copy_calibration:
+        from hoomd_amd import _lib
+        nb = a.copy_calibration << 20
+        src = _lib.DeviceArray((nb,), "u1", fill=1)
+        dst = _lib.DeviceArray((nb,), "u1", fill=0)
+        for _ in range(3):
+            _lib.check(_lib.lib().hm_memcpy_dtod(dst.ptr, src.ptr, nb))
+        _lib.check(_lib.lib().hm_synchronize())
+        log(f"[bench] copied {a.copy_calibration} MiB x3")
+        return 0
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(no domain decomposition); run with --gpus 1", file=sys.stderr)
+        return 2
+    n_cells = int(round((a.n / 4) ** (1 / 3)))
+    if 4 * n_cells**3 != a.n:
+        print("--n must be 4 * k^3 (FCC)", file=sys.stderr)
+        return 2
+    rho = 0.8442
+
+    import hoomd_amd as hoomd
+    from hoomd_amd import _lib
+
+    log(f"[bench] building N={a.n} LJ system")
+    sim, lj, pos, L = build_sim(hoomd, n_cells, rho, a.seed)
+    N = len(pos)
+    sim.run(0)
+    if a.pair_tpp or a.pair_block:
+        _lib.check(_lib.lib().hm_sys_set_pair_launch(sim._system, a.pair_tpp, a.pair_block))
+    log(f"[bench] warmup {a.warmup} steps")
+    sim.run(a.warmup)
+    _lib.check(_lib.lib().hm_synchronize())
+    if a.pair_only:
+        import ctypes as C
+        ms = C.c_double()
+        _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, a.pair_only,
+                                               C.byref(ms), None))
+        log(f"[bench] {a.pair_only} pair launches, {ms.value * 1e3:.1f} us each")
+        return 0
+    st0 = np.zeros(6, np.uint64)
+    _lib.check(_lib.lib().hm_sys_stats(sim._system, st0.ctypes.data))
+    log(f"[bench] timing {a.steps} steps")
+    _lib.check(_lib.lib().hm_synchronize())
+    t0 = time.perf_counter()
+    sim.run(a.steps)
+    _lib.check(_lib.lib().hm_synchronize())
+    elapsed = time.perf_counter() - t0
+    st1 = np.zeros(6, np.uint64)
+    _lib.check(_lib.lib().hm_sys_stats(sim._system, st1.ctypes.data))
+    tps = a.steps / elapsed
+    builds = int(st1[0] - st0[0])
+    log(f"[bench] {tps:.1f} steps/s, {builds} neighbour-list builds in the timed steps")
+
+    ms, entries, B, achieved = pair_roofline(sim, lj, N)
+    log(f"[bench] pair kernel {ms * 1e3:.1f} us, {B / 1e9:.3f} GB algorithmic, {achieved:.0f} GB/s")
+    traffic = load_traffic(N)
+
+    cpu = None
+    if not a.no_cpu:
+        log(f"[bench] CPU baseline: oracle, {a.cpu_steps} steps of the same workload, 1 core")
+        cps, cdt, cbuilds = cpu_baseline(n_cells, rho, a.seed, a.cpu_steps)
+        cpu = {"value": cps, "unit": "timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"N={N} LJ Bussi workload, {a.cpu_steps} steps after the first "
+                         f"(includes {cbuilds} list rebuilds), {cdt:.1f} s on one host core; "
+                         f"{cps * N:.3e} particle-steps/s",
+               "particle_steps_per_s": cps * N}
+        log(f"[bench] CPU {cps:.3f} steps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": tps,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 positions/integration",
+        "data": "synthetic (FCC lattice, Philox Maxwell-Boltzmann velocities)",
+        "particle_steps_per_s": tps * N,
+        "config": {"workload": f"LJ liquid N={N} (FCC {n_cells}^3x4), rho=0.8442, r_cut=2.5, "
+                               "buffer=0.4, Bussi kT=1.2 tau=0, dt=0.005, Hilbert sort every 500",
+                   "N": N, "box_L": L, "nlist_builds_in_timed_steps": builds,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "pair_kernel<EvaluatorPairLJ>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_us": ms * 1e3, "algorithmic_bytes": B,
+                     "nlist_entries": int(entries)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

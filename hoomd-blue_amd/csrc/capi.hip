@@ -72,6 +72,11 @@ int hm_memcpy_dtoh(void* dst, const void* src, size_t bytes)
     return check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hm_memcpy_dtoh");
 }
 
+int hm_memcpy_dtod(void* dst, const void* src, size_t bytes)
+{
+    return check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hm_memcpy_dtod");
+}
+
 int hm_memset(void* dst, int value, size_t bytes)
 {
     return check(hipMemset(dst, value, bytes), "hm_memset");
@@ -114,6 +119,8 @@ int hm_cell_list_build(const hm_cell_args* a, hm_stream_t stream)
 {
     if (!a || !a->pos || !a->cell_size || !a->entries || !a->conditions || a->cmax == 0)
         return fail(HM_ERR_ARGS, "hm_cell_list_build: invalid arguments");
+    if (a->cmax > 256)
+        return fail(HM_ERR_ARGS, "hm_cell_list_build: cmax above 256");
     hipStream_t s = (hipStream_t)stream;
     size_t ncell = (size_t)a->dim[0] * a->dim[1] * a->dim[2];
     int r = check(hipMemsetAsync(a->cell_size, 0, ncell * sizeof(unsigned int), s),
@@ -121,16 +128,20 @@ int hm_cell_list_build(const hm_cell_args* a, hm_stream_t stream)
     if (r)
         return r;
     DevBox box = make_devbox(a->box);
-    return check(launch_cell_build(a->N, (const double4*)a->pos, box, a->dim, a->cmax,
-                                   a->cell_size, a->entries, (d3*)a->last_pos, a->conditions,
-                                   nullptr, s),
-                 "hm_cell_list_build");
+    r = check(launch_cell_build(a->N, (const double4*)a->pos, box, a->dim, a->cmax, a->cell_size,
+                                a->entries, (d3*)a->last_pos, a->conditions, nullptr, s),
+              "hm_cell_list_build");
+    if (r)
+        return r;
+    return check(launch_cell_sort((unsigned int)ncell, a->cmax, a->cell_size, a->entries, nullptr,
+                                  s),
+                 "hm_cell_list_build: order");
 }
 
 int hm_nlist_build(const hm_nlist_args* a, hm_stream_t stream)
 {
     if (!a || !a->cell_size || !a->entries || !a->cell_adj || !a->nlist || !a->n_neigh
-        || !a->conditions || a->ntypes == 0 || a->ntypes > HM_MAX_TYPES || (a->nmax & 3u)
+        || !a->conditions || a->ntypes == 0 || a->ntypes > HM_MAX_TYPES || (a->nmax & 7u)
         || a->nmax == 0)
         return fail(HM_ERR_ARGS, "hm_nlist_build: invalid arguments");
     if (a->cmax > 256)
@@ -141,6 +152,10 @@ int hm_nlist_build(const hm_nlist_args* a, hm_stream_t stream)
     memset(&k, 0, sizeof(k));
     k.N = a->N;
     k.box = make_devbox(a->box);
+    k.dim[0] = a->dim[0];
+    k.dim[1] = a->dim[1];
+    k.dim[2] = a->dim[2];
+    k.pos = (const double4*)a->pos;
     k.ncell = a->dim[0] * a->dim[1] * a->dim[2];
     k.cmax = a->cmax;
     k.cell_size = a->cell_size;
@@ -181,7 +196,7 @@ int hm_pair_lj(const hm_pair_args* a, const hm_lj_params* params, hm_stream_t st
 {
     if (!a || !params || !a->pos || !a->nlist || !a->n_neigh || !a->force
         || (a->compute_virial && !a->virial) || a->ntypes == 0 || a->ntypes > HM_MAX_TYPES
-        || (a->nmax & 3u) || a->nmax == 0)
+        || (a->nmax & 7u) || a->nmax == 0)
         return fail(HM_ERR_ARGS, "hm_pair_lj: invalid arguments");
     if (a->tpp && a->tpp != 1 && a->tpp != 2 && a->tpp != 4 && a->tpp != 8 && a->tpp != 16)
         return fail(HM_ERR_ARGS, "hm_pair_lj: tpp must be 1, 2, 4, 8 or 16");

@@ -282,7 +282,7 @@ struct hm_system
         {
             double expect = rho * 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
             nmax = (unsigned int)std::ceil(expect * 1.3 + 16.0);
-            nmax = (nmax + 3) & ~3u;
+            nmax = (nmax + 7) & ~7u;
             dfree(nlist);
             nlist = dalloc<unsigned int>((size_t)N * nmax);
             hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
@@ -310,10 +310,11 @@ struct hm_system
 
     void grow_nmax(unsigned int need)
     {
-        // Nmax rounds up to a multiple of 4 (NeighborList.cc:1601-1617); extra headroom avoids
+        // Nmax rounds up to a multiple of 8 (the reference uses 4, NeighborList.cc:1601-1617,
+        // the pair kernel reads rows 8 entries at a time); extra headroom avoids
         // repeated replays when the maximum drifts upward by one or two.
         unsigned int m = (unsigned int)std::ceil(need * 1.15 + 8.0);
-        m = (m + 3) & ~3u;
+        m = (m + 7) & ~7u;
         if (m <= nmax)
             m = nmax + 8;
         nmax = m;
@@ -336,6 +337,10 @@ struct hm_system
         memset(&k, 0, sizeof(k));
         k.N = N;
         k.box = box;
+        k.dim[0] = dim[0];
+        k.dim[1] = dim[1];
+        k.dim[2] = dim[2];
+        k.pos = pos;
         k.ncell = ncell;
         k.cmax = cmax;
         k.cell_size = cell_size;
@@ -445,6 +450,7 @@ struct hm_system
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
                                    &cond->cell_need, st, stream),
                  "cell build");
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, st, stream), "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, true);
             hchk(launch_nlist_build(k, stream), "nlist build");
@@ -495,6 +501,7 @@ struct hm_system
                 grow_cmax(h_cond->cell_need);
                 continue;
             }
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, nullptr, stream), "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, false);
             k.counter = nullptr;
@@ -1045,6 +1052,15 @@ uint64_t hm_sys_timestep(const hm_system* s)
     return s ? s->timestep : 0;
 }
 
+int hm_sys_set_timestep(hm_system* s, uint64_t timestep)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_timestep: NULL system");
+    s->timestep = timestep;
+    s->prepared = false;
+    return HM_OK;
+}
+
 int hm_sys_get_state(hm_system* s, double* pos3, double* vel3, int* image3, double* accel3)
 {
     if (!s)
@@ -1282,6 +1298,48 @@ int hm_sys_thermalize(hm_system* s, double kT)
 hm_stream_t hm_sys_stream(hm_system* s)
 {
     return s ? (hm_stream_t)s->stream : nullptr;
+}
+
+int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg_ms,
+                     uint64_t* nlist_entries)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size() || s->forces[force_id].kind != 0
+        || iters == 0 || !avg_ms)
+        return fail(HM_ERR_ARGS, "hm_sys_time_pair: invalid arguments");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        auto& f = s->forces[force_id];
+        PairKernelArgs k;
+        fill_pair_args(k, s->N, s->pos, s->hbox, s->nlist, s->nmax, s->n_neigh, s->ntypes, f.rcut,
+                       f.ron, f.mode, s->compute_virial, f.f, f.v);
+        hipEvent_t e0, e1;
+        hchk(hipEventCreate(&e0), "event");
+        hchk(hipEventCreate(&e1), "event");
+        hchk(launch_pair_lj(k, f.p, s->pair_cfg, s->stream), "pair warm-up");
+        hchk(hipEventRecord(e0, s->stream), "event");
+        for (unsigned int q = 0; q < iters; q++)
+            hchk(launch_pair_lj(k, f.p, s->pair_cfg, s->stream), "pair");
+        hchk(hipEventRecord(e1, s->stream), "event");
+        hchk(hipEventSynchronize(e1), "event");
+        float ms = 0.f;
+        hchk(hipEventElapsedTime(&ms, e0, e1), "event");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = (double)ms / iters;
+        if (nlist_entries)
+        {
+            std::vector<unsigned int> nn(s->N);
+            hchk(hipMemcpy(nn.data(), s->n_neigh, sizeof(unsigned int) * s->N,
+                           hipMemcpyDeviceToHost),
+                 "download");
+            uint64_t tot = 0;
+            for (unsigned int v : nn)
+                tot += std::min(v, s->nmax);
+            *nlist_entries = tot;
+        }
+        return HM_OK;
+    });
 }
 
 } // extern "C"

@@ -67,6 +67,10 @@ struct NlistKernelArgs
 {
     unsigned int N;
     DevBox box;
+    unsigned int dim[3];
+    const double4* pos;       // current positions (FP64 re-decision in the guard band)
+    float rlistsq_f[HM_MAX_TYPES * HM_MAX_TYPES];  // set by launch_nlist_build
+    float guard[HM_MAX_TYPES * HM_MAX_TYPES];      // set by launch_nlist_build
     unsigned int ncell;
     unsigned int cmax;
     const unsigned int* cell_size;
@@ -86,6 +90,9 @@ struct NlistKernelArgs
     const StepState* gate;
     StepState* counter;       // n_builds incremented by block 0 when non-NULL
 };
+// order each cell's slots by particle index (deterministic candidate order)
+hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
+                            hm_cell_entry* entries, const StepState* gate, hipStream_t s);
 hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
 
 hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,

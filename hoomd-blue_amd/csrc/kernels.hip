@@ -142,22 +142,103 @@ hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& b
 
 // ======================================================================== neighbour list
 // NeighborListBinned::buildNlist (hoomd/md/NeighborListBinned.cc:36-179) in full storage,
-// restructured for CDNA: one workgroup per home cell stages the adjacent cells' particles in
-// LDS (each cell ordered by particle index), one wavefront per home particle sweeps the staged
-// candidates 64 at a time, and ballot + mbcnt compaction writes the hits contiguously into the
-// particle's row. Inclusion is dr^2 <= (r_cut + r_buff)^2 in FP64 with the same operation order
-// as the CPU path, so the rows equal the CPU full list entry for entry.
+// restructured for CDNA:
+//  * cell_sort_kernel orders every cell's slots by particle index once per build, so candidate
+//    order (adjacency cell order, then index) equals the CPU list's;
+//  * one workgroup per home cell stages the adjacent cells' particles in LDS, one wavefront per
+//    home particle sweeps the staged candidates 64 at a time, and ballot + mbcnt compaction
+//    writes the hits contiguously into the particle's row.
+// Inclusion is dr^2 <= (r_cut + r_buff)^2 decided exactly as the FP64 CPU path decides it. The
+// fast kernel tests in FP32 on coordinates relative to the home-cell centre and re-decides with
+// the CPU's FP64 formula every candidate whose FP32 distance lies within a guard band of the
+// threshold, the band being four times the worst-case FP32 error; rows therefore equal the CPU
+// rows entry for entry. Grids with fewer than 4 cells along an axis use the FP64 kernel.
 constexpr int NLB = 256;
 constexpr int NLCAP = 1024;
 constexpr int NLHOME = 256;
 
+__global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsigned int cmax,
+                                                        const unsigned int* __restrict__ cell_size,
+                                                        hm_cell_entry* entries,
+                                                        const StepState* gate)
+{
+    __shared__ unsigned int sidx[4][NLHOME];
+    __shared__ hm_cell_entry sent[4][NLHOME];
+    if (gate && gate->rebuild_at != gate->timestep + 1)
+        return;
+    const unsigned int w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const unsigned int c = blockIdx.x * 4 + w;
+    const unsigned int n = c < ncell ? min(cell_size[c], cmax) : 0u;
+    for (unsigned int k = lane; k < n; k += 64)
+    {
+        hm_cell_entry e = entries[(size_t)c * cmax + k];
+        sent[w][k] = e;
+        sidx[w][k] = e.idx;
+    }
+    __syncthreads();
+    for (unsigned int k = lane; k < n; k += 64)
+    {
+        const unsigned int my = sidx[w][k];
+        unsigned int rank = 0;
+        for (unsigned int m = 0; m < n; m++)
+            rank += sidx[w][m] < my ? 1u : 0u;
+        entries[(size_t)c * cmax + rank] = sent[w][k];
+    }
+}
+
+hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
+                            hm_cell_entry* entries, const StepState* gate, hipStream_t s)
+{
+    if (ncell == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(cell_sort_kernel, dim3((ncell + 3) / 4), dim3(256), 0, s, ncell, cmax,
+                       cell_size, entries, gate);
+    return hipGetLastError();
+}
+
+// Shared prologue: cell id, pass bounds and adjacency (sizes capped at cmax).
+__device__ __forceinline__ unsigned int find_segment(const unsigned int* aoff, unsigned int first,
+                                                     unsigned int last, unsigned int k)
+{
+    // largest q in [first, last) with aoff[q] <= k (aoff is non-decreasing)
+    unsigned int lo = first, hi = last - 1;
+    while (lo < hi)
+    {
+        unsigned int mid = (lo + hi + 1) >> 1;
+        if (aoff[mid] <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool pass_bounds(unsigned int n_adj, const unsigned int* asize,
+                                            unsigned int* aoff, unsigned int first,
+                                            unsigned int* pass_last, unsigned int* pass_total)
+{
+    if (threadIdx.x == 0)
+    {
+        unsigned int tot = 0, l = first;
+        while (l < n_adj && tot + asize[l] <= NLCAP)
+        {
+            aoff[l] = tot;
+            tot += asize[l];
+            l++;
+        }
+        *pass_last = l;
+        *pass_total = tot;
+    }
+    return true;
+}
+
+// General kernel: FP64 test for every candidate (any grid, including < 4 cells per axis).
 template<bool MT, bool EXCL>
 __global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
 {
 #pragma clang fp contract(off)
     __shared__ double sx[NLCAP], sy[NLCAP], sz[NLCAP];
     __shared__ unsigned int sj[NLCAP];
-    __shared__ unsigned int sraw[NLCAP];
     __shared__ unsigned char stp[MT ? NLCAP : 1];
     __shared__ unsigned int acell[27], asize[27], aoff[27];
     __shared__ unsigned int hcnt[NLHOME];
@@ -187,62 +268,21 @@ __global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
     unsigned int first = 0;
     while (first < a.n_adj)
     {
-        if (tid == 0)
-        {
-            unsigned int tot = 0, l = first;
-            while (l < a.n_adj && tot + asize[l] <= NLCAP)
-            {
-                aoff[l] = tot;
-                tot += asize[l];
-                l++;
-            }
-            pass_last = l;
-            pass_total = tot;
-        }
+        pass_bounds(a.n_adj, asize, aoff, first, &pass_last, &pass_total);
         __syncthreads();
         const unsigned int last = pass_last, total = pass_total;
-
-        // stage candidates: raw load, then rank by particle index inside each cell
-        hm_cell_entry e[NLCAP / NLB];
-        unsigned int segb[NLCAP / NLB], segn[NLCAP / NLB];
-#pragma unroll
-        for (int r = 0; r < NLCAP / NLB; r++)
+        for (unsigned int k = tid; k < total; k += NLB)
         {
-            unsigned int k = tid + r * NLB;
-            segn[r] = 0;
-            if (k < total)
-            {
-                unsigned int q = first;
-                while (q + 1 < last && aoff[q + 1] <= k)
-                    q++;
-                e[r] = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
-                segb[r] = aoff[q];
-                segn[r] = asize[q];
-                sraw[k] = e[r].idx;
-            }
+            const unsigned int q = find_segment(aoff, first, last, k);
+            const hm_cell_entry e = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
+            sx[k] = e.x;
+            sy[k] = e.y;
+            sz[k] = e.z;
+            sj[k] = e.idx;
+            if (MT)
+                stp[k] = (unsigned char)e.type;
         }
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < NLCAP / NLB; r++)
-        {
-            unsigned int k = tid + r * NLB;
-            if (k < total)
-            {
-                unsigned int rank = 0;
-                for (unsigned int m = 0; m < segn[r]; m++)
-                    rank += sraw[segb[r] + m] < e[r].idx ? 1u : 0u;
-                unsigned int p = segb[r] + rank;
-                sx[p] = e[r].x;
-                sy[p] = e[r].y;
-                sz[p] = e[r].z;
-                sj[p] = e[r].idx;
-                if (MT)
-                    stp[p] = (unsigned char)e[r].type;
-            }
-        }
-        __syncthreads();
-
-        // one wavefront per home particle
         for (unsigned int s = wave; s < nhome; s += NLB / 64)
         {
             const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
@@ -308,13 +348,195 @@ __global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
     }
 }
 
-hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s)
+// Minimum-image offset of a position from a cell centre, rounded to FP32 (dims >= 4 keeps every
+// staged particle within 1.5 cell widths of the centre, well inside half a box).
+__device__ __forceinline__ float3 rel_to_centre(const DevBox& b, double x, double y, double z,
+                                                double cx, double cy, double cz)
 {
-    if (a.ncell == 0)
+#pragma clang fp contract(off)
+    double dx = x - cx, dy = y - cy, dz = z - cz;
+    min_image(b, dx, dy, dz);
+    return make_float3((float)dx, (float)dy, (float)dz);
+}
+
+// Fast kernel: FP32 test on centre-relative coordinates, FP64 re-decision inside the guard band.
+template<bool MT, bool EXCL>
+__global__ __launch_bounds__(NLB) void nlist_f32_kernel(NlistKernelArgs a)
+{
+    __shared__ float4 sc[NLCAP];
+    __shared__ unsigned char stp[MT ? NLCAP : 1];
+    __shared__ float4 sh[NLHOME];
+    __shared__ unsigned char shtp[MT ? NLHOME : 1];
+    __shared__ unsigned int acell[27], asize[27], aoff[27];
+    __shared__ unsigned int hcnt[NLHOME];
+    __shared__ unsigned int pass_last, pass_total;
+
+    if (a.gate && a.gate->rebuild_at != a.gate->timestep + 1)
+        return;
+    const unsigned int cell = xcd_block(blockIdx.x, gridDim.x);
+    if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
+        a.counter->n_builds++;
+    if (cell >= a.ncell)
+        return;
+    const unsigned int nhome = min(a.cell_size[cell], a.cmax);
+    if (nhome == 0)
+        return;
+    const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    double cx, cy, cz;
+    {
+#pragma clang fp contract(off)
+        const unsigned int ci = cell % a.dim[0], cj = (cell / a.dim[0]) % a.dim[1],
+                           ck = cell / (a.dim[0] * a.dim[1]);
+        cx = a.box.lo[0] + ((double)ci + 0.5) * (a.box.L[0] / (double)a.dim[0]);
+        cy = a.box.lo[1] + ((double)cj + 0.5) * (a.box.L[1] / (double)a.dim[1]);
+        cz = a.box.lo[2] + ((double)ck + 0.5) * (a.box.L[2] / (double)a.dim[2]);
+    }
+    if (tid < a.n_adj)
+    {
+        unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
+        acell[tid] = c;
+        asize[tid] = min(a.cell_size[c], a.cmax);
+    }
+    for (unsigned int s = tid; s < nhome; s += NLB)
+    {
+        const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
+        const float3 r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
+        sh[s] = make_float4(r.x, r.y, r.z, __uint_as_float(h.idx));
+        if (MT)
+            shtp[s] = (unsigned char)h.type;
+        hcnt[s] = 0;
+    }
+    __syncthreads();
+
+    unsigned int first = 0;
+    while (first < a.n_adj)
+    {
+        pass_bounds(a.n_adj, asize, aoff, first, &pass_last, &pass_total);
+        __syncthreads();
+        const unsigned int last = pass_last, total = pass_total;
+        for (unsigned int k = tid; k < total; k += NLB)
+        {
+            const unsigned int q = find_segment(aoff, first, last, k);
+            const hm_cell_entry e = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
+            const float3 r = rel_to_centre(a.box, e.x, e.y, e.z, cx, cy, cz);
+            sc[k] = make_float4(r.x, r.y, r.z, __uint_as_float(e.idx));
+            if (MT)
+                stp[k] = (unsigned char)e.type;
+        }
+        __syncthreads();
+        for (unsigned int s = wave; s < nhome; s += NLB / 64)
+        {
+            const float4 h = sh[s];
+            const unsigned int i = __float_as_uint(h.w);
+            const unsigned int ti = MT ? shtp[s] : 0u;
+            unsigned int ex0 = 0xffffffffu, ex1 = 0xffffffffu, ex2 = 0xffffffffu,
+                         ex3 = 0xffffffffu;
+            if (EXCL)
+            {
+                unsigned int ne = min(a.n_excl[i], a.max_excl);
+                const unsigned int* er = a.excl + (size_t)i * a.max_excl;
+                if (ne > 0)
+                    ex0 = er[0];
+                if (ne > 1)
+                    ex1 = er[1];
+                if (ne > 2)
+                    ex2 = er[2];
+                if (ne > 3)
+                    ex3 = er[3];
+            }
+            unsigned int cnt = hcnt[s];
+            unsigned int* row = a.nlist + (size_t)i * a.nmax;
+            for (unsigned int base = 0; base < total; base += 64)
+            {
+                const unsigned int k = base + lane;
+                bool hit = false;
+                unsigned int j = 0;
+                if (k < total)
+                {
+                    const float4 c4 = sc[k];
+                    j = __float_as_uint(c4.w);
+                    const float dx = h.x - c4.x, dy = h.y - c4.y, dz = h.z - c4.z;
+                    const float dr2 = dx * dx + dy * dy + dz * dz;
+                    const unsigned int tp = MT ? ti * a.ntypes + stp[k] : 0u;
+                    const float rl = a.rlistsq_f[tp];
+                    hit = dr2 <= rl;
+                    if (fabsf(dr2 - rl) <= a.guard[tp])
+                    {
+                        // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
+#pragma clang fp contract(off)
+                        const double4 pi = a.pos[i];
+                        const double4 pj = a.pos[j];
+                        double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                        min_image(a.box, ddx, ddy, ddz);
+                        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        hit = d2 <= a.rlistsq[tp];
+                    }
+                    hit = hit && (j != i) && a.enabled[tp];
+                    if (EXCL)
+                        hit = hit && j != ex0 && j != ex1 && j != ex2 && j != ex3;
+                }
+                const uint64_t m = __ballot(hit);
+                if (hit)
+                {
+                    unsigned int p = cnt + lanes_below(m);
+                    if (p < a.nmax)
+                        row[p] = j;
+                }
+                cnt += (unsigned int)__popcll(m);
+            }
+            if (lane == 0)
+                hcnt[s] = cnt;
+        }
+        __syncthreads();
+        first = last;
+    }
+    for (unsigned int s = tid; s < nhome; s += NLB)
+    {
+        unsigned int i = __float_as_uint(sh[s].w);
+        unsigned int c = hcnt[s];
+        a.n_neigh[i] = c;
+        if (c > a.nmax)
+            atomicMax(a.cond, c);
+    }
+}
+
+hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
+{
+    if (a_in.ncell == 0)
         return hipSuccess;
-    unsigned int grid = round_up(a.ncell, 8);
-    bool mt = a.ntypes > 1;
-    bool ex = a.n_excl != nullptr && a.max_excl > 0;
+    NlistKernelArgs a = a_in;
+    const unsigned int grid = round_up(a.ncell, 8);
+    const bool mt = a.ntypes > 1;
+    const bool ex = a.n_excl != nullptr && a.max_excl > 0;
+    const bool fast = a.pos != nullptr && a.dim[0] >= 4 && a.dim[1] >= 4 && a.dim[2] >= 4;
+    if (fast)
+    {
+        // worst-case FP32 error of dr^2 near the threshold: coordinates are within
+        // dmax = 1.5 cell widths of the centre (rounding u*dmax each), differences within 2*dmax
+        const double u = 1.0 / 16777216.0;
+        double wmax = 0.0;
+        for (int d = 0; d < 3; d++)
+            wmax = fmax(wmax, a.box.L[d] / (double)a.dim[d]);
+        const double dmax = 1.5 * wmax;
+        const double e_dx = u * 2.0 * dmax + 2.0 * u * dmax;
+        const unsigned int nt = a.ntypes ? a.ntypes : 1;
+        for (unsigned int q = 0; q < nt * nt; q++)
+        {
+            const double rl2 = a.rlistsq[q];
+            const double rl = sqrt(rl2);
+            a.rlistsq_f[q] = (float)rl2;
+            a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rl * e_dx + 6.0 * u * rl2) + 1e-12);
+        }
+        if (mt && ex)
+            hipLaunchKernelGGL((nlist_f32_kernel<true, true>), dim3(grid), dim3(NLB), 0, s, a);
+        else if (mt)
+            hipLaunchKernelGGL((nlist_f32_kernel<true, false>), dim3(grid), dim3(NLB), 0, s, a);
+        else if (ex)
+            hipLaunchKernelGGL((nlist_f32_kernel<false, true>), dim3(grid), dim3(NLB), 0, s, a);
+        else
+            hipLaunchKernelGGL((nlist_f32_kernel<false, false>), dim3(grid), dim3(NLB), 0, s, a);
+        return hipGetLastError();
+    }
     if (mt && ex)
         hipLaunchKernelGGL((nlist_kernel<true, true>), dim3(grid), dim3(NLB), 0, s, a);
     else if (mt)
@@ -359,8 +581,12 @@ hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last
 // ======================================================================== pair forces
 // PotentialPair::computeForces (hoomd/md/PotentialPair.h:587-794) with a full list, the
 // structure of gpu_compute_pair_forces_shared_kernel (hoomd/md/PotentialPairGPU.cuh:140-404):
-// TPP lanes per particle stride the particle's neighbour row four entries (16 B) at a time;
-// the partial sums are combined by xor-shuffles inside the TPP group.
+// TPP lanes per particle share the particle's neighbour row, TPP * PAIR_BATCH entries per
+// batch. All PAIR_BATCH position gathers of a batch are issued before any is used, so each lane
+// keeps PAIR_BATCH loads in flight; slots past the row end gather the particle itself and are
+// masked out arithmetically. The partial sums are combined by xor-shuffles inside the TPP group.
+constexpr int PAIR_BATCH = 8;
+
 template<class Eval> struct ParamTable
 {
     typename Eval::param_type p[HM_MAX_TYPES * HM_MAX_TYPES];
@@ -385,67 +611,81 @@ __global__ __launch_bounds__(1024) void pair_kernel(PairKernelArgs a, ParamTable
     float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
     const unsigned int* row = a.nlist + (size_t)ii * a.nmax;
 
-    for (unsigned int k0 = sub * 4; k0 < n; k0 += TPP * 4)
+    // Entry k0 + q*TPP + sub belongs to lane sub: in every load instruction the TPP lanes of a
+    // particle read TPP consecutive row entries, whose positions (cell order, index-sorted inside
+    // a cell, Hilbert-sorted indices) share few cache lines.
+    for (unsigned int k0 = 0; k0 < n; k0 += TPP * PAIR_BATCH)
     {
-        const uint4 jj = *reinterpret_cast<const uint4*>(row + k0);
-        const unsigned int js[4] = {jj.x, jj.y, jj.z, jj.w};
+        unsigned int js[PAIR_BATCH];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < PAIR_BATCH; q++)
         {
-            if (k0 + q < n)
+            const unsigned int e = k0 + q * TPP + sub;
+            js[q] = e < n ? row[e] : ii;
+        }
+        double px[PAIR_BATCH], py[PAIR_BATCH], pz[PAIR_BATCH];
+        unsigned int tj[PAIR_BATCH];
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+        {
+            const unsigned int j = min(js[q], jmax);
+            const double4 pj = a.pos[j];
+            px[q] = pj.x;
+            py[q] = pj.y;
+            pz[q] = pj.z;
+            tj[q] = MT ? (unsigned int)pj.w : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+        {
+            double dx, dy, dz, rsq;
             {
-                const double4 pj = a.pos[min(js[q], jmax)];
-                double dx, dy, dz, rsq;
-                {
 #pragma clang fp contract(off)
-                    dx = pi.x - pj.x;
-                    dy = pi.y - pj.y;
-                    dz = pi.z - pj.z;
-                    min_image(a.box, dx, dy, dz);
-                    rsq = dx * dx + dy * dy + dz * dz;
-                }
-                const unsigned int tp = MT ? ti * a.ntypes + (unsigned int)pj.w : 0u;
-                if (rsq < a.rcutsq[tp])
+                dx = pi.x - px[q];
+                dy = pi.y - py[q];
+                dz = pi.z - pz[q];
+                min_image(a.box, dx, dy, dz);
+                rsq = dx * dx + dy * dy + dz * dz;
+            }
+            const unsigned int tp = MT ? ti * a.ntypes + tj[q] : 0u;
+            const bool in = (k0 + q * TPP + sub < n) && (rsq < a.rcutsq[tp]);
+            const float rsqf = in ? (float)rsq : 1.0f;
+            const float rcf = a.rcutsq_f[tp];
+            Eval ev(rsqf, rcf, tab.p[tp]);
+            float fdr = 0.f, e = 0.f;
+            const bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
+            ev.evalForceAndEnergy(fdr, e, shift);
+            if (SHIFT == 2)
+            {
+                // XPLOR smoothing (PotentialPair.h:706-731)
+                const float ron = a.ronsq_f[tp];
+                if (rsqf >= ron && rsqf < rcf)
                 {
-                    const float rsqf = (float)rsq;
-                    const float rcf = a.rcutsq_f[tp];
-                    Eval ev(rsqf, rcf, tab.p[tp]);
-                    float fdr = 0.f, e = 0.f;
-                    bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
-                    if (ev.evalForceAndEnergy(fdr, e, shift))
-                    {
-                        if (SHIFT == 2)
-                        {
-                            // XPLOR smoothing (PotentialPair.h:706-731)
-                            const float ron = a.ronsq_f[tp];
-                            if (rsqf >= ron && rsqf < rcf)
-                            {
-                                float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
-                                float rmc = rsqf - rcf;
-                                float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
-                                float ds = 12.f * (rsqf - ron) * rmc * den;
-                                float e0 = e;
-                                e = e0 * sw;
-                                fdr = sw * fdr - ds * e0;
-                            }
-                        }
-                        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
-                        fx += dxf * fdr;
-                        fy += dyf * fdr;
-                        fz += dzf * fdr;
-                        pe += 0.5f * e;
-                        if (VIRIAL)
-                        {
-                            const float f2 = 0.5f * fdr;
-                            vxx += f2 * dxf * dxf;
-                            vxy += f2 * dxf * dyf;
-                            vxz += f2 * dxf * dzf;
-                            vyy += f2 * dyf * dyf;
-                            vyz += f2 * dyf * dzf;
-                            vzz += f2 * dzf * dzf;
-                        }
-                    }
+                    float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
+                    float rmc = rsqf - rcf;
+                    float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
+                    float ds = 12.f * (rsqf - ron) * rmc * den;
+                    float e0 = e;
+                    e = e0 * sw;
+                    fdr = sw * fdr - ds * e0;
                 }
+            }
+            fdr = in ? fdr : 0.f;
+            e = in ? e : 0.f;
+            const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+            fx += dxf * fdr;
+            fy += dyf * fdr;
+            fz += dzf * fdr;
+            pe += 0.5f * e;
+            if (VIRIAL)
+            {
+                const float f2 = 0.5f * fdr;
+                vxx += f2 * dxf * dxf;
+                vxy += f2 * dxf * dyf;
+                vxz += f2 * dxf * dzf;
+                vyy += f2 * dyf * dyf;
+                vyz += f2 * dyf * dzf;
+                vzz += f2 * dzf * dzf;
             }
         }
     }

@@ -72,6 +72,7 @@ SIGNATURES = {
     "hm_memcpy_htod": (_i, [_p, _p, C.c_size_t]),
     "hm_memcpy_dtoh": (_i, [_p, _p, C.c_size_t]),
     "hm_memset": (_i, [_p, _i, C.c_size_t]),
+    "hm_memcpy_dtod": (_i, [_p, _p, C.c_size_t]),
     "hm_synchronize": (_i, []),
     "hm_cell_dims": (_i, [C.POINTER(HmBox), _d, _p]),
     "hm_cell_n_adj": (_u, [_p]),
@@ -101,6 +102,7 @@ SIGNATURES = {
     "hm_sys_set_chunk": (_i, [_p, _u, _i]),
     "hm_sys_run": (_i, [_p, _u64]),
     "hm_sys_timestep": (_u64, [_p]),
+    "hm_sys_set_timestep": (_i, [_p, _u64]),
     "hm_sys_get_state": (_i, [_p, _p, _p, _p, _p]),
     "hm_sys_set_state": (_i, [_p, _p, _p, _p]),
     "hm_sys_get_forces": (_i, [_p, _i, _p, _p]),
@@ -109,6 +111,7 @@ SIGNATURES = {
     "hm_sys_stats": (_i, [_p, _p]),
     "hm_sys_thermalize": (_i, [_p, _d]),
     "hm_sys_stream": (_p, [_p]),
+    "hm_sys_time_pair": (_i, [_p, _i, _u, C.POINTER(_d), C.POINTER(_u64)]),
 }
 
 

@@ -1,0 +1,71 @@
+"""md.compute.ThermodynamicQuantities (hoomd/md/compute.py -> ComputeThermo.cc:76-292)."""
+
+import numpy as np
+
+from .. import _lib
+
+
+class ThermodynamicQuantities:
+    def __init__(self, filter):
+        self.filter = filter
+        self._simulation = None
+
+    def _needs_virial(self):
+        return False  # the virial is evaluated on demand when a pressure is read
+
+    def _thermo(self):
+        sim = self._simulation
+        if sim is None or sim._system is None:
+            raise RuntimeError("ThermodynamicQuantities needs an attached simulation")
+        out = np.zeros(9)
+        _lib.check(_lib.lib().hm_sys_thermo(sim._system, out.ctypes.data), "thermo")
+        return out
+
+    @property
+    def num_particles(self):
+        return self._simulation.state.N_particles
+
+    @property
+    def translational_degrees_of_freedom(self):
+        integ = self._simulation.operations.integrator
+        from .methods import Langevin
+        N = self.num_particles
+        if integ is not None and integ.methods and isinstance(integ.methods[0], Langevin):
+            return 3.0 * N
+        return 3.0 * N - 3.0
+
+    degrees_of_freedom = translational_degrees_of_freedom
+
+    @property
+    def volume(self):
+        b = self._simulation.state.box
+        return b[0] * b[1] * b[2]
+
+    @property
+    def kinetic_energy(self):
+        return float(self._thermo()[0])
+
+    translational_kinetic_energy = kinetic_energy
+
+    @property
+    def potential_energy(self):
+        return float(self._thermo()[1])
+
+    @property
+    def kinetic_temperature(self):
+        return 2.0 * self.kinetic_energy / self.translational_degrees_of_freedom
+
+    @property
+    def pressure(self):
+        o = self._thermo()
+        return float((2.0 * o[0] / 3.0 + o[2]) / self.volume)
+
+    @property
+    def pressure_tensor(self):
+        o = self._thermo()
+        snap = self._simulation.state.get_snapshot()
+        v = snap.particles.velocity
+        m = snap.particles.mass
+        kin = [np.sum(m * v[:, a] * v[:, b]) for a, b in
+               ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+        return tuple(float((k + w) / self.volume) for k, w in zip(kin, o[3:9]))

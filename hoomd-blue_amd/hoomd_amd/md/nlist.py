@@ -1,0 +1,55 @@
+"""Neighbour lists (hoomd/md/nlist.py). ``Cell`` is the binned list (NeighborListGPUBinned)."""
+
+import numpy as np
+
+from .. import _lib
+
+
+class NeighborList:
+    def __init__(self, buffer, exclusions=("bond",), rebuild_check_delay=1, check_dist=True):
+        self.buffer = float(buffer)
+        self.exclusions = tuple(exclusions)
+        if rebuild_check_delay != 1 or not check_dist:
+            raise NotImplementedError(
+                "hoomd_amd checks distances every step (rebuild_check_delay=1, check_dist=True)")
+        for e in self.exclusions:
+            if e != "bond":
+                raise NotImplementedError(f"exclusion '{e}' is not supported (only 'bond')")
+        self.rebuild_check_delay = rebuild_check_delay
+        self.check_dist = check_dist
+        self._simulation = None
+
+    def _signature(self):
+        return (self.buffer, self.exclusions)
+
+    @property
+    def pair_list(self):
+        """(tag_i, tag_j) pairs with tag_i < tag_j of the current list (NeighborList.h
+        getPairList), as an array [P, 2]."""
+        sim = self._simulation
+        if sim is None or sim._system is None:
+            raise RuntimeError("pair_list is available after the simulation has run")
+        n = _lib.lib().hm_sys_get_pairs(sim._system, None, 0)
+        _lib.check(n, "pair_list")
+        out = np.zeros((max(n, 1), 2), np.uint32)
+        _lib.check(_lib.lib().hm_sys_get_pairs(sim._system, out.ctypes.data, n), "pair_list")
+        return out[:n]
+
+    @property
+    def num_builds(self):
+        sim = self._simulation
+        if sim is None or sim._system is None:
+            return 0
+        st = np.zeros(6, np.uint64)
+        _lib.check(_lib.lib().hm_sys_stats(sim._system, st.ctypes.data), "stats")
+        return int(st[0])
+
+
+class Cell(NeighborList):
+    """hoomd.md.nlist.Cell(buffer, exclusions=('bond',), rebuild_check_delay=1, check_dist=True)."""
+
+    def __init__(self, buffer, exclusions=("bond",), rebuild_check_delay=1, check_dist=True,
+                 deterministic=False, mesh=None):
+        super().__init__(buffer, exclusions, rebuild_check_delay, check_dist)
+        # the GPU build is deterministic by construction (cells ordered by particle index)
+        self.deterministic = deterministic

@@ -1,0 +1,73 @@
+"""Pair forces (hoomd/md/pair/pair.py). ``LJ`` drives the EvaluatorPairLJ kernel."""
+
+import numpy as np
+
+from .. import _lib
+from .force import Force
+from .typeparam import TypePairParameter
+
+
+class Pair(Force):
+    """Base of pair potentials: nlist, r_cut, r_on per type pair, energy shift mode
+    (pair.py:65-146). The evaluator runs in FP32 on the GPU with a full neighbour list."""
+
+    _modes = {"none": 0, "shift": 1, "xplor": 2}
+
+    def __init__(self, nlist, default_r_cut=None, default_r_on=0.0, mode="none"):
+        super().__init__()
+        if mode not in self._modes:
+            raise ValueError(f"mode must be one of {list(self._modes)}")
+        self.nlist = nlist
+        self.mode = mode
+        self.r_cut = TypePairParameter(default_r_cut)
+        self.r_on = TypePairParameter(default_r_on)
+
+    def _matrices(self, types):
+        nt = len(types)
+        rc = np.zeros((nt, nt))
+        ro = np.zeros((nt, nt))
+        for i, a in enumerate(types):
+            for j, b in enumerate(types):
+                r = self.r_cut.get(a, b)
+                if r is None:
+                    raise RuntimeError(f"r_cut for ({a}, {b}) is not set")
+                rc[i, j] = r
+                ro[i, j] = self.r_on.get(a, b, 0.0)
+        return rc, ro
+
+
+class LJ(Pair):
+    """hoomd.md.pair.LJ(nlist, default_r_cut=None, default_r_on=0., mode='none'):
+    params[(a, b)] = dict(epsilon=..., sigma=...) (EvaluatorPairLJ.h:99-155)."""
+
+    def __init__(self, nlist, default_r_cut=None, default_r_on=0.0, mode="none",
+                 tail_correction=False):
+        super().__init__(nlist, default_r_cut, default_r_on, mode)
+        if tail_correction:
+            raise NotImplementedError("tail_correction is not supported")
+        self.params = TypePairParameter()
+
+    def _signature(self):
+        return ("LJ", self.mode, self.params.signature(), self.r_cut.signature(),
+                self.r_on.signature(), self.nlist._signature())
+
+    def _attach(self, sim, h):
+        types = sim.state.particle_types
+        nt = len(types)
+        rc, ro = self._matrices(types)
+        P = (_lib.HmLJParams * (nt * nt))()
+        for i, a in enumerate(types):
+            for j, b in enumerate(types):
+                p = self.params.get(a, b)
+                if p is None:
+                    raise RuntimeError(f"params for ({a}, {b}) are not set")
+                P[i * nt + j].sigma = float(p["sigma"])
+                P[i * nt + j].epsilon = float(p["epsilon"])
+        rcf = np.ascontiguousarray(rc.ravel())
+        rof = np.ascontiguousarray(ro.ravel())
+        fid = _lib.lib().hm_sys_add_pair_lj(h, rcf.ctypes.data, rof.ctypes.data, P,
+                                            self._modes[self.mode])
+        _lib.check(fid, "pair.LJ")
+        self._force_id = fid
+        self._simulation = sim
+        self.nlist._simulation = sim

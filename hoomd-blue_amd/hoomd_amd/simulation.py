@@ -1,0 +1,196 @@
+"""Simulation and Operations (hoomd/simulation.py, hoomd/operations.py).
+
+``Simulation.run`` hands the whole step loop to the native engine (hm_sys_run): the Python layer
+only describes the system, exactly as hoomd.Simulation hands it to System::run
+(hoomd/System.cc:80-181).
+"""
+
+import time
+
+import numpy as np
+
+from . import _lib
+from .state import State
+
+
+class Operations:
+    """hoomd.Operations: integrator, tuners (a ParticleSorter by default), computes."""
+
+    def __init__(self, simulation):
+        from .tune import ParticleSorter
+        self._simulation = simulation
+        self._integrator = None
+        self.tuners = [ParticleSorter()]  # operations.py:75 adds one by default
+        self.computes = []
+        self.updaters = []
+        self.writers = []
+
+    @property
+    def integrator(self):
+        return self._integrator
+
+    @integrator.setter
+    def integrator(self, value):
+        self._integrator = value
+        self._simulation._invalidate()
+
+    def add(self, op):
+        from .md.compute import ThermodynamicQuantities
+        from .tune import ParticleSorter
+        if isinstance(op, ParticleSorter):
+            self.tuners.append(op)
+        elif isinstance(op, ThermodynamicQuantities):
+            self.computes.append(op)
+            op._simulation = self._simulation
+        else:
+            raise TypeError(f"hoomd_amd.Operations cannot hold {type(op).__name__}")
+        self._simulation._invalidate()
+
+    def __iadd__(self, op):
+        self.add(op)
+        return self
+
+    def remove(self, op):
+        for lst in (self.tuners, self.computes):
+            if op in lst:
+                lst.remove(op)
+        self._simulation._invalidate()
+
+
+class Simulation:
+    """hoomd.Simulation(device, seed)."""
+
+    def __init__(self, device, seed=None):
+        self.device = device
+        self._seed = 0 if seed is None else int(seed)
+        self._state = None
+        self._system = None
+        self._attached_key = None
+        self._timestep = 0
+        self._tps = None
+        self._walltime = 0.0
+        self._configured = False
+        self.operations = Operations(self)
+
+    # -------------------------------------------------------------- properties
+    @property
+    def seed(self):
+        return self._seed
+
+    @seed.setter
+    def seed(self, v):
+        self._seed = int(v)
+        self._invalidate()
+
+    @property
+    def state(self):
+        return self._state
+
+    @property
+    def timestep(self):
+        if self._system is not None:
+            return int(_lib.lib().hm_sys_timestep(self._system))
+        return self._timestep
+
+    @timestep.setter
+    def timestep(self, v):
+        if self._state is not None:
+            raise RuntimeError("Cannot set timestep after the state is initialised")
+        self._timestep = int(v)
+
+    @property
+    def tps(self):
+        return self._tps
+
+    @property
+    def walltime(self):
+        return self._walltime
+
+    # -------------------------------------------------------------- state
+    def create_state_from_snapshot(self, snapshot):
+        if self._state is not None:
+            raise RuntimeError("Cannot initialize more than once")
+        self._state = State(self, snapshot)
+
+    def _invalidate(self):
+        """Operations changed: the native system is rebuilt (from the current state) on next use."""
+        self._attached_key = None
+
+    def _destroy_system(self):
+        if self._system is not None:
+            self._state._sync_from_device()
+            self._timestep = int(_lib.lib().hm_sys_timestep(self._system))
+            _lib.lib().hm_sys_destroy(self._system)
+            self._system = None
+            self._configured = False
+
+    def _ensure_system(self):
+        if self._state is None:
+            raise RuntimeError("Call create_state_from_snapshot first")
+        if self._system is not None:
+            return
+        snap = self._state._snapshot
+        p = snap.particles
+        N = p.N
+        ntypes = len(p.types)
+        if ntypes > _lib.MAX_TYPES:
+            raise ValueError(f"at most {_lib.MAX_TYPES} particle types are supported")
+        pos = np.ascontiguousarray(p.position, np.float64)
+        vel = np.ascontiguousarray(p.velocity, np.float64)
+        tid = np.ascontiguousarray(p.typeid, np.uint32)
+        mass = np.ascontiguousarray(p.mass, np.float64)
+        img = np.ascontiguousarray(p.image, np.int32)
+        h = _lib.lib().hm_sys_create(N, _lib.box(snap.configuration.box[:3]), ntypes,
+                                     pos.ctypes.data, vel.ctypes.data, tid.ctypes.data,
+                                     mass.ctypes.data, img.ctypes.data, self._seed & 0xffff)
+        if not h:
+            raise _lib.HoomdAmdError(-1, _lib.lib().hm_last_error().decode())
+        self._system = h
+        self._configured = False
+        _lib.check(_lib.lib().hm_sys_set_timestep(h, self._timestep), "timestep")
+
+    def _attach(self):
+        integ = self.operations.integrator
+        if integ is None:
+            raise RuntimeError("Simulation.run needs an md.Integrator")
+        key = (id(integ), integ._signature(),
+               tuple((type(t).__name__, t._signature()) for t in self.operations.tuners))
+        for c in self.operations.computes:
+            c._simulation = self
+        if self._system is not None and self._configured and self._attached_key == key:
+            return
+        if self._system is not None and self._configured:
+            self._destroy_system()
+        self._ensure_system()
+        L = _lib.lib()
+        h = self._system
+        integ._attach(self, h)
+        sorter = [t for t in self.operations.tuners if type(t).__name__ == "ParticleSorter"]
+        if sorter:
+            s = sorter[-1]
+            _lib.check(L.hm_sys_set_sorter(h, s._period(), s._grid()), "sorter")
+        else:
+            _lib.check(L.hm_sys_set_sorter(h, 0, 256), "sorter")
+        need_virial = any(c._needs_virial() for c in self.operations.computes)
+        _lib.check(L.hm_sys_set_compute_virial(h, int(need_virial)), "virial")
+        self._attached_key = key
+        self._configured = True
+
+    # -------------------------------------------------------------- run
+    def run(self, steps, write_at_start=False):
+        steps = int(steps)
+        if steps < 0:
+            raise ValueError("steps must be non-negative")
+        self._attach()
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib().hm_sys_run(self._system, steps), "run")
+        _lib.check(_lib.lib().hm_synchronize(), "run")
+        dt = time.perf_counter() - t0
+        self._walltime += dt
+        self._tps = steps / dt if dt > 0 and steps else self._tps
+
+    def __del__(self):
+        h = getattr(self, "_system", None)
+        if h is not None and _lib._LIB is not None:
+            _lib._LIB.hm_sys_destroy(h)
+            self._system = None

@@ -84,6 +84,7 @@ int hm_free(void* ptr);
 int hm_memcpy_htod(void* dst, const void* src, size_t bytes);
 int hm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int hm_memset(void* dst, int value, size_t bytes);
+int hm_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int hm_synchronize(void);
 
 /* ------------------------------------------------------------------------------------------ */
@@ -134,7 +135,7 @@ typedef struct hm_nlist_args
     const unsigned int* n_excl; /* [N] or NULL: excluded neighbours (indices) per particle */
     const unsigned int* excl;   /* [N*max_excl] */
     unsigned int max_excl;
-    unsigned int nmax;          /* row length of nlist (multiple of 4) */
+    unsigned int nmax;          /* row length of nlist (multiple of 8) */
     unsigned int* nlist;        /* [N*nmax] */
     unsigned int* n_neigh;      /* [N] */
     unsigned int* conditions;   /* [1] atomicMax'd: largest n_neigh when it exceeds nmax */
@@ -288,6 +289,8 @@ int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
  * chunk boundary on data errors). */
 int hm_sys_run(hm_system* s, uint64_t nsteps);
 uint64_t hm_sys_timestep(const hm_system* s);
+/* Set the current timestep (before the first run, or to continue a trajectory). */
+int hm_sys_set_timestep(hm_system* s, uint64_t timestep);
 /* Copy out state by tag (each pointer may be NULL): pos3, vel3, image3, accel3. */
 int hm_sys_get_state(hm_system* s, double* pos3, double* vel3, int* image3, double* accel3);
 /* Replace positions/velocities/images by tag (forces a neighbour-list rebuild). */
@@ -306,6 +309,11 @@ int hm_sys_stats(hm_system* s, uint64_t* out6);
 int hm_sys_thermalize(hm_system* s, double kT);
 /* Launch stream used by the engine (a hipStream_t). */
 hm_stream_t hm_sys_stream(hm_system* s);
+/* Benchmark helper (no counterpart in the reference): average duration in ms of the pair-force
+ * kernel of force force_id over iters back-to-back launches on the engine stream, timed with
+ * HIP events on that stream, and the number of neighbour-list entries it reads. */
+int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg_ms,
+                     uint64_t* nlist_entries);
 
 #ifdef __cplusplus
 }

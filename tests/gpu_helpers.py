@@ -83,7 +83,7 @@ def nlist(pos, L, r_cut, r_buff, nmax=None, cmax=64, excl=None, types=None, ntyp
     if nmax is None:
         rho = N / float(np.prod(L3))
         nmax = int(np.ceil(rho * 4 / 3 * np.pi * width**3 * 1.5 + 16))
-        nmax = (nmax + 3) & ~3
+        nmax = (nmax + 7) & ~7
     d_nl = H.DeviceArray((N * nmax,), np.uint32, fill=0xff)
     d_nn = H.DeviceArray((N,), np.uint32, fill=0)
     keep = [d_pos, d_size, d_ent, d_adj, d_nl, d_nn, d_cond]

@@ -1,0 +1,241 @@
+"""GPU parity of whole timesteps (the native engine behind hoomd_amd.Simulation) against the
+oracle's CPU step loop on identical states.
+
+FP32 pair forces make trajectories drift apart slowly from the FP64 oracle (the dynamics is
+chaotic), so trajectories are compared over short horizons with stated tolerances; lists and
+first forces are compared exactly / at 1e-5 relative RMS.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_helpers import rel_rms
+from lattices import maxwell_boltzmann, perturbed_fcc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hoomd():
+    import hoomd_amd
+    from hoomd_amd import _lib
+    if _lib.lib().hm_device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+    return hoomd_amd
+
+
+def make_sim(hoomd, pos, vel, L, *, r_cut=2.5, mode="none", buffer=0.4, dt=0.005,
+             thermostat=None, langevin=None, sorter=None, seed=1, bonds=None, fene=None):
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = len(pos)
+    snap.particles.types = ["A"]
+    snap.particles.position[:] = pos
+    snap.particles.velocity[:] = vel
+    if bonds is not None:
+        snap.bonds.N = len(bonds)
+        snap.bonds.types = ["A-A"]
+        snap.bonds.group[:] = bonds
+    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
+    sim.create_state_from_snapshot(snap)
+    nl = hoomd.md.nlist.Cell(buffer=buffer)
+    lj = hoomd.md.pair.LJ(nlist=nl, default_r_cut=r_cut, mode=mode)
+    lj.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0)
+    forces = [lj]
+    if fene is not None:
+        fb = hoomd.md.bond.FENEWCA()
+        fb.params["A-A"] = fene
+        forces.append(fb)
+    if langevin is not None:
+        method = hoomd.md.methods.Langevin(hoomd.filter.All(), kT=langevin[0],
+                                           default_gamma=langevin[1])
+    else:
+        method = hoomd.md.methods.ConstantVolume(hoomd.filter.All(), thermostat=thermostat)
+    sim.operations.integrator = hoomd.md.Integrator(dt=dt, forces=forces, methods=[method])
+    sim.operations.tuners.clear()
+    if sorter:
+        sim.operations.add(hoomd.tune.ParticleSorter(trigger=sorter[0], grid=sorter[1]))
+    return sim, lj
+
+
+def lj_state(n=8, kT=1.2, seed=3):
+    pos, L = perturbed_fcc(n, 0.8442, r=0.05, seed=seed)
+    vel = maxwell_boltzmann(len(pos), kT, seed=seed)
+    return pos, vel, L
+
+
+def test_first_forces_match_fp64(hoomd):
+    pos, vel, L = lj_state(10)
+    sim, lj = make_sim(hoomd, pos, vel, L)
+    sim.run(0)
+    nl, nn = O.nlist_binned(O.Box(L), pos, 2.5, 0.4, full=True)
+    rf, rv = O.pair_lj(O.Box(L), pos, nl, nn, third_law=False)
+    f = lj.forces
+    e = lj.energies
+    v = lj.virials
+    assert rel_rms(f, rf[:, :3]) < 1e-5
+    assert rel_rms(e, rf[:, 3]) < 1e-5
+    assert rel_rms(v.T, rv) < 1e-5
+    assert abs(lj.energy - rf[:, 3].sum()) / abs(rf[:, 3].sum()) < 1e-6
+
+
+def test_pair_list_matches_oracle(hoomd):
+    pos, vel, L = lj_state(8)
+    sim, lj = make_sim(hoomd, pos, vel, L)
+    sim.run(0)
+    got = lj.nlist.pair_list.astype(np.int64)
+    got = got[np.lexsort((got[:, 1], got[:, 0]))]
+    onl, onn = O.nlist_binned(O.Box(L), pos, 2.5, 0.4, full=False)
+    assert np.array_equal(got, O.pair_array(onl, onn))
+
+
+@pytest.mark.parametrize("use_sorter", [False, True])
+def test_nve_trajectory_matches_oracle(hoomd, use_sorter):
+    pos, vel, L = lj_state(8)
+    sim, lj = make_sim(hoomd, pos, vel, L, sorter=(20, 32) if use_sorter else None)
+    sim.run(60)
+    ref = O.System(pos, vel, L, dt=0.005)
+    if use_sorter:
+        ref.set_sorter(20, 32)
+    ref.run(60)
+    rp, rv, ri = ref.state()
+    snap = sim.state.get_snapshot()
+    gp = snap.particles.position + snap.particles.image * L
+    assert np.abs(gp - (rp + ri * L)).max() < 1e-6
+    assert np.abs(snap.particles.velocity - rv).max() < 1e-5
+    assert sim.timestep == 60
+
+
+def test_graph_replay_equals_eager(hoomd):
+    """Chunks captured as hipGraphs and replayed give bit-identical trajectories to eager
+    launches (deterministic kernels)."""
+    from hoomd_amd import _lib
+    pos, vel, L = lj_state(8)
+    out = []
+    for graph in (True, False):
+        sim, lj = make_sim(hoomd, pos, vel, L, thermostat=hoomd.md.methods.thermostats.Bussi(kT=1.2))
+        sim.run(0)
+        _lib.check(_lib.lib().hm_sys_set_chunk(sim._system, 7, int(graph)))
+        sim.run(50)
+        s = sim.state.get_snapshot()
+        out.append((s.particles.position.copy(), s.particles.velocity.copy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_bussi_trajectory_and_temperature(hoomd):
+    pos, vel, L = lj_state(8, kT=2.0)
+    sim, lj = make_sim(hoomd, pos, vel, L, thermostat=hoomd.md.methods.thermostats.Bussi(kT=1.2),
+                       seed=7)
+    sim.run(10)
+    ref = O.System(pos, vel, L, dt=0.005, seed=7)
+    ref.set_bussi(1.2, 0.0)
+    ref.run(10)
+    _, rv, _ = ref.state()
+    v = sim.state.get_snapshot().particles.velocity
+    assert np.abs(v - rv).max() < 1e-6
+    thermo = hoomd.md.compute.ThermodynamicQuantities(hoomd.filter.All())
+    sim.operations.add(thermo)
+    ts = []
+    for _ in range(30):
+        sim.run(10)
+        ts.append(thermo.kinetic_temperature)
+    assert abs(np.mean(ts) / 1.2 - 1.0) < 0.02
+
+
+def test_thermo_matches_oracle(hoomd):
+    pos, vel, L = lj_state(8)
+    sim, lj = make_sim(hoomd, pos, vel, L, mode="shift")
+    thermo = hoomd.md.compute.ThermodynamicQuantities(hoomd.filter.All())
+    sim.operations.add(thermo)
+    sim.run(0)
+    nl, nn = O.nlist_binned(O.Box(L), pos, 2.5, 0.4, full=True)
+    rf, rv = O.pair_lj(O.Box(L), pos, nl, nn, third_law=False, mode="shift")
+    ke = O.kinetic_energy(vel, np.ones(len(pos)))
+    assert abs(thermo.kinetic_energy - ke) < 1e-9 * ke
+    assert abs(thermo.potential_energy - rf[:, 3].sum()) < 1e-5 * abs(rf[:, 3].sum())
+    W = (rv[0].sum() + rv[3].sum() + rv[5].sum()) / 3
+    P = (2 * ke / 3 + W) / L**3
+    assert abs(thermo.pressure - P) < 1e-5 * abs(P) + 1e-8
+
+
+def test_langevin_bead_spring_matches_oracle(hoomd):
+    rng = np.random.default_rng(5)
+    nch, M = 120, 10  # chains (9.7 long) shorter than the box (11.2)
+    N = nch * M
+    L = (N / 0.85) ** (1 / 3)
+    pos = np.zeros((N, 3))
+    bonds = []
+    # chains on a grid of straight lines along z (no overlaps), bond 0.97
+    side = int(np.ceil(nch ** 0.5))
+    for c in range(nch):
+        x0 = np.array([(c % side) * L / side, (c // side) * L / side, 0.0]) - L / 2 + 0.3
+        for k in range(M):
+            i = c * M + k
+            pos[i] = x0 + [0, 0, 0.97 * k]
+            if k:
+                bonds.append((i - 1, i))
+    pos = (pos + L / 2) % L - L / 2
+    pos += rng.uniform(-0.02, 0.02, pos.shape)
+    pos = (pos + L / 2) % L - L / 2
+    vel = maxwell_boltzmann(N, 1.0, seed=5)
+    fene = dict(k=30.0, r0=1.5, epsilon=1.0, sigma=1.0, delta=0.0)
+    sim, lj = make_sim(hoomd, pos, vel, L, r_cut=2 ** (1 / 6), mode="shift", dt=0.002,
+                       langevin=(1.0, 1.0), seed=11, bonds=np.array(bonds), fene=fene)
+    sim.run(20)
+    ref = O.System(pos, vel, L, r_cut=2 ** (1 / 6), mode="shift", dt=0.002, seed=11)
+    ref.set_langevin(1.0, 1.0)
+    ref.set_bonds(np.array(bonds), **fene)
+    ref.run(20)
+    rp, rv, ri = ref.state()
+    snap = sim.state.get_snapshot()
+    gp = snap.particles.position + snap.particles.image * L
+    assert np.abs(gp - (rp + ri * L)).max() < 1e-6
+    assert np.abs(snap.particles.velocity - rv).max() < 1e-4
+    # bonded pairs are excluded from the pair list
+    pl = {tuple(p) for p in lj.nlist.pair_list.tolist()}
+    assert not any((a, b) in pl for a, b in bonds)
+
+
+def test_nve_energy_conservation(hoomd):
+    pos, vel, L = lj_state(8, kT=1.2)
+    sim, lj = make_sim(hoomd, pos, vel, L, mode="shift", dt=0.002, sorter=(500, 64))
+    thermo = hoomd.md.compute.ThermodynamicQuantities(hoomd.filter.All())
+    sim.operations.add(thermo)
+    sim.run(100)
+    e0 = thermo.kinetic_energy + thermo.potential_energy
+    es = []
+    for _ in range(20):
+        sim.run(250)
+        es.append(thermo.kinetic_energy + thermo.potential_energy)
+    N = len(pos)
+    drift = np.abs(np.array(es) - e0).max() / N
+    print("max |E - E0| / N over 5000 steps:", drift)
+    assert drift < 1e-3
+
+
+def test_thermalize_matches_oracle(hoomd):
+    pos, vel, L = lj_state(6)
+    sim, lj = make_sim(hoomd, pos, np.zeros_like(vel), L, seed=9)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), 1.5)
+    v = sim.state.get_snapshot().particles.velocity
+    ref = O.thermalize(np.ones(len(pos)), np.arange(len(pos)), 1.5, 0, 9)
+    assert np.allclose(v, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_errors_are_reported(hoomd):
+    from hoomd_amd import _lib
+    pos, vel, L = lj_state(4)
+    bad = pos.copy()
+    sim, lj = make_sim(hoomd, bad, vel * 0, L)
+    sim.run(1)
+    snap = sim.state.get_snapshot()
+    snap.particles.position[3] = [np.nan, 0, 0]
+    sim.state.set_snapshot(snap)
+    with pytest.raises(_lib.HoomdAmdError):
+        sim.run(1)
+    # Bussi with zero kinetic energy
+    sim2, _ = make_sim(hoomd, pos, vel * 0, L, thermostat=hoomd.md.methods.thermostats.Bussi(kT=1.0))
+    with pytest.raises(_lib.HoomdAmdError):
+        sim2.run(5)

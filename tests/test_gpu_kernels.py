@@ -163,7 +163,7 @@ def test_pair_lj_golden_two_particles(H, golden):
     for p, forces, energies in zip(lj["params"], lj["forces"], lj["energies"]):
         for i, d in enumerate(fx["distances"]):
             pos = np.array([[0.0, 0.0, 0.1], [0.0, 0.0, d + 0.1]])
-            nl = np.array([[1, 0, 0, 0], [0, 0, 0, 0]], np.uint32)
+            nl = np.array([[1] + [0] * 7, [0] * 8], np.uint32)
             nn = np.array([1, 1], np.uint32)
             f, _ = pair_lj(pos, 20.0, nl, nn, r_cut=fx["r_cut"], sigma=p["sigma"],
                            epsilon=p["epsilon"])
