@@ -133,8 +133,8 @@ int hm_cell_list_build(const hm_cell_args* a, hm_stream_t stream)
               "hm_cell_list_build");
     if (r)
         return r;
-    return check(launch_cell_sort((unsigned int)ncell, a->cmax, a->cell_size, a->entries, nullptr,
-                                  s),
+    return check(launch_cell_sort((unsigned int)ncell, a->cmax, a->cell_size, a->entries,
+                                  a->cell_idx, nullptr, s),
                  "hm_cell_list_build: order");
 }
 
@@ -177,8 +177,52 @@ int hm_nlist_build(const hm_nlist_args* a, hm_stream_t stream)
     k.nmax = a->nmax;
     k.nlist = a->nlist;
     k.n_neigh = a->n_neigh;
+    k.lrow = a->local_rows;
     k.cond = a->conditions;
     return check(launch_nlist_build(k, (hipStream_t)stream), "hm_nlist_build");
+}
+
+int hm_pair_lj_cells(const hm_pair_cell_args* a, const hm_lj_params* params, hm_stream_t stream)
+{
+    if (!a || !params || !a->pos || !a->cell_size || !a->cell_idx || !a->cell_adj
+        || !a->local_rows || !a->n_neigh || !a->force || (a->compute_virial && !a->virial)
+        || a->ntypes == 0 || a->ntypes > HM_MAX_TYPES || a->nmax == 0 || a->cmax == 0
+        || a->cmax > 256 || a->dim[0] == 0 || a->dim[1] == 0 || a->dim[2] == 0)
+        return fail(HM_ERR_ARGS, "hm_pair_lj_cells: invalid arguments");
+    if (a->tpp && a->tpp != 4 && a->tpp != 8 && a->tpp != 16)
+        return fail(HM_ERR_ARGS, "hm_pair_lj_cells: tpp must be 4, 8 or 16");
+    if (a->shift_mode < 0 || a->shift_mode > 2)
+        return fail(HM_ERR_ARGS, "hm_pair_lj_cells: shift_mode must be 0, 1 or 2");
+    PairCellArgs c;
+    memset(&c, 0, sizeof(c));
+    c.N = a->N;
+    c.pos = (const double4*)a->pos;
+    c.box = make_devbox(a->box);
+    for (int d = 0; d < 3; d++)
+        c.dim[d] = a->dim[d];
+    c.ncell = a->dim[0] * a->dim[1] * a->dim[2];
+    c.cmax = a->cmax;
+    c.cell_size = a->cell_size;
+    c.cell_idx = a->cell_idx;
+    c.cell_adj = a->cell_adj;
+    c.n_adj = a->n_adj;
+    c.lrow = a->local_rows;
+    c.nmax = a->nmax;
+    c.n_neigh = a->n_neigh;
+    c.ntypes = a->ntypes;
+    for (unsigned int q = 0; q < a->ntypes * a->ntypes; q++)
+    {
+        const double rc = a->r_cut[q] > 0.0 ? a->r_cut[q] : 0.0;
+        c.rcutsq[q] = rc * rc;
+        c.rcutsq_f[q] = (float)(rc * rc);
+        c.ronsq_f[q] = (float)(a->r_on[q] * a->r_on[q]);
+    }
+    c.shift_mode = a->shift_mode;
+    c.compute_virial = a->compute_virial;
+    c.force = (float4*)a->force;
+    c.virial = a->virial;
+    PairLaunch cfg{a->tpp, a->workgroups};
+    return check(launch_pair_cell_lj(c, params, cfg, (hipStream_t)stream), "hm_pair_lj_cells");
 }
 
 int hm_nlist_check(unsigned int N, const double* pos, const double* last_pos, hm_box box,

@@ -107,6 +107,8 @@ struct hm_system
     unsigned int ncell = 0, n_adj = 0, cmax = 0, nmax = 0;
     unsigned int *cell_size = nullptr, *cell_adj = nullptr, *nlist = nullptr, *n_neigh = nullptr;
     hm_cell_entry* entries = nullptr;
+    unsigned int* cell_idx = nullptr;
+    unsigned short* lrow = nullptr;
     double rcut_mat[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     double rcut_max = 0.0;
     unsigned int max_excl = 0;
@@ -142,7 +144,9 @@ struct hm_system
     uint64_t timestep = 0;
     bool prepared = false;
     int compute_virial = 0;
-    PairLaunch pair_cfg{4, 256};
+    PairLaunch pair_cfg{4, 512};      // gather kernel: lanes per particle, block size
+    PairLaunch pair_cell_cfg{8, 0};   // cell-tiled kernel: lanes per particle, workgroups
+    int pair_kernel_kind = 0;         // 0 cell-tiled (default), 1 neighbour-index gather
     unsigned int chunk = 100;
     bool use_graph = true;
     std::map<unsigned int, hipGraphExec_t> graphs;
@@ -184,6 +188,8 @@ struct hm_system
         dfree(nlist);
         dfree(n_neigh);
         dfree(entries);
+        dfree(cell_idx);
+        dfree(lrow);
         dfree(n_excl_tag);
         dfree(excl_tag);
         dfree(n_excl);
@@ -275,17 +281,14 @@ struct hm_system
         if (cmax == 0)
         {
             cmax = (unsigned int)std::min(256.0, std::ceil(rho * w3 * 2.0 + 16.0));
-            dfree(entries);
-            entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
+            alloc_cells();
         }
         if (nmax == 0)
         {
             double expect = rho * 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
             nmax = (unsigned int)std::ceil(expect * 1.3 + 16.0);
             nmax = (nmax + 7) & ~7u;
-            dfree(nlist);
-            nlist = dalloc<unsigned int>((size_t)N * nmax);
-            hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+            alloc_rows();
         }
         if (!n_neigh)
         {
@@ -303,9 +306,29 @@ struct hm_system
         if (c <= cmax)
             c = std::min(256u, cmax + 8);
         cmax = c;
-        dfree(entries);
-        entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
+        alloc_cells();
         clear_graphs();
+    }
+
+    void alloc_cells()
+    {
+        dfree(entries);
+        dfree(cell_idx);
+        entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
+        cell_idx = dalloc<unsigned int>((size_t)ncell * cmax);
+        hchk(hipMemsetAsync(cell_idx, 0, sizeof(unsigned int) * ncell * cmax, stream), "clear");
+    }
+
+    // neighbour rows (32-bit particle indices) and local rows (16-bit staged positions);
+    // zero-filled so that stale rows never hold an out-of-range index
+    void alloc_rows()
+    {
+        dfree(nlist);
+        dfree(lrow);
+        nlist = dalloc<unsigned int>((size_t)N * nmax);
+        lrow = dalloc<unsigned short>((size_t)N * nmax);
+        hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+        hchk(hipMemsetAsync(lrow, 0, sizeof(unsigned short) * N * nmax, stream), "clear nlist");
     }
 
     void grow_nmax(unsigned int need)
@@ -318,9 +341,7 @@ struct hm_system
         if (m <= nmax)
             m = nmax + 8;
         nmax = m;
-        dfree(nlist);
-        nlist = dalloc<unsigned int>((size_t)N * nmax);
-        hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+        alloc_rows();
         clear_graphs();
     }
 
@@ -362,23 +383,61 @@ struct hm_system
         k.nmax = nmax;
         k.nlist = nlist;
         k.n_neigh = n_neigh;
+        k.lrow = lrow;
         k.cond = &cond->nlist_need;
         k.gate = gated ? st : nullptr;
         k.counter = st;
     }
 
     // ------------------------------------------------------------------ kernels of one step
+    void launch_pair(ForceSlot& f, int with_virial)
+    {
+        if (pair_kernel_kind == 1)
+        {
+            PairKernelArgs k;
+            fill_pair_args(k, N, pos, hbox, nlist, nmax, n_neigh, ntypes, f.rcut, f.ron, f.mode,
+                           with_virial, f.f, f.v);
+            hchk(launch_pair_lj(k, f.p, pair_cfg, stream), "pair force");
+            return;
+        }
+        PairCellArgs c;
+        memset(&c, 0, sizeof(c));
+        c.N = N;
+        c.pos = pos;
+        c.box = box;
+        c.dim[0] = dim[0];
+        c.dim[1] = dim[1];
+        c.dim[2] = dim[2];
+        c.ncell = ncell;
+        c.cmax = cmax;
+        c.cell_size = cell_size;
+        c.cell_idx = cell_idx;
+        c.cell_adj = cell_adj;
+        c.n_adj = n_adj;
+        c.lrow = lrow;
+        c.nmax = nmax;
+        c.n_neigh = n_neigh;
+        c.ntypes = ntypes;
+        for (unsigned int q = 0; q < ntypes * ntypes; q++)
+        {
+            const double rc = f.rcut[q] > 0.0 ? f.rcut[q] : 0.0;
+            c.rcutsq[q] = rc * rc;
+            c.rcutsq_f[q] = (float)(rc * rc);
+            c.ronsq_f[q] = (float)(f.ron[q] * f.ron[q]);
+        }
+        c.shift_mode = f.mode;
+        c.compute_virial = with_virial;
+        c.force = f.f;
+        c.virial = f.v;
+        hchk(launch_pair_cell_lj(c, f.p, pair_cell_cfg, stream), "pair force");
+    }
+
     void launch_forces(int with_virial)
     {
         for (auto& f : forces)
         {
             if (f.kind == 0)
-            {
-                PairKernelArgs k;
-                fill_pair_args(k, N, pos, hbox, nlist, nmax, n_neigh, ntypes, f.rcut, f.ron,
-                               f.mode, with_virial, f.f, f.v);
-                hchk(launch_pair_lj(k, f.p, pair_cfg, stream), "pair force");
-            }
+                launch_pair(f, with_virial);
             else
             {
                 BondKernelArgs k;
@@ -441,16 +500,16 @@ struct hm_system
             a1.last = last;
             a1.maxsq = r_buff < 1e-6 ? -1.0 : distance_check_maxsq(rcut_max, r_buff);
             a1.flag_state = st;
-            a1.cell_size = cell_size;
-            a1.ncell = ncell;
         }
         hchk(launch_step_one(a1, stream), "step one");
         if (have_nl)
         {
+            hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
                                    &cond->cell_need, st, stream),
                  "cell build");
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, st, stream), "cell order");
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, st, stream),
+                 "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, true);
             hchk(launch_nlist_build(k, stream), "nlist build");
@@ -501,7 +560,8 @@ struct hm_system
                 grow_cmax(h_cond->cell_need);
                 continue;
             }
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, nullptr, stream), "cell order");
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, nullptr, stream),
+                 "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, false);
             k.counter = nullptr;
@@ -1030,6 +1090,24 @@ int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_si
     return HM_OK;
 }
 
+int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned int size)
+{
+    if (!s || kind < 0 || kind > 1)
+        return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: invalid arguments");
+    if (kind == 0 && tpp && tpp != 4 && tpp != 8 && tpp != 16)
+        return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: cell kernel tpp must be 4, 8 or 16");
+    s->pair_kernel_kind = kind;
+    if (kind == 0)
+    {
+        if (tpp)
+            s->pair_cell_cfg.tpp = tpp;
+        s->pair_cell_cfg.block = size;
+        s->clear_graphs();
+        return HM_OK;
+    }
+    return hm_sys_set_pair_launch(s, tpp, size);
+}
+
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph)
 {
     if (!s)
@@ -1310,16 +1388,13 @@ int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg
         if (!s->prepared)
             s->prep();
         auto& f = s->forces[force_id];
-        PairKernelArgs k;
-        fill_pair_args(k, s->N, s->pos, s->hbox, s->nlist, s->nmax, s->n_neigh, s->ntypes, f.rcut,
-                       f.ron, f.mode, s->compute_virial, f.f, f.v);
         hipEvent_t e0, e1;
         hchk(hipEventCreate(&e0), "event");
         hchk(hipEventCreate(&e1), "event");
-        hchk(launch_pair_lj(k, f.p, s->pair_cfg, s->stream), "pair warm-up");
+        s->launch_pair(f, s->compute_virial);
         hchk(hipEventRecord(e0, s->stream), "event");
         for (unsigned int q = 0; q < iters; q++)
-            hchk(launch_pair_lj(k, f.p, s->pair_cfg, s->stream), "pair");
+            s->launch_pair(f, s->compute_virial);
         hchk(hipEventRecord(e1, s->stream), "event");
         hchk(hipEventSynchronize(e1), "event");
         float ms = 0.f;

@@ -61,7 +61,8 @@ hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStre
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s);
+                             unsigned int* cond3, const StepState* gate, hipStream_t s,
+                             float4* rel = nullptr);
 
 struct NlistKernelArgs
 {
@@ -86,13 +87,47 @@ struct NlistKernelArgs
     unsigned int nmax;
     unsigned int* nlist;
     unsigned int* n_neigh;
+    unsigned short* lrow;     // optional: local rows (position in the home cell's candidates)
     unsigned int* cond;       // needed nmax
     const StepState* gate;
     StepState* counter;       // n_builds incremented by block 0 when non-NULL
 };
+
+// Cell-tiled pair kernel: neighbours read from the home cell's LDS-staged candidates through
+// the local rows written by the neighbour-list build.
+struct PairCellArgs
+{
+    unsigned int N;
+    const double4* pos;
+    DevBox box;
+    unsigned int dim[3];
+    unsigned int ncell;
+    unsigned int cmax;
+    const unsigned int* cell_size;
+    const unsigned int* cell_idx;   // [ncell*cmax] particle index of each slot (index order)
+    const unsigned int* cell_adj;
+    unsigned int n_adj;
+    const unsigned short* lrow;     // [N*nmax]
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double rcutsq[HM_MAX_TYPES * HM_MAX_TYPES];
+    float rcutsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float ronsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;
+    int compute_virial;
+    float4* force;
+    float* virial;
+};
+hipError_t launch_pair_cell_lj(const PairCellArgs& a, const hm_lj_params* params, PairLaunch cfg,
+                               hipStream_t s);
+// clear the cell occupancy counters when a rebuild is due (gate) or unconditionally (NULL)
+hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const StepState* gate,
+                             hipStream_t s);
 // order each cell's slots by particle index (deterministic candidate order)
 hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
-                            hm_cell_entry* entries, const StepState* gate, hipStream_t s);
+                            hm_cell_entry* entries, unsigned int* cell_idx, const StepState* gate,
+                            hipStream_t s);
 hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
 
 hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,
@@ -119,6 +154,51 @@ struct PairKernelArgs
 };
 hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params,
                           PairLaunch cfg, hipStream_t s);
+
+// Reference-cell relative positions: rel.xyz = position - origin of the cell the particle was
+// binned into at the last build (minimum image, FP32); rel.w = bits of
+// cx | cy << 8 | cz << 16 | type << 24. Needs 3 <= dim <= 255 per axis.
+struct RelFrame
+{
+    float4* rel;
+    unsigned int dim[3];
+    double w[3];      // cell widths (FP64)
+    float wf[3];      // cell widths rounded to FP32
+};
+
+__host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, unsigned int cz,
+                                              unsigned int type)
+{
+    return cx | (cy << 8) | (cz << 16) | (type << 24);
+}
+
+// Pair kernel reading one 16-byte relative position per neighbour (see RelFrame).
+struct PairRelArgs
+{
+    unsigned int N;
+    const double4* pos;      // FP64 positions for the guard-band re-decision of the cutoff
+    const float4* rel;
+    DevBox box;
+    unsigned int dim[3];
+    float wf[3];
+    const unsigned int* nlist;
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double rcutsq[HM_MAX_TYPES * HM_MAX_TYPES];
+    float rcutsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float ronsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float guard[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;
+    int compute_virial;
+    float4* force;
+    float* virial;
+};
+hipError_t launch_pair_rel_lj(const PairRelArgs& a, const hm_lj_params* params, PairLaunch cfg,
+                              hipStream_t s);
+// (re)write rel for every particle from its current position and the cell stored in rel.w
+hipError_t launch_rel_refresh(unsigned int N, const double4* pos, const DevBox& box,
+                              const RelFrame& f, hipStream_t s);
 
 // ---------------------------------------------------------------- bond
 struct BondKernelArgs
@@ -157,6 +237,8 @@ struct StepOneArgs
     const d3* last;
     double maxsq;
     StepState* flag_state;     // rebuild_at = t + 1 when a particle moved too far
+    // fused refresh of the reference-cell relative positions (rel.rel may be NULL)
+    RelFrame rel;
     // fused clearing of the cell occupancy counters
     unsigned int* cell_size;
     unsigned int ncell;

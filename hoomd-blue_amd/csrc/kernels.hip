@@ -64,6 +64,35 @@ hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStre
     return hipGetLastError();
 }
 
+// ======================================================================== persistent grids
+// Kernels that only run on a neighbour-list rebuild are launched every step and exit at once
+// when the device-side rebuild flag is not set. They walk their work in grid-stride loops over a
+// bounded grid, so an idle step costs a few hundred empty workgroups instead of one per cell.
+constexpr unsigned int GATED_BLOCKS = 2048;
+
+__device__ __forceinline__ bool gate_closed(const StepState* gate)
+{
+    return gate && gate->rebuild_at != gate->timestep + 1;
+}
+
+// XCD-aware persistent loop: workgroup b runs on XCD (b % 8); it visits the items of that XCD's
+// contiguous eighth of [0, n) with stride gridDim.x / 8.
+struct XcdRange
+{
+    unsigned int begin, end, stride;
+};
+
+__device__ __forceinline__ XcdRange xcd_range(unsigned int n)
+{
+    const unsigned int xcd = blockIdx.x & 7u, r = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const unsigned int chunk = (n + 7) / 8;
+    XcdRange x;
+    x.begin = xcd * chunk + r;
+    x.end = min(n, (xcd + 1) * chunk);
+    x.stride = per;
+    return x;
+}
+
 // ======================================================================== cell build
 // CellList::computeCellList (hoomd/CellList.cc:404-547) / gpu_compute_cell_list_kernel
 // (hoomd/CellListGPU.cu:42-171): FP64 makeFraction, truncation to the cell grid, wrap at dim,
@@ -76,56 +105,57 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
                                                          unsigned int* cond, const StepState* gate)
 {
 #pragma clang fp contract(off)
-    if (gate && gate->rebuild_at != gate->timestep + 1)
+    if (gate_closed(gate))
         return;
-    unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N)
-        return;
-    double4 p = pos[n];
-    if (last)
-        last[n] = d3{p.x, p.y, p.z};
-    if (isnan(p.x) || isnan(p.y) || isnan(p.z))
+    for (unsigned int n = blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += gridDim.x * blockDim.x)
     {
-        atomicMax(&cond[1], n + 1);
-        return;
+        double4 p = pos[n];
+        if (last)
+            last[n] = d3{p.x, p.y, p.z};
+        if (isnan(p.x) || isnan(p.y) || isnan(p.z))
+        {
+            atomicMax(&cond[1], n + 1);
+            continue;
+        }
+        double fx = (p.x - box.lo[0]) * box.Linv[0];
+        double fy = (p.y - box.lo[1]) * box.Linv[1];
+        double fz = (p.z - box.lo[2]) * box.Linv[2];
+        int ib = (int)(fx * (double)dx);
+        int jb = (int)(fy * (double)dy);
+        int kb = (int)(fz * (double)dz);
+        if (fx < -0.00001 || fx >= 1.00001 || fy < -0.00001 || fy >= 1.00001 || fz < -0.00001
+            || fz >= 1.00001)
+        {
+            atomicMax(&cond[2], n + 1);
+            continue;
+        }
+        if (ib == (int)dx)
+            ib = 0;
+        if (jb == (int)dy)
+            jb = 0;
+        if (kb == (int)dz)
+            kb = 0;
+        if (ib < 0 || ib >= (int)dx || jb < 0 || jb >= (int)dy || kb < 0 || kb >= (int)dz)
+        {
+            atomicMax(&cond[2], n + 1);
+            continue;
+        }
+        unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
+        unsigned int slot = atomicAdd(&cell_size[bin], 1u);
+        if (slot < cmax)
+        {
+            hm_cell_entry e;
+            e.x = p.x;
+            e.y = p.y;
+            e.z = p.z;
+            e.idx = n;
+            e.type = (uint32_t)p.w;
+            entries[(size_t)bin * cmax + slot] = e;
+        }
+        else
+            atomicMax(&cond[0], slot + 1);
     }
-    double fx = (p.x - box.lo[0]) * box.Linv[0];
-    double fy = (p.y - box.lo[1]) * box.Linv[1];
-    double fz = (p.z - box.lo[2]) * box.Linv[2];
-    int ib = (int)(fx * (double)dx);
-    int jb = (int)(fy * (double)dy);
-    int kb = (int)(fz * (double)dz);
-    if (fx < -0.00001 || fx >= 1.00001 || fy < -0.00001 || fy >= 1.00001 || fz < -0.00001
-        || fz >= 1.00001)
-    {
-        atomicMax(&cond[2], n + 1);
-        return;
-    }
-    if (ib == (int)dx)
-        ib = 0;
-    if (jb == (int)dy)
-        jb = 0;
-    if (kb == (int)dz)
-        kb = 0;
-    if (ib < 0 || ib >= (int)dx || jb < 0 || jb >= (int)dy || kb < 0 || kb >= (int)dz)
-    {
-        atomicMax(&cond[2], n + 1);
-        return;
-    }
-    unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
-    unsigned int slot = atomicAdd(&cell_size[bin], 1u);
-    if (slot < cmax)
-    {
-        hm_cell_entry e;
-        e.x = p.x;
-        e.y = p.y;
-        e.z = p.z;
-        e.idx = n;
-        e.type = (uint32_t)p.w;
-        entries[(size_t)bin * cmax + slot] = e;
-    }
-    else
-        atomicMax(&cond[0], slot + 1);
 }
 
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
@@ -135,8 +165,29 @@ hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& b
 {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(cell_build_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, box,
-                       dim[0], dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
+    unsigned int blocks = min((N + 255) / 256, GATED_BLOCKS);
+    hipLaunchKernelGGL(cell_build_kernel, dim3(blocks), dim3(256), 0, s, N, pos, box, dim[0],
+                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
+    return hipGetLastError();
+}
+
+__global__ void cell_clear_kernel(unsigned int ncell, unsigned int* cell_size,
+                                  const StepState* gate)
+{
+    if (gate_closed(gate))
+        return;
+    for (unsigned int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += gridDim.x * blockDim.x)
+        cell_size[c] = 0;
+}
+
+hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const StepState* gate,
+                             hipStream_t s)
+{
+    if (ncell == 0)
+        return hipSuccess;
+    unsigned int blocks = min((ncell + 255) / 256, 256u);
+    hipLaunchKernelGGL(cell_clear_kernel, dim3(blocks), dim3(256), 0, s, ncell, cell_size, gate);
     return hipGetLastError();
 }
 
@@ -147,7 +198,8 @@ hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& b
 //    order (adjacency cell order, then index) equals the CPU list's;
 //  * one workgroup per home cell stages the adjacent cells' particles in LDS, one wavefront per
 //    home particle sweeps the staged candidates 64 at a time, and ballot + mbcnt compaction
-//    writes the hits contiguously into the particle's row.
+//    writes the hits contiguously into the particle's row (and, for the cell-tiled pair kernel,
+//    the hit's 16-bit position in the home cell's candidate sequence).
 // Inclusion is dr^2 <= (r_cut + r_buff)^2 decided exactly as the FP64 CPU path decides it. The
 // fast kernel tests in FP32 on coordinates relative to the home-cell centre and re-decides with
 // the CPU's FP64 formula every candidate whose FP32 distance lies within a guard band of the
@@ -160,47 +212,56 @@ constexpr int NLHOME = 256;
 __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsigned int cmax,
                                                         const unsigned int* __restrict__ cell_size,
                                                         hm_cell_entry* entries,
+                                                        unsigned int* cell_idx,
                                                         const StepState* gate)
 {
     __shared__ unsigned int sidx[4][NLHOME];
     __shared__ hm_cell_entry sent[4][NLHOME];
-    if (gate && gate->rebuild_at != gate->timestep + 1)
+    if (gate_closed(gate))
         return;
     const unsigned int w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const unsigned int c = blockIdx.x * 4 + w;
-    const unsigned int n = c < ncell ? min(cell_size[c], cmax) : 0u;
-    for (unsigned int k = lane; k < n; k += 64)
+    const unsigned int rounds = (ncell + gridDim.x * 4 - 1) / (gridDim.x * 4);
+    for (unsigned int r = 0; r < rounds; r++)
     {
-        hm_cell_entry e = entries[(size_t)c * cmax + k];
-        sent[w][k] = e;
-        sidx[w][k] = e.idx;
-    }
-    __syncthreads();
-    for (unsigned int k = lane; k < n; k += 64)
-    {
-        const unsigned int my = sidx[w][k];
-        unsigned int rank = 0;
-        for (unsigned int m = 0; m < n; m++)
-            rank += sidx[w][m] < my ? 1u : 0u;
-        entries[(size_t)c * cmax + rank] = sent[w][k];
+        const unsigned int c = (r * gridDim.x + blockIdx.x) * 4 + w;
+        const unsigned int n = c < ncell ? min(cell_size[c], cmax) : 0u;
+        for (unsigned int k = lane; k < n; k += 64)
+        {
+            hm_cell_entry e = entries[(size_t)c * cmax + k];
+            sent[w][k] = e;
+            sidx[w][k] = e.idx;
+        }
+        __syncthreads();
+        for (unsigned int k = lane; k < n; k += 64)
+        {
+            const unsigned int my = sidx[w][k];
+            unsigned int rank = 0;
+            for (unsigned int m = 0; m < n; m++)
+                rank += sidx[w][m] < my ? 1u : 0u;
+            entries[(size_t)c * cmax + rank] = sent[w][k];
+            if (cell_idx)
+                cell_idx[(size_t)c * cmax + rank] = my;
+        }
+        __syncthreads();
     }
 }
 
 hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
-                            hm_cell_entry* entries, const StepState* gate, hipStream_t s)
+                            hm_cell_entry* entries, unsigned int* cell_idx, const StepState* gate,
+                            hipStream_t s)
 {
     if (ncell == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(cell_sort_kernel, dim3((ncell + 3) / 4), dim3(256), 0, s, ncell, cmax,
-                       cell_size, entries, gate);
+    unsigned int blocks = min((ncell + 3) / 4, GATED_BLOCKS);
+    hipLaunchKernelGGL(cell_sort_kernel, dim3(blocks), dim3(256), 0, s, ncell, cmax, cell_size,
+                       entries, cell_idx, gate);
     return hipGetLastError();
 }
 
-// Shared prologue: cell id, pass bounds and adjacency (sizes capped at cmax).
+// largest q in [first, last) with aoff[q] <= k (aoff is non-decreasing)
 __device__ __forceinline__ unsigned int find_segment(const unsigned int* aoff, unsigned int first,
                                                      unsigned int last, unsigned int k)
 {
-    // largest q in [first, last) with aoff[q] <= k (aoff is non-decreasing)
     unsigned int lo = first, hi = last - 1;
     while (lo < hi)
     {
@@ -213,7 +274,8 @@ __device__ __forceinline__ unsigned int find_segment(const unsigned int* aoff, u
     return lo;
 }
 
-__device__ __forceinline__ bool pass_bounds(unsigned int n_adj, const unsigned int* asize,
+// thread 0: offsets of the adjacent cells of one staging pass (at most NLCAP candidates)
+__device__ __forceinline__ void pass_bounds(unsigned int n_adj, const unsigned int* asize,
                                             unsigned int* aoff, unsigned int first,
                                             unsigned int* pass_last, unsigned int* pass_total)
 {
@@ -229,123 +291,22 @@ __device__ __forceinline__ bool pass_bounds(unsigned int n_adj, const unsigned i
         *pass_last = l;
         *pass_total = tot;
     }
-    return true;
 }
 
-// General kernel: FP64 test for every candidate (any grid, including < 4 cells per axis).
-template<bool MT, bool EXCL>
-__global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
+__device__ __forceinline__ void load_exclusions(const NlistKernelArgs& a, unsigned int i,
+                                                unsigned int& ex0, unsigned int& ex1,
+                                                unsigned int& ex2, unsigned int& ex3)
 {
-#pragma clang fp contract(off)
-    __shared__ double sx[NLCAP], sy[NLCAP], sz[NLCAP];
-    __shared__ unsigned int sj[NLCAP];
-    __shared__ unsigned char stp[MT ? NLCAP : 1];
-    __shared__ unsigned int acell[27], asize[27], aoff[27];
-    __shared__ unsigned int hcnt[NLHOME];
-    __shared__ unsigned int pass_last, pass_total;
-
-    if (a.gate && a.gate->rebuild_at != a.gate->timestep + 1)
-        return;
-    const unsigned int cell = xcd_block(blockIdx.x, gridDim.x);
-    if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
-        a.counter->n_builds++;
-    if (cell >= a.ncell)
-        return;
-    const unsigned int nhome = min(a.cell_size[cell], a.cmax);
-    if (nhome == 0)
-        return;
-    const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid < a.n_adj)
-    {
-        unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
-        acell[tid] = c;
-        asize[tid] = min(a.cell_size[c], a.cmax);
-    }
-    for (unsigned int s = tid; s < nhome; s += NLB)
-        hcnt[s] = 0;
-    __syncthreads();
-
-    unsigned int first = 0;
-    while (first < a.n_adj)
-    {
-        pass_bounds(a.n_adj, asize, aoff, first, &pass_last, &pass_total);
-        __syncthreads();
-        const unsigned int last = pass_last, total = pass_total;
-        for (unsigned int k = tid; k < total; k += NLB)
-        {
-            const unsigned int q = find_segment(aoff, first, last, k);
-            const hm_cell_entry e = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
-            sx[k] = e.x;
-            sy[k] = e.y;
-            sz[k] = e.z;
-            sj[k] = e.idx;
-            if (MT)
-                stp[k] = (unsigned char)e.type;
-        }
-        __syncthreads();
-        for (unsigned int s = wave; s < nhome; s += NLB / 64)
-        {
-            const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
-            const unsigned int i = h.idx;
-            const unsigned int ti = MT ? h.type : 0u;
-            unsigned int ex0 = 0xffffffffu, ex1 = 0xffffffffu, ex2 = 0xffffffffu,
-                         ex3 = 0xffffffffu;
-            if (EXCL)
-            {
-                unsigned int ne = min(a.n_excl[i], a.max_excl);
-                const unsigned int* er = a.excl + (size_t)i * a.max_excl;
-                if (ne > 0)
-                    ex0 = er[0];
-                if (ne > 1)
-                    ex1 = er[1];
-                if (ne > 2)
-                    ex2 = er[2];
-                if (ne > 3)
-                    ex3 = er[3];
-            }
-            unsigned int cnt = hcnt[s];
-            unsigned int* row = a.nlist + (size_t)i * a.nmax;
-            for (unsigned int base = 0; base < total; base += 64)
-            {
-                const unsigned int k = base + lane;
-                bool hit = false;
-                unsigned int j = 0;
-                if (k < total)
-                {
-                    j = sj[k];
-                    double dx = h.x - sx[k];
-                    double dy = h.y - sy[k];
-                    double dz = h.z - sz[k];
-                    min_image(a.box, dx, dy, dz);
-                    double dr2 = dx * dx + dy * dy + dz * dz;
-                    unsigned int tp = MT ? ti * a.ntypes + stp[k] : 0u;
-                    hit = (j != i) && a.enabled[tp] && (dr2 <= a.rlistsq[tp]);
-                    if (EXCL)
-                        hit = hit && j != ex0 && j != ex1 && j != ex2 && j != ex3;
-                }
-                const uint64_t m = __ballot(hit);
-                if (hit)
-                {
-                    unsigned int p = cnt + lanes_below(m);
-                    if (p < a.nmax)
-                        row[p] = j;
-                }
-                cnt += (unsigned int)__popcll(m);
-            }
-            if (lane == 0)
-                hcnt[s] = cnt;
-        }
-        __syncthreads();
-        first = last;
-    }
-    for (unsigned int s = tid; s < nhome; s += NLB)
-    {
-        unsigned int i = a.entries[(size_t)cell * a.cmax + s].idx;
-        unsigned int c = hcnt[s];
-        a.n_neigh[i] = c;
-        if (c > a.nmax)
-            atomicMax(a.cond, c);
-    }
+    unsigned int ne = min(a.n_excl[i], a.max_excl);
+    const unsigned int* er = a.excl + (size_t)i * a.max_excl;
+    if (ne > 0)
+        ex0 = er[0];
+    if (ne > 1)
+        ex1 = er[1];
+    if (ne > 2)
+        ex2 = er[2];
+    if (ne > 3)
+        ex3 = er[3];
 }
 
 // Minimum-image offset of a position from a cell centre, rounded to FP32 (dims >= 4 keeps every
@@ -359,117 +320,150 @@ __device__ __forceinline__ float3 rel_to_centre(const DevBox& b, double x, doubl
     return make_float3((float)dx, (float)dy, (float)dz);
 }
 
-// Fast kernel: FP32 test on centre-relative coordinates, FP64 re-decision inside the guard band.
-template<bool MT, bool EXCL>
-__global__ __launch_bounds__(NLB) void nlist_f32_kernel(NlistKernelArgs a)
+__device__ __forceinline__ void cell_centre(const DevBox& b, const unsigned int* dim,
+                                            unsigned int cell, double& cx, double& cy, double& cz)
 {
-    __shared__ float4 sc[NLCAP];
-    __shared__ unsigned char stp[MT ? NLCAP : 1];
-    __shared__ float4 sh[NLHOME];
-    __shared__ unsigned char shtp[MT ? NLHOME : 1];
-    __shared__ unsigned int acell[27], asize[27], aoff[27];
-    __shared__ unsigned int hcnt[NLHOME];
-    __shared__ unsigned int pass_last, pass_total;
+#pragma clang fp contract(off)
+    const unsigned int ci = cell % dim[0], cj = (cell / dim[0]) % dim[1],
+                       ck = cell / (dim[0] * dim[1]);
+    cx = b.lo[0] + ((double)ci + 0.5) * (b.L[0] / (double)dim[0]);
+    cy = b.lo[1] + ((double)cj + 0.5) * (b.L[1] / (double)dim[1]);
+    cz = b.lo[2] + ((double)ck + 0.5) * (b.L[2] / (double)dim[2]);
+}
 
-    if (a.gate && a.gate->rebuild_at != a.gate->timestep + 1)
-        return;
-    const unsigned int cell = xcd_block(blockIdx.x, gridDim.x);
-    if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
-        a.counter->n_builds++;
-    if (cell >= a.ncell)
-        return;
+// One home cell of the neighbour-list build. FAST: FP32 test with FP64 guard-band re-decision
+// (needs >= 4 cells per axis); otherwise every candidate is tested in FP64.
+template<bool FAST, bool MT, bool EXCL> struct NlistShared
+{
+    float4 sc[FAST ? NLCAP : 1];
+    double sx[FAST ? 1 : NLCAP], sy[FAST ? 1 : NLCAP], sz[FAST ? 1 : NLCAP];
+    unsigned int sj[FAST ? 1 : NLCAP];
+    unsigned char stp[MT ? NLCAP : 1];
+    float4 sh[NLHOME];
+    unsigned char shtp[MT ? NLHOME : 1];
+    unsigned int acell[27], asize[27], aoff[27];
+    unsigned int hcnt[NLHOME];
+    unsigned int pass_last, pass_total;
+};
+
+template<bool FAST, bool MT, bool EXCL>
+__device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
+                           NlistShared<FAST, MT, EXCL>& S)
+{
+#pragma clang fp contract(off)
     const unsigned int nhome = min(a.cell_size[cell], a.cmax);
     if (nhome == 0)
         return;
     const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    double cx, cy, cz;
-    {
-#pragma clang fp contract(off)
-        const unsigned int ci = cell % a.dim[0], cj = (cell / a.dim[0]) % a.dim[1],
-                           ck = cell / (a.dim[0] * a.dim[1]);
-        cx = a.box.lo[0] + ((double)ci + 0.5) * (a.box.L[0] / (double)a.dim[0]);
-        cy = a.box.lo[1] + ((double)cj + 0.5) * (a.box.L[1] / (double)a.dim[1]);
-        cz = a.box.lo[2] + ((double)ck + 0.5) * (a.box.L[2] / (double)a.dim[2]);
-    }
+    double cx = 0, cy = 0, cz = 0;
+    if (FAST)
+        cell_centre(a.box, a.dim, cell, cx, cy, cz);
     if (tid < a.n_adj)
     {
         unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
-        acell[tid] = c;
-        asize[tid] = min(a.cell_size[c], a.cmax);
+        S.acell[tid] = c;
+        S.asize[tid] = min(a.cell_size[c], a.cmax);
     }
     for (unsigned int s = tid; s < nhome; s += NLB)
     {
         const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
-        const float3 r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
-        sh[s] = make_float4(r.x, r.y, r.z, __uint_as_float(h.idx));
+        if (FAST)
+        {
+            const float3 r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
+            S.sh[s] = make_float4(r.x, r.y, r.z, __uint_as_float(h.idx));
+        }
+        else
+            S.sh[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(h.idx));
         if (MT)
-            shtp[s] = (unsigned char)h.type;
-        hcnt[s] = 0;
+            S.shtp[s] = (unsigned char)h.type;
+        S.hcnt[s] = 0;
     }
     __syncthreads();
 
-    unsigned int first = 0;
+    unsigned int first = 0, base_k = 0;
     while (first < a.n_adj)
     {
-        pass_bounds(a.n_adj, asize, aoff, first, &pass_last, &pass_total);
+        pass_bounds(a.n_adj, S.asize, S.aoff, first, &S.pass_last, &S.pass_total);
         __syncthreads();
-        const unsigned int last = pass_last, total = pass_total;
+        const unsigned int last = S.pass_last, total = S.pass_total;
         for (unsigned int k = tid; k < total; k += NLB)
         {
-            const unsigned int q = find_segment(aoff, first, last, k);
-            const hm_cell_entry e = a.entries[(size_t)acell[q] * a.cmax + (k - aoff[q])];
-            const float3 r = rel_to_centre(a.box, e.x, e.y, e.z, cx, cy, cz);
-            sc[k] = make_float4(r.x, r.y, r.z, __uint_as_float(e.idx));
+            const unsigned int q = find_segment(S.aoff, first, last, k);
+            const hm_cell_entry e = a.entries[(size_t)S.acell[q] * a.cmax + (k - S.aoff[q])];
+            if (FAST)
+            {
+                const float3 r = rel_to_centre(a.box, e.x, e.y, e.z, cx, cy, cz);
+                S.sc[k] = make_float4(r.x, r.y, r.z, __uint_as_float(e.idx));
+            }
+            else
+            {
+                S.sx[k] = e.x;
+                S.sy[k] = e.y;
+                S.sz[k] = e.z;
+                S.sj[k] = e.idx;
+            }
             if (MT)
-                stp[k] = (unsigned char)e.type;
+                S.stp[k] = (unsigned char)e.type;
         }
         __syncthreads();
         for (unsigned int s = wave; s < nhome; s += NLB / 64)
         {
-            const float4 h = sh[s];
-            const unsigned int i = __float_as_uint(h.w);
-            const unsigned int ti = MT ? shtp[s] : 0u;
+            const float4 h4 = S.sh[s];
+            const unsigned int i = __float_as_uint(h4.w);
+            const unsigned int ti = MT ? S.shtp[s] : 0u;
             unsigned int ex0 = 0xffffffffu, ex1 = 0xffffffffu, ex2 = 0xffffffffu,
                          ex3 = 0xffffffffu;
             if (EXCL)
+                load_exclusions(a, i, ex0, ex1, ex2, ex3);
+            double hx = 0, hy = 0, hz = 0;
+            if (!FAST)
             {
-                unsigned int ne = min(a.n_excl[i], a.max_excl);
-                const unsigned int* er = a.excl + (size_t)i * a.max_excl;
-                if (ne > 0)
-                    ex0 = er[0];
-                if (ne > 1)
-                    ex1 = er[1];
-                if (ne > 2)
-                    ex2 = er[2];
-                if (ne > 3)
-                    ex3 = er[3];
+                const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
+                hx = h.x;
+                hy = h.y;
+                hz = h.z;
             }
-            unsigned int cnt = hcnt[s];
+            unsigned int cnt = S.hcnt[s];
             unsigned int* row = a.nlist + (size_t)i * a.nmax;
-            for (unsigned int base = 0; base < total; base += 64)
+            unsigned short* lrow = a.lrow ? a.lrow + (size_t)i * a.nmax : nullptr;
+            for (unsigned int b = 0; b < total; b += 64)
             {
-                const unsigned int k = base + lane;
+                const unsigned int k = b + lane;
                 bool hit = false;
                 unsigned int j = 0;
                 if (k < total)
                 {
-                    const float4 c4 = sc[k];
-                    j = __float_as_uint(c4.w);
-                    const float dx = h.x - c4.x, dy = h.y - c4.y, dz = h.z - c4.z;
-                    const float dr2 = dx * dx + dy * dy + dz * dz;
-                    const unsigned int tp = MT ? ti * a.ntypes + stp[k] : 0u;
-                    const float rl = a.rlistsq_f[tp];
-                    hit = dr2 <= rl;
-                    if (fabsf(dr2 - rl) <= a.guard[tp])
+                    unsigned int tp = 0;
+                    if (FAST)
                     {
-                        // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
-#pragma clang fp contract(off)
-                        const double4 pi = a.pos[i];
-                        const double4 pj = a.pos[j];
-                        double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-                        min_image(a.box, ddx, ddy, ddz);
-                        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        hit = d2 <= a.rlistsq[tp];
+                        const float4 c4 = S.sc[k];
+                        j = __float_as_uint(c4.w);
+                        const float dx = h4.x - c4.x, dy = h4.y - c4.y, dz = h4.z - c4.z;
+                        const float dr2 = dx * dx + dy * dy + dz * dz;
+                        tp = MT ? ti * a.ntypes + S.stp[k] : 0u;
+                        const float rl = a.rlistsq_f[tp];
+                        hit = dr2 <= rl;
+                        if (fabsf(dr2 - rl) <= a.guard[tp])
+                        {
+                            // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
+                            const double4 pi = a.pos[i];
+                            const double4 pj = a.pos[j];
+                            double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                            min_image(a.box, ddx, ddy, ddz);
+                            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            hit = d2 <= a.rlistsq[tp];
+                        }
+                    }
+                    else
+                    {
+                        j = S.sj[k];
+                        double dx = hx - S.sx[k];
+                        double dy = hy - S.sy[k];
+                        double dz = hz - S.sz[k];
+                        min_image(a.box, dx, dy, dz);
+                        const double dr2 = dx * dx + dy * dy + dz * dz;
+                        tp = MT ? ti * a.ntypes + S.stp[k] : 0u;
+                        hit = dr2 <= a.rlistsq[tp];
                     }
                     hit = hit && (j != i) && a.enabled[tp];
                     if (EXCL)
@@ -478,26 +472,59 @@ __global__ __launch_bounds__(NLB) void nlist_f32_kernel(NlistKernelArgs a)
                 const uint64_t m = __ballot(hit);
                 if (hit)
                 {
-                    unsigned int p = cnt + lanes_below(m);
+                    const unsigned int p = cnt + lanes_below(m);
                     if (p < a.nmax)
+                    {
                         row[p] = j;
+                        if (lrow)
+                            lrow[p] = (unsigned short)(base_k + k);
+                    }
                 }
                 cnt += (unsigned int)__popcll(m);
             }
             if (lane == 0)
-                hcnt[s] = cnt;
+                S.hcnt[s] = cnt;
         }
         __syncthreads();
+        base_k += total;
         first = last;
     }
     for (unsigned int s = tid; s < nhome; s += NLB)
     {
-        unsigned int i = __float_as_uint(sh[s].w);
-        unsigned int c = hcnt[s];
+        const unsigned int i = __float_as_uint(S.sh[s].w);
+        const unsigned int c = S.hcnt[s];
         a.n_neigh[i] = c;
         if (c > a.nmax)
             atomicMax(a.cond, c);
     }
+    __syncthreads();
+}
+
+template<bool FAST, bool MT, bool EXCL>
+__global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
+{
+    __shared__ NlistShared<FAST, MT, EXCL> S;
+    if (gate_closed(a.gate))
+        return;
+    if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
+        a.counter->n_builds++;
+    const XcdRange r = xcd_range(a.ncell);
+    for (unsigned int cell = r.begin; cell < r.end; cell += r.stride)
+        nlist_cell<FAST, MT, EXCL>(a, cell, S);
+}
+
+template<bool FAST>
+static void nlist_dispatch(const NlistKernelArgs& a, bool mt, bool ex, unsigned int grid,
+                           hipStream_t s)
+{
+    if (mt && ex)
+        hipLaunchKernelGGL((nlist_kernel<FAST, true, true>), dim3(grid), dim3(NLB), 0, s, a);
+    else if (mt)
+        hipLaunchKernelGGL((nlist_kernel<FAST, true, false>), dim3(grid), dim3(NLB), 0, s, a);
+    else if (ex)
+        hipLaunchKernelGGL((nlist_kernel<FAST, false, true>), dim3(grid), dim3(NLB), 0, s, a);
+    else
+        hipLaunchKernelGGL((nlist_kernel<FAST, false, false>), dim3(grid), dim3(NLB), 0, s, a);
 }
 
 hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
@@ -505,7 +532,7 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
     if (a_in.ncell == 0)
         return hipSuccess;
     NlistKernelArgs a = a_in;
-    const unsigned int grid = round_up(a.ncell, 8);
+    const unsigned int grid = round_up(min(a.ncell, GATED_BLOCKS), 8);
     const bool mt = a.ntypes > 1;
     const bool ex = a.n_excl != nullptr && a.max_excl > 0;
     const bool fast = a.pos != nullptr && a.dim[0] >= 4 && a.dim[1] >= 4 && a.dim[2] >= 4;
@@ -527,24 +554,10 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
             a.rlistsq_f[q] = (float)rl2;
             a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rl * e_dx + 6.0 * u * rl2) + 1e-12);
         }
-        if (mt && ex)
-            hipLaunchKernelGGL((nlist_f32_kernel<true, true>), dim3(grid), dim3(NLB), 0, s, a);
-        else if (mt)
-            hipLaunchKernelGGL((nlist_f32_kernel<true, false>), dim3(grid), dim3(NLB), 0, s, a);
-        else if (ex)
-            hipLaunchKernelGGL((nlist_f32_kernel<false, true>), dim3(grid), dim3(NLB), 0, s, a);
-        else
-            hipLaunchKernelGGL((nlist_f32_kernel<false, false>), dim3(grid), dim3(NLB), 0, s, a);
-        return hipGetLastError();
+        nlist_dispatch<true>(a, mt, ex, grid, s);
     }
-    if (mt && ex)
-        hipLaunchKernelGGL((nlist_kernel<true, true>), dim3(grid), dim3(NLB), 0, s, a);
-    else if (mt)
-        hipLaunchKernelGGL((nlist_kernel<true, false>), dim3(grid), dim3(NLB), 0, s, a);
-    else if (ex)
-        hipLaunchKernelGGL((nlist_kernel<false, true>), dim3(grid), dim3(NLB), 0, s, a);
     else
-        hipLaunchKernelGGL((nlist_kernel<false, false>), dim3(grid), dim3(NLB), 0, s, a);
+        nlist_dispatch<false>(a, mt, ex, grid, s);
     return hipGetLastError();
 }
 
@@ -790,6 +803,286 @@ hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params, P
     for (unsigned int q = 0; q < nt * nt; q++)
         tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
     return pair_dispatch<E>(a, tab, cfg, s);
+}
+
+// ======================================================================== cell-tiled pair forces
+// The same sums as pair_kernel, organised per home cell: the workgroup stages the current FP64
+// positions of the 27 adjacent cells' particles in LDS (shifted into the home cell's periodic
+// frame, so no per-pair minimum image is needed), then TPP lanes per home particle walk the
+// particle's 16-bit local row, which indexes that staged sequence. Each position is fetched from
+// memory once per adjacent home cell instead of once per neighbour pair, and the row stream is
+// half the size of the 32-bit list. Cells whose candidates exceed PC_CAP fall back to global
+// gathers with the full minimum image.
+constexpr int PC_CAP = 1024;
+constexpr int PC_BLOCK = 256;
+
+template<class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__global__ __launch_bounds__(PC_BLOCK) void pair_cell_kernel(PairCellArgs a, ParamTable<Eval> tab)
+{
+    __shared__ double sx[PC_CAP], sy[PC_CAP], sz[PC_CAP];
+    __shared__ unsigned char stp[MT ? PC_CAP : 1];
+    __shared__ unsigned int acell[27], asize[27], aoff[27];
+    __shared__ unsigned int s_home_off, s_total;
+    const unsigned int tid = threadIdx.x;
+    const XcdRange range = xcd_range(a.ncell);
+    for (unsigned int cell = range.begin; cell < range.end; cell += range.stride)
+    {
+        const unsigned int nhome = min(a.cell_size[cell], a.cmax);
+        if (nhome == 0)
+            continue;
+        if (tid < a.n_adj)
+        {
+            const unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
+            acell[tid] = c;
+            asize[tid] = min(a.cell_size[c], a.cmax);
+        }
+        __syncthreads();
+        if (tid == 0)
+        {
+            unsigned int tot = 0, home = 0;
+            for (unsigned int q = 0; q < a.n_adj; q++)
+            {
+                aoff[q] = tot;
+                if (acell[q] == cell)
+                    home = tot;
+                tot += asize[q];
+            }
+            s_total = tot;
+            s_home_off = home;
+        }
+        __syncthreads();
+        const unsigned int total = s_total, home_off = s_home_off;
+        const bool staged = total <= (unsigned int)PC_CAP;
+        double cx, cy, cz;
+        cell_centre(a.box, a.dim, cell, cx, cy, cz);
+        if (staged)
+        {
+            for (unsigned int k = tid; k < total; k += PC_BLOCK)
+            {
+#pragma clang fp contract(off)
+                const unsigned int q = find_segment(aoff, 0, a.n_adj, k);
+                const unsigned int idx = a.cell_idx[(size_t)acell[q] * a.cmax + (k - aoff[q])];
+                const double4 p = a.pos[idx];
+                sx[k] = p.x - a.box.L[0] * rint((p.x - cx) * a.box.Linv[0]);
+                sy[k] = p.y - a.box.L[1] * rint((p.y - cy) * a.box.Linv[1]);
+                sz[k] = p.z - a.box.L[2] * rint((p.z - cz) * a.box.Linv[2]);
+                if (MT)
+                    stp[k] = (unsigned char)p.w;
+            }
+        }
+        __syncthreads();
+        for (unsigned int hb = 0; hb < nhome; hb += PC_BLOCK / TPP)
+        {
+            const unsigned int hp = hb + tid / TPP, sub = tid % TPP;
+            const bool active = hp < nhome;
+            const unsigned int i = a.cell_idx[(size_t)cell * a.cmax + (active ? hp : 0u)];
+            double xi, yi, zi;
+            unsigned int ti = 0;
+            if (staged)
+            {
+                const unsigned int kh = home_off + (active ? hp : 0u);
+                xi = sx[kh];
+                yi = sy[kh];
+                zi = sz[kh];
+                if (MT)
+                    ti = stp[kh];
+            }
+            else
+            {
+                const double4 p = a.pos[i];
+                xi = p.x;
+                yi = p.y;
+                zi = p.z;
+                ti = MT ? (unsigned int)p.w : 0u;
+            }
+            const unsigned int n = active ? min(a.n_neigh[i], a.nmax) : 0u;
+            const unsigned short* row = a.lrow + (size_t)i * a.nmax;
+            float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+            float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
+            for (unsigned int k0 = 0; k0 < n; k0 += TPP * PAIR_BATCH)
+            {
+                double px[PAIR_BATCH], py[PAIR_BATCH], pz[PAIR_BATCH];
+                unsigned int tj[PAIR_BATCH];
+                bool val[PAIR_BATCH];
+#pragma unroll
+                for (int q = 0; q < PAIR_BATCH; q++)
+                {
+                    const unsigned int e = k0 + q * TPP + sub;
+                    val[q] = e < n;
+                    // local rows index [0, total); clamp so a stale row cannot leave the tables
+                    const unsigned int l = val[q] ? min((unsigned int)row[e], total - 1) : 0u;
+                    if (staged)
+                    {
+                        const unsigned int ls = val[q] ? l : home_off + (active ? hp : 0u);
+                        px[q] = sx[ls];
+                        py[q] = sy[ls];
+                        pz[q] = sz[ls];
+                        tj[q] = MT ? stp[ls] : 0u;
+                    }
+                    else
+                    {
+#pragma clang fp contract(off)
+                        const unsigned int qq = find_segment(aoff, 0, a.n_adj, l);
+                        const unsigned int j =
+                            val[q] ? a.cell_idx[(size_t)acell[qq] * a.cmax + (l - aoff[qq])] : i;
+                        const double4 pj = a.pos[j];
+                        double ddx = xi - pj.x, ddy = yi - pj.y, ddz = zi - pj.z;
+                        min_image(a.box, ddx, ddy, ddz);
+                        px[q] = xi - ddx;
+                        py[q] = yi - ddy;
+                        pz[q] = zi - ddz;
+                        tj[q] = MT ? (unsigned int)pj.w : 0u;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PAIR_BATCH; q++)
+                {
+                    double dx, dy, dz, rsq;
+                    {
+#pragma clang fp contract(off)
+                        dx = xi - px[q];
+                        dy = yi - py[q];
+                        dz = zi - pz[q];
+                        rsq = dx * dx + dy * dy + dz * dz;
+                    }
+                    const unsigned int tp = MT ? ti * a.ntypes + tj[q] : 0u;
+                    const bool in = val[q] && (rsq < a.rcutsq[tp]);
+                    const float rsqf = in ? (float)rsq : 1.0f;
+                    const float rcf = a.rcutsq_f[tp];
+                    Eval ev(rsqf, rcf, tab.p[tp]);
+                    float fdr = 0.f, e = 0.f;
+                    const bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
+                    ev.evalForceAndEnergy(fdr, e, shift);
+                    if (SHIFT == 2)
+                    {
+                        const float ron = a.ronsq_f[tp];
+                        if (rsqf >= ron && rsqf < rcf)
+                        {
+                            float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
+                            float rmc = rsqf - rcf;
+                            float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
+                            float ds = 12.f * (rsqf - ron) * rmc * den;
+                            float e0 = e;
+                            e = e0 * sw;
+                            fdr = sw * fdr - ds * e0;
+                        }
+                    }
+                    fdr = in ? fdr : 0.f;
+                    e = in ? e : 0.f;
+                    const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+                    fx += dxf * fdr;
+                    fy += dyf * fdr;
+                    fz += dzf * fdr;
+                    pe += 0.5f * e;
+                    if (VIRIAL)
+                    {
+                        const float f2 = 0.5f * fdr;
+                        vxx += f2 * dxf * dxf;
+                        vxy += f2 * dxf * dyf;
+                        vxz += f2 * dxf * dzf;
+                        vyy += f2 * dyf * dyf;
+                        vyz += f2 * dyf * dzf;
+                        vzz += f2 * dzf * dzf;
+                    }
+                }
+            }
+            if (TPP > 1)
+            {
+#pragma unroll
+                for (int off = TPP / 2; off > 0; off >>= 1)
+                {
+                    fx += __shfl_xor(fx, off, TPP);
+                    fy += __shfl_xor(fy, off, TPP);
+                    fz += __shfl_xor(fz, off, TPP);
+                    pe += __shfl_xor(pe, off, TPP);
+                    if (VIRIAL)
+                    {
+                        vxx += __shfl_xor(vxx, off, TPP);
+                        vxy += __shfl_xor(vxy, off, TPP);
+                        vxz += __shfl_xor(vxz, off, TPP);
+                        vyy += __shfl_xor(vyy, off, TPP);
+                        vyz += __shfl_xor(vyz, off, TPP);
+                        vzz += __shfl_xor(vzz, off, TPP);
+                    }
+                }
+            }
+            if (active && sub == 0)
+            {
+                a.force[i] = make_float4(fx, fy, fz, pe);
+                if (VIRIAL)
+                {
+                    const size_t P = a.N;
+                    a.virial[0 * P + i] = vxx;
+                    a.virial[1 * P + i] = vxy;
+                    a.virial[2 * P + i] = vxz;
+                    a.virial[3 * P + i] = vyy;
+                    a.virial[4 * P + i] = vyz;
+                    a.virial[5 * P + i] = vzz;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template<class Eval, int TPP, bool VIRIAL, bool MT>
+static void pair_cell_shift(const PairCellArgs& a, const ParamTable<Eval>& tab, dim3 g,
+                            hipStream_t s)
+{
+    const dim3 b(PC_BLOCK);
+    switch (a.shift_mode)
+    {
+    case 0:
+        hipLaunchKernelGGL((pair_cell_kernel<Eval, TPP, VIRIAL, MT, 0>), g, b, 0, s, a, tab);
+        break;
+    case 1:
+        hipLaunchKernelGGL((pair_cell_kernel<Eval, TPP, VIRIAL, MT, 1>), g, b, 0, s, a, tab);
+        break;
+    default:
+        hipLaunchKernelGGL((pair_cell_kernel<Eval, TPP, VIRIAL, MT, 2>), g, b, 0, s, a, tab);
+        break;
+    }
+}
+
+template<class Eval, int TPP>
+static void pair_cell_tpp(const PairCellArgs& a, const ParamTable<Eval>& tab, dim3 g,
+                          hipStream_t s)
+{
+    const bool mt = a.ntypes > 1;
+    if (a.compute_virial)
+        mt ? pair_cell_shift<Eval, TPP, true, true>(a, tab, g, s)
+           : pair_cell_shift<Eval, TPP, true, false>(a, tab, g, s);
+    else
+        mt ? pair_cell_shift<Eval, TPP, false, true>(a, tab, g, s)
+           : pair_cell_shift<Eval, TPP, false, false>(a, tab, g, s);
+}
+
+hipError_t launch_pair_cell_lj(const PairCellArgs& a, const hm_lj_params* params, PairLaunch cfg,
+                               hipStream_t s)
+{
+    if (a.N == 0 || a.ncell == 0)
+        return hipSuccess;
+    using E = EvaluatorPairLJ<float>;
+    ParamTable<E> tab;
+    const unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+    // cfg.block is the number of workgroups for the persistent cell loop (0: one per cell)
+    const unsigned int grid = round_up(cfg.block ? min(a.ncell, cfg.block) : a.ncell, 8);
+    const dim3 g(grid);
+    switch (cfg.tpp ? cfg.tpp : 8)
+    {
+    case 4:
+        pair_cell_tpp<E, 4>(a, tab, g, s);
+        break;
+    case 16:
+        pair_cell_tpp<E, 16>(a, tab, g, s);
+        break;
+    default:
+        pair_cell_tpp<E, 8>(a, tab, g, s);
+        break;
+    }
+    return hipGetLastError();
 }
 
 // ======================================================================== FENE bonds

@@ -29,7 +29,8 @@ class HmBox(C.Structure):
 
 class HmCellArgs(C.Structure):
     _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("dim", _u * 3), ("cmax", _u),
-                ("cell_size", _p), ("entries", _p), ("last_pos", _p), ("conditions", _p)]
+                ("cell_size", _p), ("entries", _p), ("last_pos", _p), ("conditions", _p),
+                ("cell_idx", _p)]
 
 
 class HmNlistArgs(C.Structure):
@@ -37,7 +38,7 @@ class HmNlistArgs(C.Structure):
                 ("cell_size", _p), ("entries", _p), ("cell_adj", _p), ("n_adj", _u),
                 ("ntypes", _u), ("r_cut", _d * (MAX_TYPES * MAX_TYPES)), ("r_buff", _d),
                 ("n_excl", _p), ("excl", _p), ("max_excl", _u), ("nmax", _u), ("nlist", _p),
-                ("n_neigh", _p), ("conditions", _p)]
+                ("n_neigh", _p), ("conditions", _p), ("local_rows", _p)]
 
 
 class HmLJParams(C.Structure):
@@ -50,6 +51,15 @@ class HmPairArgs(C.Structure):
                 ("r_on", _d * (MAX_TYPES * MAX_TYPES)), ("shift_mode", _i),
                 ("compute_virial", _i), ("force", _p), ("virial", _p), ("tpp", _u),
                 ("block_size", _u)]
+
+
+class HmPairCellArgs(C.Structure):
+    _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("dim", _u * 3), ("cmax", _u),
+                ("cell_size", _p), ("cell_idx", _p), ("cell_adj", _p), ("n_adj", _u),
+                ("local_rows", _p), ("nmax", _u), ("n_neigh", _p), ("ntypes", _u),
+                ("r_cut", _d * (MAX_TYPES * MAX_TYPES)), ("r_on", _d * (MAX_TYPES * MAX_TYPES)),
+                ("shift_mode", _i), ("compute_virial", _i), ("force", _p), ("virial", _p),
+                ("tpp", _u), ("workgroups", _u)]
 
 
 class HmFeneParams(C.Structure):
@@ -81,6 +91,7 @@ SIGNATURES = {
     "hm_nlist_build": (_i, [C.POINTER(HmNlistArgs), _p]),
     "hm_nlist_check": (_i, [_u, _p, _p, HmBox, _d, _d, _p, _p]),
     "hm_pair_lj": (_i, [C.POINTER(HmPairArgs), _p, _p]),
+    "hm_pair_lj_cells": (_i, [C.POINTER(HmPairCellArgs), _p, _p]),
     "hm_bond_fene": (_i, [C.POINTER(HmBondArgs), _p, _p]),
     "hm_nvt_step_one": (_i, [_u, _p, _p, _p, _p, HmBox, _d, _d, _p]),
     "hm_nvt_step_two": (_i, [_u, _p, _p, _p, _u, _d, _d, _p]),
@@ -99,6 +110,7 @@ SIGNATURES = {
     "hm_sys_set_sorter": (_i, [_p, _u, _u]),
     "hm_sys_set_compute_virial": (_i, [_p, _i]),
     "hm_sys_set_pair_launch": (_i, [_p, _u, _u]),
+    "hm_sys_set_pair_kernel": (_i, [_p, _i, _u, _u]),
     "hm_sys_set_chunk": (_i, [_p, _u, _i]),
     "hm_sys_run": (_i, [_p, _u64]),
     "hm_sys_timestep": (_u64, [_p]),

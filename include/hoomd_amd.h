@@ -111,9 +111,10 @@ typedef struct hm_cell_args
     hm_cell_entry* entries;     /* [ncell*cmax] */
     double* last_pos;           /* [3N] or NULL: receives the binned positions */
     unsigned int* conditions;   /* [3] atomicMax'd: needed cmax, NaN particle + 1, outside + 1 */
+    unsigned int* cell_idx;     /* [ncell*cmax] or NULL: particle index of each (ordered) slot */
 } hm_cell_args;
-/* Bin every particle. Slot order inside a cell is unspecified (atomics); the neighbour-list
- * build orders each cell by particle index. Replaces gpu_compute_cell_list
+/* Bin every particle, then order each cell's slots by particle index (cmax <= 256). Replaces
+ * gpu_compute_cell_list
  * (hoomd/CellListGPU.cuh, kernel CellListGPU.cu:42-171) driven by CellListGPU::computeCellList
  * (hoomd/CellListGPU.cc:39-232). */
 int hm_cell_list_build(const hm_cell_args* args, hm_stream_t stream);
@@ -139,6 +140,9 @@ typedef struct hm_nlist_args
     unsigned int* nlist;        /* [N*nmax] */
     unsigned int* n_neigh;      /* [N] */
     unsigned int* conditions;   /* [1] atomicMax'd: largest n_neigh when it exceeds nmax */
+    uint16_t* local_rows;       /* [N*nmax] or NULL: each neighbour's position in its home
+                                   cell's candidate sequence (adjacent cells in order, slots in
+                                   order), the row format of hm_pair_lj_cells */
 } hm_nlist_args;
 /* Full neighbour list: for each i every j != i with |r_ij|^2 <= (r_cut + r_buff)^2 in FP64,
  * candidates in adjacency-cell order and particle-index order within a cell.
@@ -183,6 +187,37 @@ typedef struct hm_pair_args
  * gpu_compute_pair_forces<EvaluatorPairLJ> (hoomd/md/PotentialPairGPU.cuh:579-656, kernel
  * :140-404) as instantiated by PotentialPairGPUKernel.cu.inc:19-21. */
 int hm_pair_lj(const hm_pair_args* args, const hm_lj_params* params, hm_stream_t stream);
+
+typedef struct hm_pair_cell_args
+{
+    unsigned int N;
+    const double* pos;
+    hm_box box;
+    unsigned int dim[3];
+    unsigned int cmax;
+    const unsigned int* cell_size;
+    const unsigned int* cell_idx;   /* from hm_cell_list_build */
+    const unsigned int* cell_adj;
+    unsigned int n_adj;
+    const uint16_t* local_rows;     /* from hm_nlist_build */
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double r_cut[HM_MAX_TYPES * HM_MAX_TYPES];
+    double r_on[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;
+    int compute_virial;
+    float* force;
+    float* virial;
+    unsigned int tpp;               /* 4, 8 or 16 lanes per particle; 0 = 8 */
+    unsigned int workgroups;        /* persistent cell loop size; 0 = one workgroup per cell */
+} hm_pair_cell_args;
+/* The same pair sums as hm_pair_lj, tiled by cell: one workgroup per home cell stages the
+ * adjacent cells' current positions in LDS and reads each particle's neighbours through its
+ * 16-bit local row. The cell list and local rows must come from the last build; positions may
+ * have moved since (the list buffer). */
+int hm_pair_lj_cells(const hm_pair_cell_args* args, const hm_lj_params* params,
+                     hm_stream_t stream);
 
 typedef struct hm_fene_params
 {
@@ -281,8 +316,12 @@ int hm_sys_set_method(hm_system* s, int method, double dt, double kT, const doub
 int hm_sys_set_sorter(hm_system* s, unsigned int period, unsigned int grid);
 /* Request the pressure tensor (virial) on every step (default off; on demand for logging). */
 int hm_sys_set_compute_virial(hm_system* s, int on);
-/* Launch configuration of the pair kernel (0 = keep/tuned). */
+/* Launch configuration of the neighbour-index gather pair kernel (0 = keep/tuned). */
 int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_size);
+/* Pair kernel used by the engine: kind 0 = cell-tiled (default; tpp 4, 8 or 16 lanes per
+ * particle, size = workgroups of the persistent cell loop, 0 = one per cell), kind 1 =
+ * neighbour-index gather (tpp, size = block size). */
+int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned int size);
 /* Steps per captured hipGraph chunk (0 disables graphs; default 100). */
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
 /* Advance nsteps timesteps. Returns HM_OK or a negative error (state is left at the last good

@@ -146,3 +146,78 @@ def rel_rms(got, ref):
     ref = np.asarray(ref, np.float64)
     den = np.sqrt(np.mean(ref**2))
     return np.sqrt(np.mean((got - ref) ** 2)) / den if den > 0 else np.sqrt(np.mean(got**2))
+
+
+class CellPipeline:
+    """Cell list (with slot index table) + neighbour list (with 16-bit local rows) built on the
+    GPU from build_pos; forces can then be evaluated at later positions with the same lists."""
+
+    def __init__(self, build_pos, L, r_cut, r_buff, cmax=64, nmax=None):
+        N = len(build_pos)
+        self.N, self.L, self.r_cut = N, L, r_cut
+        width = r_cut + r_buff
+        b = H.box(L)
+        dim = (C.c_uint * 3)()
+        H.check(H.lib().hm_cell_dims(C.byref(b), width, dim))
+        self.dim = tuple(dim)
+        ncell = dim[0] * dim[1] * dim[2]
+        self.n_adj = H.lib().hm_cell_n_adj(dim)
+        self.cmax = cmax
+        self.d_pos = dev(pos4(build_pos))
+        self.d_size = H.DeviceArray((ncell,), np.uint32)
+        self.d_idx = H.DeviceArray((ncell * cmax,), np.uint32, fill=0)
+        self.d_ent = H.DeviceArray((ncell * cmax * 4,), np.float64)
+        self.d_adj = H.DeviceArray((ncell * self.n_adj,), np.uint32)
+        self.d_cond = H.DeviceArray((4,), np.uint32, fill=0)
+        H.check(H.lib().hm_cell_adj(dim, self.d_adj.ptr, None))
+        a = H.HmCellArgs()
+        a.N, a.pos, a.box = N, self.d_pos.ptr, b
+        for k in range(3):
+            a.dim[k] = dim[k]
+        a.cmax, a.cell_size, a.entries, a.last_pos = cmax, self.d_size.ptr, self.d_ent.ptr, None
+        a.conditions, a.cell_idx = self.d_cond.ptr, self.d_idx.ptr
+        H.check(H.lib().hm_cell_list_build(C.byref(a), None), "cell")
+        if nmax is None:
+            rho = N / float(L) ** 3
+            nmax = int(np.ceil(rho * 4 / 3 * np.pi * width**3 * 1.5 + 16))
+            nmax = (nmax + 7) & ~7
+        self.nmax = nmax
+        self.d_nl = H.DeviceArray((N * nmax,), np.uint32, fill=0)
+        self.d_lr = H.DeviceArray((N * nmax,), np.uint16, fill=0)
+        self.d_nn = H.DeviceArray((N,), np.uint32, fill=0)
+        na = H.HmNlistArgs()
+        na.N, na.pos, na.box = N, self.d_pos.ptr, b
+        for k in range(3):
+            na.dim[k] = dim[k]
+        na.cmax, na.cell_size, na.entries = cmax, self.d_size.ptr, self.d_ent.ptr
+        na.cell_adj, na.n_adj, na.ntypes = self.d_adj.ptr, self.n_adj, 1
+        na.r_cut[0], na.r_buff = r_cut, r_buff
+        na.nmax, na.nlist, na.n_neigh = nmax, self.d_nl.ptr, self.d_nn.ptr
+        na.conditions, na.local_rows = self.d_cond.ptr + 12, self.d_lr.ptr
+        H.check(H.lib().hm_nlist_build(C.byref(na), None), "nlist")
+        H.check(H.lib().hm_synchronize())
+        self.cond = self.d_cond.numpy()
+
+    def lists(self):
+        return (self.d_nl.numpy().reshape(self.N, self.nmax), self.d_nn.numpy())
+
+    def forces(self, pos, mode=0, r_on=0.0, virial=True, tpp=8, workgroups=0):
+        N = self.N
+        d_pos = dev(pos4(pos))
+        d_f = H.DeviceArray((N * 4,), np.float32, fill=0)
+        d_v = H.DeviceArray((N * 6,), np.float32, fill=0)
+        a = H.HmPairCellArgs()
+        a.N, a.pos, a.box = N, d_pos.ptr, H.box(self.L)
+        for k in range(3):
+            a.dim[k] = self.dim[k]
+        a.cmax, a.cell_size, a.cell_idx = self.cmax, self.d_size.ptr, self.d_idx.ptr
+        a.cell_adj, a.n_adj, a.local_rows = self.d_adj.ptr, self.n_adj, self.d_lr.ptr
+        a.nmax, a.n_neigh, a.ntypes = self.nmax, self.d_nn.ptr, 1
+        a.r_cut[0], a.r_on[0], a.shift_mode, a.compute_virial = self.r_cut, r_on, mode, int(virial)
+        a.force, a.virial, a.tpp, a.workgroups = d_f.ptr, d_v.ptr, tpp, workgroups
+        P = (H.HmLJParams * 1)()
+        P[0].sigma, P[0].epsilon = 1.0, 1.0
+        H.check(H.lib().hm_pair_lj_cells(C.byref(a), P, None), "pair cells")
+        H.check(H.lib().hm_synchronize())
+        return (d_f.numpy().reshape(N, 4).astype(np.float64),
+                d_v.numpy().reshape(6, N).astype(np.float64))

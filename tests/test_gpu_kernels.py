@@ -399,3 +399,55 @@ def test_sfc_order_matches_oracle(H, grid):
     H.check(H.lib().hm_sfc_order(len(pos), d_pos.ptr, H.box(L), grid, order.ptr, None))
     H.check(H.lib().hm_synchronize())
     assert np.array_equal(order.numpy(), O.sort_order(O.Box(L), pos, grid))
+
+
+# ------------------------------------------------------------------ cell-tiled pair kernel
+from gpu_helpers import CellPipeline  # noqa: E402
+
+
+@pytest.mark.parametrize("mode", ["none", "shift", "xplor"])
+@pytest.mark.parametrize("tpp,wg", [(4, 0), (8, 0), (16, 0), (8, 512)])
+def test_pair_cells_within_tolerance_of_fp64(H, mode, tpp, wg):
+    pos, L = perturbed_fcc(16, 0.8442, r=0.08, seed=tpp + wg)
+    cp = CellPipeline(pos, L, 2.5, 0.4)
+    nl, nn = cp.lists()
+    m = {"none": 0, "shift": 1, "xplor": 2}[mode]
+    f, v = cp.forces(pos, mode=m, r_on=2.0, tpp=tpp, workgroups=wg)
+    rf, rv = _fp64_forces(pos, L, nl, nn, mode=mode, r_on=2.0)
+    errs = {"force": rel_rms(f[:, :3], rf[:, :3]), "energy": rel_rms(f[:, 3], rf[:, 3]),
+            "virial": rel_rms(v, rv)}
+    for k, e in errs.items():
+        assert e < FORCE_TOL, (k, e)
+
+
+def test_pair_cells_after_motion_and_wrap(H):
+    """Lists built at one configuration, forces at a later one (every displacement below half the
+    buffer, particles wrapped across the box edges): the cell-tiled kernel must use the current
+    positions in the right periodic frame."""
+    pos, L = perturbed_fcc(12, 0.8442, r=0.05, seed=21)
+    cp = CellPipeline(pos, L, 2.5, 0.4)
+    nl, nn = cp.lists()
+    rng = np.random.default_rng(21)
+    step = rng.normal(size=pos.shape)
+    step *= 0.19 / np.linalg.norm(step, axis=1)[:, None] * rng.uniform(0, 1, (len(pos), 1))
+    moved = pos + step
+    moved = (moved + L / 2) % L - L / 2
+    assert not O.distance_check(O.Box(L), moved, pos, 2.5, 0.4)
+    f, v = cp.forces(moved)
+    rf, rv = _fp64_forces(moved, L, nl, nn)
+    assert rel_rms(f[:, :3], rf[:, :3]) < FORCE_TOL
+    assert rel_rms(f[:, 3], rf[:, 3]) < FORCE_TOL
+    assert rel_rms(v, rv) < FORCE_TOL
+
+
+def test_pair_cells_fallback_path(H):
+    """More than 1024 candidates per home cell (long cutoff at liquid density) takes the
+    global-gather fallback inside the same kernel."""
+    pos, L = perturbed_fcc(10, 0.8442, r=0.05, seed=5)
+    cp = CellPipeline(pos, L, 3.3, 0.4, cmax=96)
+    nl, nn = cp.lists()
+    assert cp.cond[3] == 0
+    f, v = cp.forces(pos)
+    rf, rv = O.pair_lj(O.Box(L), pos, nl, nn, third_law=False, r_cut=3.3)
+    assert rel_rms(f[:, :3], rf[:, :3]) < FORCE_TOL
+    assert rel_rms(f[:, 3], rf[:, 3]) < FORCE_TOL
