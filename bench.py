@@ -59,10 +59,22 @@ def build_sim(hoomd, n_cells, rho, seed, sort_period=500):
     return sim, lj, pos, L
 
 
-def pair_roofline(sim, lj, N):
+PAIR_KERNELS = {
+    # kind: (kernel, bytes per neighbour-list entry, bytes per particle)
+    # 16-bit local rows; n_neigh + FP64 position + force, plus the cell's slot-index table
+    0: ("pair_cell_kernel<EvaluatorPairLJ>", 2, 4 + 32 + 16 + 4),
+    # 32-bit rows; n_neigh + FP64 position + force
+    1: ("pair_kernel<EvaluatorPairLJ>", 4, 4 + 32 + 16),
+    # 32-bit rows; n_neigh + FP32 cell-relative position + force
+    2: ("pair_rel_kernel<EvaluatorPairLJ>", 4, 4 + 16 + 16),
+}
+
+
+def pair_roofline(sim, lj, N, kind):
     """Average pair-kernel launch duration (HIP events on the engine stream) and the
-    algorithmic bytes of one launch: 4 B per neighbour-list entry + N * (4 n_neigh + 32 position
-    + 16 force) [+ 24 virial]."""
+    algorithmic bytes of one launch: each neighbour-list entry once plus, per particle, its
+    n_neigh, its own position and its force (PAIR_KERNELS); every neighbour position gathered
+    is some particle's own position, so it is counted once."""
     import ctypes as C
 
     from hoomd_amd import _lib
@@ -70,7 +82,8 @@ def pair_roofline(sim, lj, N):
     ent = C.c_uint64()
     _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, 200, C.byref(ms),
                                            C.byref(ent)), "time pair")
-    B = 4.0 * ent.value + N * (4 + 32 + 16)
+    _, per_entry, per_particle = PAIR_KERNELS[kind]
+    B = float(per_entry) * ent.value + N * per_particle
     achieved = B / (ms.value * 1e-3) / 1e9
     return ms.value, ent.value, B, achieved
 
@@ -116,6 +129,8 @@ def main():
     ap.add_argument("--pair-only", type=int, default=0,
                     help="profiling mode: after warmup launch only the pair kernel this many "
                          "times (for rocprofv3 counter passes) and exit")
+    ap.add_argument("--pair-kind", type=int, default=2, choices=(0, 1, 2),
+                    help="pair kernel: 0 cell-tiled, 1 FP64 gather, 2 FP32 cell-relative gather")
     ap.add_argument("--pair-tpp", type=int, default=0, help="pair kernel lanes per particle")
     ap.add_argument("--pair-block", type=int, default=0, help="pair kernel block size")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -149,8 +164,8 @@ def main():
     sim, lj, pos, L = build_sim(hoomd, n_cells, rho, a.seed)
     N = len(pos)
     sim.run(0)
-    if a.pair_tpp or a.pair_block:
-        _lib.check(_lib.lib().hm_sys_set_pair_launch(sim._system, a.pair_tpp, a.pair_block))
+    _lib.check(_lib.lib().hm_sys_set_pair_kernel(sim._system, a.pair_kind, a.pair_tpp,
+                                                 a.pair_block))
     log(f"[bench] warmup {a.warmup} steps")
     sim.run(a.warmup)
     _lib.check(_lib.lib().hm_synchronize())
@@ -175,7 +190,7 @@ def main():
     builds = int(st1[0] - st0[0])
     log(f"[bench] {tps:.1f} steps/s, {builds} neighbour-list builds in the timed steps")
 
-    ms, entries, B, achieved = pair_roofline(sim, lj, N)
+    ms, entries, B, achieved = pair_roofline(sim, lj, N, a.pair_kind)
     log(f"[bench] pair kernel {ms * 1e3:.1f} us, {B / 1e9:.3f} GB algorithmic, {achieved:.0f} GB/s")
     traffic = load_traffic(N)
 
@@ -208,7 +223,7 @@ def main():
                                "buffer=0.4, Bussi kT=1.2 tau=0, dt=0.005, Hilbert sort every 500",
                    "N": N, "box_L": L, "nlist_builds_in_timed_steps": builds,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "pair_kernel<EvaluatorPairLJ>",
+        "roofline": {"bound": "hbm", "kernel": PAIR_KERNELS[a.pair_kind][0],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": ms * 1e3, "algorithmic_bytes": B,

@@ -28,7 +28,8 @@ CFLAGS = [
     "-fPIC",
     "-Wall",
     "-Wno-unused-function",
-    # no packed FP32 instructions on this target (hardware notes)
+    # no packed FP32 instructions (v_pk_*_f32): some of their forms have given silently wrong
+    # results on these MI355X machines; hipcc would otherwise form them from float2-like code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{os.path.join(ROOT, 'include')}",
     f"-I{CSRC}",

@@ -109,6 +109,7 @@ struct hm_system
     hm_cell_entry* entries = nullptr;
     unsigned int* cell_idx = nullptr;
     unsigned short* lrow = nullptr;
+    float4* rel = nullptr;   // reference-cell relative FP32 positions (pair kernel kind 2)
     double rcut_mat[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     double rcut_max = 0.0;
     unsigned int max_excl = 0;
@@ -144,9 +145,13 @@ struct hm_system
     uint64_t timestep = 0;
     bool prepared = false;
     int compute_virial = 0;
-    PairLaunch pair_cfg{4, 512};      // gather kernel: lanes per particle, block size
+    PairLaunch pair_cfg{4, 256};      // gather kernels: lanes per particle, block size
     PairLaunch pair_cell_cfg{8, 0};   // cell-tiled kernel: lanes per particle, workgroups
-    int pair_kernel_kind = 0;         // 0 cell-tiled (default), 1 neighbour-index gather
+    // 0 cell-tiled, 1 neighbour-index gather of FP64 positions, 2 gather of FP32 positions
+    // relative to the build cell (needs 3..255 cells per axis, else kind 1 runs)
+    int pair_kernel_kind = 2;
+    bool autotune = true;             // sweep the pair launch configuration at the next prep
+    bool pair_tuned = false;
     unsigned int chunk = 100;
     bool use_graph = true;
     std::map<unsigned int, hipGraphExec_t> graphs;
@@ -190,6 +195,7 @@ struct hm_system
         dfree(entries);
         dfree(cell_idx);
         dfree(lrow);
+        dfree(rel);
         dfree(n_excl_tag);
         dfree(excl_tag);
         dfree(n_excl);
@@ -290,6 +296,8 @@ struct hm_system
             nmax = (nmax + 7) & ~7u;
             alloc_rows();
         }
+        if (!rel)
+            rel = dalloc<float4>(N);
         if (!n_neigh)
         {
             n_neigh = dalloc<unsigned int>(N);
@@ -389,10 +397,53 @@ struct hm_system
         k.counter = st;
     }
 
+    bool rel_ok() const
+    {
+        for (int d = 0; d < 3; d++)
+            if (dim[d] < 3 || dim[d] > 255)
+                return false;
+        return true;
+    }
+    int pair_kind() const { return pair_kernel_kind == 2 && !rel_ok() ? 1 : pair_kernel_kind; }
+    float4* rel_out() const { return pair_kind() == 2 ? rel : nullptr; }
+
     // ------------------------------------------------------------------ kernels of one step
     void launch_pair(ForceSlot& f, int with_virial)
     {
-        if (pair_kernel_kind == 1)
+        const int kind = pair_kind();
+        if (kind == 2)
+        {
+            PairRelArgs k;
+            memset(&k, 0, sizeof(k));
+            k.N = N;
+            k.pos = pos;
+            k.rel = rel;
+            k.box = box;
+            const RelFrame fr = make_rel_frame(rel, box, dim);
+            for (int d = 0; d < 3; d++)
+            {
+                k.dim[d] = dim[d];
+                k.wf[d] = fr.wf[d];
+            }
+            k.nlist = nlist;
+            k.nmax = nmax;
+            k.n_neigh = n_neigh;
+            k.ntypes = ntypes;
+            for (unsigned int q = 0; q < ntypes * ntypes; q++)
+            {
+                const double rc = f.rcut[q] > 0.0 ? f.rcut[q] : 0.0;
+                k.rcutsq[q] = rc * rc;
+                k.rcutsq_f[q] = (float)(rc * rc);
+                k.ronsq_f[q] = (float)(f.ron[q] * f.ron[q]);
+            }
+            k.shift_mode = f.mode;
+            k.compute_virial = with_virial;
+            k.force = f.f;
+            k.virial = f.v;
+            hchk(launch_pair_rel_lj(k, f.p, pair_cfg, stream), "pair force");
+            return;
+        }
+        if (kind == 1)
         {
             PairKernelArgs k;
             fill_pair_args(k, N, pos, hbox, nlist, nmax, n_neigh, ntypes, f.rcut, f.ron, f.mode,
@@ -500,13 +551,14 @@ struct hm_system
             a1.last = last;
             a1.maxsq = r_buff < 1e-6 ? -1.0 : distance_check_maxsq(rcut_max, r_buff);
             a1.flag_state = st;
+            a1.rel = make_rel_frame(rel_out(), box, dim);
         }
         hchk(launch_step_one(a1, stream), "step one");
         if (have_nl)
         {
             hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, st, stream),
+                                   &cond->cell_need, st, stream, rel_out()),
                  "cell build");
             hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, st, stream),
                  "cell order");
@@ -546,7 +598,7 @@ struct hm_system
             hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
             hchk(hipMemsetAsync(cell_size, 0, sizeof(unsigned int) * ncell, stream), "clear cells");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, nullptr, stream),
+                                   &cond->cell_need, nullptr, stream, rel_out()),
                  "cell build");
             hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
                  "read conditions");
@@ -579,6 +631,75 @@ struct hm_system
         throw HmError{HM_ERR_LAUNCH, "neighbour list did not converge"};
     }
 
+    // Autotuner (the role of Autotuner.h): every candidate launch configuration of the pair
+    // kernel in use is timed over 5 launches with HIP events on the engine stream, and the one
+    // with the smallest median launch time is kept -- the reference's protocol of sampling each
+    // parameter tuple 5 times and taking the median (Autotuner.h:503-664). Candidates: lanes per
+    // particle {1, 2, 4, 8, 16} x block size {64, 128, 256, 512, 1024} for the gather kernels,
+    // lanes per particle {4, 8, 16} for the cell-tiled kernel. The pair launches only rewrite
+    // the force arrays with the same values, so tuning does not change the state.
+    void tune_pair()
+    {
+        ForceSlot* f = nullptr;
+        for (auto& x : forces)
+            if (x.kind == 0)
+            {
+                f = &x;
+                break;
+            }
+        pair_tuned = true;
+        if (!f || N == 0)
+            return;
+        const int kind = pair_kind();
+        std::vector<PairLaunch> cands;
+        if (kind == 0)
+            for (unsigned int t : {4u, 8u, 16u})
+                cands.push_back(PairLaunch{t, 0});
+        else
+            for (unsigned int t : {1u, 2u, 4u, 8u, 16u})
+                for (unsigned int b : {64u, 128u, 256u, 512u, 1024u})
+                    cands.push_back(PairLaunch{t, b});
+        PairLaunch& cfg = kind == 0 ? pair_cell_cfg : pair_cfg;
+        hipEvent_t ev[6];
+        for (auto& e : ev)
+            hchk(hipEventCreate(&e), "event");
+        float best = 1e30f;
+        PairLaunch best_cfg = cfg;
+        tune_log.clear();
+        for (const PairLaunch& c : cands)
+        {
+            cfg = c;
+            launch_pair(*f, compute_virial);  // warm-up launch
+            for (int r = 0; r < 5; r++)
+            {
+                hchk(hipEventRecord(ev[r], stream), "event");
+                launch_pair(*f, compute_virial);
+            }
+            hchk(hipEventRecord(ev[5], stream), "event");
+            hchk(hipEventSynchronize(ev[5]), "event");
+            float t[5];
+            for (int r = 0; r < 5; r++)
+                hchk(hipEventElapsedTime(&t[r], ev[r], r < 4 ? ev[r + 1] : ev[5]), "event");
+            std::sort(t, t + 5);
+            tune_log.push_back({c.tpp, c.block, t[2]});
+            if (t[2] < best)
+            {
+                best = t[2];
+                best_cfg = c;
+            }
+        }
+        for (auto& e : ev)
+            (void)hipEventDestroy(e);
+        cfg = best_cfg;
+        clear_graphs();
+    }
+    struct TuneSample
+    {
+        unsigned int tpp, size;
+        float ms;
+    };
+    std::vector<TuneSample> tune_log;
+
     void prep()
     {
         if (method < 0)
@@ -600,6 +721,8 @@ struct hm_system
         update_exclusion_index();
         if (rcut_max > 0.0)
             build_nlist_now();
+        if (autotune && !pair_tuned)
+            tune_pair();
         launch_forces(compute_virial);
         const float4* fp[4];
         unsigned int nf = force_ptrs(fp);
@@ -1086,17 +1209,24 @@ int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_si
         s->pair_cfg.tpp = tpp;
     if (block_size)
         s->pair_cfg.block = block_size;
+    if (tpp || block_size)
+        s->autotune = false;
     s->clear_graphs();
     return HM_OK;
 }
 
 int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned int size)
 {
-    if (!s || kind < 0 || kind > 1)
+    if (!s || kind < 0 || kind > 2)
         return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: invalid arguments");
     if (kind == 0 && tpp && tpp != 4 && tpp != 8 && tpp != 16)
         return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: cell kernel tpp must be 4, 8 or 16");
     s->pair_kernel_kind = kind;
+    // the relative positions of kind 2 are written by the cell-list build: build again
+    s->prepared = false;
+    // tpp = size = 0: the autotuner picks the launch configuration at the next run
+    s->autotune = tpp == 0 && size == 0;
+    s->pair_tuned = false;
     if (kind == 0)
     {
         if (tpp)
@@ -1106,6 +1236,44 @@ int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned in
         return HM_OK;
     }
     return hm_sys_set_pair_launch(s, tpp, size);
+}
+
+int hm_sys_set_autotune(hm_system* s, int on)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_autotune: NULL system");
+    s->autotune = on != 0;
+    s->pair_tuned = false;
+    if (s->autotune)
+        s->prepared = false;
+    return HM_OK;
+}
+
+int hm_sys_pair_config(const hm_system* s, unsigned int out[4])
+{
+    if (!s || !out)
+        return fail(HM_ERR_ARGS, "hm_sys_pair_config: invalid arguments");
+    const int kind = s->pair_kind();
+    const PairLaunch& c = kind == 0 ? s->pair_cell_cfg : s->pair_cfg;
+    out[0] = (unsigned int)kind;
+    out[1] = c.tpp;
+    out[2] = c.block;
+    out[3] = s->pair_tuned ? 1u : 0u;
+    return HM_OK;
+}
+
+int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows)
+{
+    if (!s || (!out && max_rows))
+        return fail(HM_ERR_ARGS, "hm_sys_tune_log: invalid arguments");
+    unsigned int n = (unsigned int)std::min<size_t>(max_rows, s->tune_log.size());
+    for (unsigned int q = 0; q < n; q++)
+    {
+        out[3 * q + 0] = (float)s->tune_log[q].tpp;
+        out[3 * q + 1] = (float)s->tune_log[q].size;
+        out[3 * q + 2] = s->tune_log[q].ms;
+    }
+    return (int)s->tune_log.size();
 }
 
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph)

@@ -55,6 +55,36 @@ struct PairLaunch
     unsigned int block;
 };
 
+// Reference-cell relative positions: rel.xyz = position - origin of the cell the particle was
+// binned into at the last build (minimum image, FP32); rel.w = bits of
+// cx | cy << 8 | cz << 16 | type << 24. Needs 3 <= dim <= 255 per axis.
+struct RelFrame
+{
+    float4* rel;
+    unsigned int dim[3];
+    double w[3];      // cell widths (FP64)
+    float wf[3];      // cell widths rounded to FP32
+};
+
+__host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, unsigned int cz,
+                                              unsigned int type)
+{
+    return cx | (cy << 8) | (cz << 16) | (type << 24);
+}
+
+inline RelFrame make_rel_frame(float4* rel, const DevBox& box, const unsigned int dim[3])
+{
+    RelFrame f;
+    f.rel = rel;
+    for (int d = 0; d < 3; d++)
+    {
+        f.dim[d] = dim[d];
+        f.w[d] = dim[d] ? box.L[d] / (double)dim[d] : 0.0;
+        f.wf[d] = (float)f.w[d];
+    }
+    return f;
+}
+
 // ---------------------------------------------------------------- cell list / nlist
 hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStream_t s);
 
@@ -154,23 +184,6 @@ struct PairKernelArgs
 };
 hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params,
                           PairLaunch cfg, hipStream_t s);
-
-// Reference-cell relative positions: rel.xyz = position - origin of the cell the particle was
-// binned into at the last build (minimum image, FP32); rel.w = bits of
-// cx | cy << 8 | cz << 16 | type << 24. Needs 3 <= dim <= 255 per axis.
-struct RelFrame
-{
-    float4* rel;
-    unsigned int dim[3];
-    double w[3];      // cell widths (FP64)
-    float wf[3];      // cell widths rounded to FP32
-};
-
-__host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, unsigned int cz,
-                                              unsigned int type)
-{
-    return cx | (cy << 8) | (cz << 16) | (type << 24);
-}
 
 // Pair kernel reading one 16-byte relative position per neighbour (see RelFrame).
 struct PairRelArgs

@@ -102,7 +102,8 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
                                                          unsigned int dz, unsigned int cmax,
                                                          unsigned int* cell_size,
                                                          hm_cell_entry* entries, d3* last,
-                                                         unsigned int* cond, const StepState* gate)
+                                                         unsigned int* cond, const StepState* gate,
+                                                         float4* rel, RelFrame fr)
 {
 #pragma clang fp contract(off)
     if (gate_closed(gate))
@@ -142,6 +143,16 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
             continue;
         }
         unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
+        if (rel)
+        {
+            // position relative to the origin of the particle's cell, minimum image
+            double ox = p.x - (box.lo[0] + (double)ib * fr.w[0]);
+            double oy = p.y - (box.lo[1] + (double)jb * fr.w[1]);
+            double oz = p.z - (box.lo[2] + (double)kb * fr.w[2]);
+            min_image(box, ox, oy, oz);
+            rel[n] = make_float4((float)ox, (float)oy, (float)oz,
+                                 __uint_as_float(pack_cell(ib, jb, kb, (unsigned int)p.w)));
+        }
         unsigned int slot = atomicAdd(&cell_size[bin], 1u);
         if (slot < cmax)
         {
@@ -161,13 +172,15 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s)
+                             unsigned int* cond3, const StepState* gate, hipStream_t s,
+                             float4* rel)
 {
     if (N == 0)
         return hipSuccess;
     unsigned int blocks = min((N + 255) / 256, GATED_BLOCKS);
+    RelFrame fr = make_rel_frame(rel, box, dim);
     hipLaunchKernelGGL(cell_build_kernel, dim3(blocks), dim3(256), 0, s, N, pos, box, dim[0],
-                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
+                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate, rel, fr);
     return hipGetLastError();
 }
 
@@ -600,6 +613,17 @@ hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last
 // masked out arithmetically. The partial sums are combined by xor-shuffles inside the TPP group.
 constexpr int PAIR_BATCH = 8;
 
+// Row loads are unconditional (their index is clamped into the row) so that a batch's loads are
+// all in flight together. The empty asm consumes each loaded value at this point, which stops
+// the compiler from sinking a load into the branch of its later use, where every load would be
+// followed by its own wait.
+template<int B> __device__ __forceinline__ void pin_loads(unsigned int (&v)[B])
+{
+#pragma unroll
+    for (int q = 0; q < B; q++)
+        asm volatile("" : "+v"(v[q]));
+}
+
 template<class Eval> struct ParamTable
 {
     typename Eval::param_type p[HM_MAX_TYPES * HM_MAX_TYPES];
@@ -619,6 +643,7 @@ __global__ __launch_bounds__(1024) void pair_kernel(PairKernelArgs a, ParamTable
     const unsigned int n = active ? min(a.n_neigh[ii], a.nmax) : 0u;
     const unsigned int ti = MT ? (unsigned int)pi.w : 0u;
     const unsigned int jmax = a.N - 1;
+    const unsigned int elast = a.nmax - 1;
 
     float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
     float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
@@ -629,13 +654,15 @@ __global__ __launch_bounds__(1024) void pair_kernel(PairKernelArgs a, ParamTable
     // a cell, Hilbert-sorted indices) share few cache lines.
     for (unsigned int k0 = 0; k0 < n; k0 += TPP * PAIR_BATCH)
     {
+        // entries past n are replaced by the particle itself
         unsigned int js[PAIR_BATCH];
 #pragma unroll
         for (int q = 0; q < PAIR_BATCH; q++)
-        {
-            const unsigned int e = k0 + q * TPP + sub;
-            js[q] = e < n ? row[e] : ii;
-        }
+            js[q] = row[min(k0 + q * TPP + sub, elast)];
+        pin_loads<PAIR_BATCH>(js);
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+            js[q] = k0 + q * TPP + sub < n ? js[q] : ii;
         double px[PAIR_BATCH], py[PAIR_BATCH], pz[PAIR_BATCH];
         unsigned int tj[PAIR_BATCH];
 #pragma unroll
@@ -805,6 +832,239 @@ hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params, P
     return pair_dispatch<E>(a, tab, cfg, s);
 }
 
+// ======================================================================== pair forces, FP32 relative
+// The pair sums of pair_kernel with one 16-byte gather per neighbour: each particle carries its
+// position relative to the origin of the cell it was binned into at the last build (RelFrame).
+// Row entries were found in adjacent cells of that build, so the cell offset of a pair wraps to
+// {-1, 0, 1} per axis and the pair vector is (ri - rj) + offset * w in FP32 -- the continuous
+// displacement, i.e. the minimum image while every particle stays within r_buff / 2 of its
+// build position (the distance check's rebuild criterion). The cutoff is decided in FP32; pairs
+// whose FP32 r^2 lies within a guard band (four times the worst-case FP32 error) of r_cut^2
+// are re-decided from the FP64 positions with the FP64 minimum image, so the set of interacting
+// pairs is the one the FP64 CPU path selects.
+template<class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, ParamTable<Eval> tab)
+{
+    const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned int gt = lb * blockDim.x + threadIdx.x;
+    const unsigned int i = gt / TPP;
+    const unsigned int sub = gt % TPP;
+    const bool active = i < a.N;
+    const unsigned int ii = active ? i : a.N - 1;
+    const float4 ri = a.rel[ii];
+    const unsigned int n = active ? min(a.n_neigh[ii], a.nmax) : 0u;
+    const unsigned int ci = __float_as_uint(ri.w);
+    const int cix = (int)(ci & 255u), ciy = (int)((ci >> 8) & 255u), ciz = (int)((ci >> 16) & 255u);
+    const unsigned int ti = MT ? (ci >> 24) : 0u;
+    const unsigned int jmax = a.N - 1;
+    const unsigned int elast = a.nmax - 1;
+    const int dimx = (int)a.dim[0], dimy = (int)a.dim[1], dimz = (int)a.dim[2];
+
+    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+    float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
+    const unsigned int* row = a.nlist + (size_t)ii * a.nmax;
+
+    for (unsigned int k0 = 0; k0 < n; k0 += TPP * PAIR_BATCH)
+    {
+        unsigned int js[PAIR_BATCH];
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+            js[q] = row[min(k0 + q * TPP + sub, elast)];
+        pin_loads<PAIR_BATCH>(js);
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+            js[q] = k0 + q * TPP + sub < n ? min(js[q], jmax) : ii;
+        float4 rj[PAIR_BATCH];
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+            rj[q] = a.rel[js[q]];
+#pragma unroll
+        for (int q = 0; q < PAIR_BATCH; q++)
+        {
+            const unsigned int cj = __float_as_uint(rj[q].w);
+            int ox = cix - (int)(cj & 255u), oy = ciy - (int)((cj >> 8) & 255u),
+                oz = ciz - (int)((cj >> 16) & 255u);
+            ox += ox > 1 ? -dimx : (ox < -1 ? dimx : 0);
+            oy += oy > 1 ? -dimy : (oy < -1 ? dimy : 0);
+            oz += oz > 1 ? -dimz : (oz < -1 ? dimz : 0);
+            float dxf, dyf, dzf, rsq;
+            {
+#pragma clang fp contract(off)
+                dxf = (ri.x - rj[q].x) + (float)ox * a.wf[0];
+                dyf = (ri.y - rj[q].y) + (float)oy * a.wf[1];
+                dzf = (ri.z - rj[q].z) + (float)oz * a.wf[2];
+                rsq = dxf * dxf + dyf * dyf + dzf * dzf;
+            }
+            const unsigned int tp = MT ? ti * a.ntypes + (cj >> 24) : 0u;
+            const bool valid = k0 + q * TPP + sub < n;
+            const float rcf = a.rcutsq_f[tp];
+            bool in = valid && rsq < rcf;
+            if (valid && fabsf(rsq - rcf) <= a.guard[tp])
+            {
+#pragma clang fp contract(off)
+                // re-decide with the FP64 formula of the CPU path
+                const double4 pi = a.pos[ii];
+                const double4 pj = a.pos[js[q]];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                min_image(a.box, dx, dy, dz);
+                in = dx * dx + dy * dy + dz * dz < a.rcutsq[tp];
+            }
+            const float rsqf = in ? rsq : 1.0f;
+            Eval ev(rsqf, rcf, tab.p[tp]);
+            float fdr = 0.f, e = 0.f;
+            const bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
+            ev.evalForceAndEnergy(fdr, e, shift);
+            if (SHIFT == 2)
+            {
+                // XPLOR smoothing (PotentialPair.h:706-731)
+                const float ron = a.ronsq_f[tp];
+                if (rsqf >= ron && rsqf < rcf)
+                {
+                    float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
+                    float rmc = rsqf - rcf;
+                    float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
+                    float ds = 12.f * (rsqf - ron) * rmc * den;
+                    float e0 = e;
+                    e = e0 * sw;
+                    fdr = sw * fdr - ds * e0;
+                }
+            }
+            fdr = in ? fdr : 0.f;
+            e = in ? e : 0.f;
+            fx += dxf * fdr;
+            fy += dyf * fdr;
+            fz += dzf * fdr;
+            pe += 0.5f * e;
+            if (VIRIAL)
+            {
+                const float f2 = 0.5f * fdr;
+                vxx += f2 * dxf * dxf;
+                vxy += f2 * dxf * dyf;
+                vxz += f2 * dxf * dzf;
+                vyy += f2 * dyf * dyf;
+                vyz += f2 * dyf * dzf;
+                vzz += f2 * dzf * dzf;
+            }
+        }
+    }
+    if (TPP > 1)
+    {
+#pragma unroll
+        for (int off = TPP / 2; off > 0; off >>= 1)
+        {
+            fx += __shfl_xor(fx, off, TPP);
+            fy += __shfl_xor(fy, off, TPP);
+            fz += __shfl_xor(fz, off, TPP);
+            pe += __shfl_xor(pe, off, TPP);
+            if (VIRIAL)
+            {
+                vxx += __shfl_xor(vxx, off, TPP);
+                vxy += __shfl_xor(vxy, off, TPP);
+                vxz += __shfl_xor(vxz, off, TPP);
+                vyy += __shfl_xor(vyy, off, TPP);
+                vyz += __shfl_xor(vyz, off, TPP);
+                vzz += __shfl_xor(vzz, off, TPP);
+            }
+        }
+    }
+    if (active && sub == 0)
+    {
+        a.force[i] = make_float4(fx, fy, fz, pe);
+        if (VIRIAL)
+        {
+            const size_t P = a.N;
+            a.virial[0 * P + i] = vxx;
+            a.virial[1 * P + i] = vxy;
+            a.virial[2 * P + i] = vxz;
+            a.virial[3 * P + i] = vyy;
+            a.virial[4 * P + i] = vyz;
+            a.virial[5 * P + i] = vzz;
+        }
+    }
+}
+
+template<class Eval, int TPP, bool VIRIAL, bool MT>
+static void pair_rel_dispatch_shift(const PairRelArgs& a, const ParamTable<Eval>& tab, dim3 grid,
+                                    dim3 block, hipStream_t s)
+{
+    switch (a.shift_mode)
+    {
+    case 0:
+        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a, tab);
+        break;
+    case 1:
+        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 1>), grid, block, 0, s, a, tab);
+        break;
+    default:
+        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 2>), grid, block, 0, s, a, tab);
+        break;
+    }
+}
+
+template<class Eval, int TPP>
+static void pair_rel_dispatch_tpp(const PairRelArgs& a, const ParamTable<Eval>& tab,
+                                  unsigned int block, hipStream_t s)
+{
+    unsigned int threads = a.N * TPP;
+    unsigned int nblocks = round_up((threads + block - 1) / block, 8);
+    dim3 g(nblocks), b(block);
+    bool mt = a.ntypes > 1;
+    if (a.compute_virial)
+        mt ? pair_rel_dispatch_shift<Eval, TPP, true, true>(a, tab, g, b, s)
+           : pair_rel_dispatch_shift<Eval, TPP, true, false>(a, tab, g, b, s);
+    else
+        mt ? pair_rel_dispatch_shift<Eval, TPP, false, true>(a, tab, g, b, s)
+           : pair_rel_dispatch_shift<Eval, TPP, false, false>(a, tab, g, b, s);
+}
+
+hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* params,
+                              PairLaunch cfg, hipStream_t s)
+{
+    if (a_in.N == 0)
+        return hipSuccess;
+    using E = EvaluatorPairLJ<float>;
+    ParamTable<E> tab;
+    PairRelArgs a = a_in;
+    const unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+    // worst-case FP32 error of r^2 near r_cut^2: |rel| <= R = 1.5 w per axis (cell width plus
+    // the r_buff / 2 drift allowance, r_buff <= w), each rel rounded (u R), their difference
+    // (2 u R), the cell offset w rounded (u w) and the final sum (u (2 R + w))
+    const double u = 1.0 / 16777216.0;
+    double wmax = 0.0;
+    for (int d = 0; d < 3; d++)
+        wmax = fmax(wmax, (double)a.box.L[d] / (double)a.dim[d]);
+    const double R = 1.5 * wmax;
+    const double e_dx = u * (2.0 * R + 2.0 * R + wmax + 2.0 * R + wmax);
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        const double rc2 = a.rcutsq[q];
+        const double rc = sqrt(rc2);
+        a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rc * e_dx + 6.0 * u * rc2) + 1e-12);
+    }
+    const unsigned int block = cfg.block ? cfg.block : 256;
+    switch (cfg.tpp ? cfg.tpp : 4)
+    {
+    case 1:
+        pair_rel_dispatch_tpp<E, 1>(a, tab, block, s);
+        break;
+    case 2:
+        pair_rel_dispatch_tpp<E, 2>(a, tab, block, s);
+        break;
+    case 8:
+        pair_rel_dispatch_tpp<E, 8>(a, tab, block, s);
+        break;
+    case 16:
+        pair_rel_dispatch_tpp<E, 16>(a, tab, block, s);
+        break;
+    default:
+        pair_rel_dispatch_tpp<E, 4>(a, tab, block, s);
+        break;
+    }
+    return hipGetLastError();
+}
+
 // ======================================================================== cell-tiled pair forces
 // The same sums as pair_kernel, organised per home cell: the workgroup stages the current FP64
 // positions of the 27 adjacent cells' particles in LDS (shifted into the home cell's periodic
@@ -904,13 +1164,18 @@ __global__ __launch_bounds__(PC_BLOCK) void pair_cell_kernel(PairCellArgs a, Par
                 double px[PAIR_BATCH], py[PAIR_BATCH], pz[PAIR_BATCH];
                 unsigned int tj[PAIR_BATCH];
                 bool val[PAIR_BATCH];
+                unsigned int lr[PAIR_BATCH];
+#pragma unroll
+                for (int q = 0; q < PAIR_BATCH; q++)
+                    lr[q] = row[min(k0 + q * TPP + sub, a.nmax - 1)];
+                pin_loads<PAIR_BATCH>(lr);
 #pragma unroll
                 for (int q = 0; q < PAIR_BATCH; q++)
                 {
                     const unsigned int e = k0 + q * TPP + sub;
                     val[q] = e < n;
                     // local rows index [0, total); clamp so a stale row cannot leave the tables
-                    const unsigned int l = val[q] ? min((unsigned int)row[e], total - 1) : 0u;
+                    const unsigned int l = val[q] ? min(lr[q], total - 1) : 0u;
                     if (staged)
                     {
                         const unsigned int ls = val[q] ? l : home_off + (active ? hp : 0u);
@@ -1174,6 +1439,38 @@ hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* param
 // TwoStepConstantVolume::integrateStepOne (hoomd/md/TwoStepConstantVolume.cc:7-80), fused with
 // NeighborList::distanceCheck for step t + 1 (NeighborList.cc:1220-1289) and with clearing the
 // cell occupancy counters for a possible rebuild.
+// Position relative to the origin of the cell packed in cellw (the cell the particle was binned
+// into at the last cell-list build), minimum image, rounded to FP32; cellw is kept.
+__device__ inline float4 rel_of(const double4& p, float cellw, const DevBox& box,
+                                const RelFrame& f)
+{
+#pragma clang fp contract(off)
+    const unsigned int c = __float_as_uint(cellw);
+    double ox = p.x - (box.lo[0] + (double)(c & 255u) * f.w[0]);
+    double oy = p.y - (box.lo[1] + (double)((c >> 8) & 255u) * f.w[1]);
+    double oz = p.z - (box.lo[2] + (double)((c >> 16) & 255u) * f.w[2]);
+    min_image(box, ox, oy, oz);
+    return make_float4((float)ox, (float)oy, (float)oz, cellw);
+}
+
+__global__ __launch_bounds__(256) void rel_refresh_kernel(unsigned int N,
+                                                          const double4* __restrict__ pos,
+                                                          DevBox box, RelFrame f)
+{
+    const unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N)
+        f.rel[n] = rel_of(pos[n], f.rel[n].w, box, f);
+}
+
+hipError_t launch_rel_refresh(unsigned int N, const double4* pos, const DevBox& box,
+                              const RelFrame& f, hipStream_t s)
+{
+    if (N == 0 || !f.rel)
+        return hipSuccess;
+    hipLaunchKernelGGL(rel_refresh_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, box, f);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
 {
 #pragma clang fp contract(off)
@@ -1212,6 +1509,8 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
         a.vel[n] = v;
         a.pos[n] = p;
         a.image[n] = img;
+        if (a.rel.rel)
+            a.rel.rel[n] = rel_of(p, a.rel.rel[n].w, a.box, a.rel);
         if (a.last)
         {
             const d3 l = a.last[n];

@@ -111,6 +111,9 @@ SIGNATURES = {
     "hm_sys_set_compute_virial": (_i, [_p, _i]),
     "hm_sys_set_pair_launch": (_i, [_p, _u, _u]),
     "hm_sys_set_pair_kernel": (_i, [_p, _i, _u, _u]),
+    "hm_sys_set_autotune": (_i, [_p, _i]),
+    "hm_sys_pair_config": (_i, [_p, _p]),
+    "hm_sys_tune_log": (_i, [_p, _p, _u]),
     "hm_sys_set_chunk": (_i, [_p, _u, _i]),
     "hm_sys_run": (_i, [_p, _u64]),
     "hm_sys_timestep": (_u64, [_p]),

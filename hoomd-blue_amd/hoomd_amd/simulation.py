@@ -50,6 +50,24 @@ class Operations:
         self.add(op)
         return self
 
+    @property
+    def is_tuning_complete(self):
+        """True once the autotuner has chosen the pair kernel's launch configuration
+        (operations.py:346-373); the sweep runs at the start of the first run()."""
+        sim = self._simulation
+        if sim._system is None:
+            return False
+        out = np.zeros(4, np.uint32)
+        _lib.check(_lib.lib().hm_sys_pair_config(sim._system, out.ctypes.data), "pair config")
+        return bool(out[3])
+
+    def tune_kernel_parameters(self):
+        """Request a new autotuner sweep at the start of the next run() (operations.py:375-392)."""
+        sim = self._simulation
+        if sim._system is None:
+            raise RuntimeError("Call Simulation.run() before tune_kernel_parameters.")
+        _lib.check(_lib.lib().hm_sys_set_autotune(sim._system, 1), "autotune")
+
     def remove(self, op):
         for lst in (self.tuners, self.computes):
             if op in lst:

@@ -318,10 +318,24 @@ int hm_sys_set_sorter(hm_system* s, unsigned int period, unsigned int grid);
 int hm_sys_set_compute_virial(hm_system* s, int on);
 /* Launch configuration of the neighbour-index gather pair kernel (0 = keep/tuned). */
 int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_size);
-/* Pair kernel used by the engine: kind 0 = cell-tiled (default; tpp 4, 8 or 16 lanes per
- * particle, size = workgroups of the persistent cell loop, 0 = one per cell), kind 1 =
- * neighbour-index gather (tpp, size = block size). */
+/* Pair kernel used by the engine: kind 0 = cell-tiled (tpp 4, 8 or 16 lanes per particle,
+ * size = workgroups of the persistent cell loop, 0 = one per cell), kind 1 = neighbour-index
+ * gather of FP64 positions (tpp, size = block size), kind 2 (default) = neighbour-index gather
+ * of FP32 positions relative to the particle's cell at the last build, with the cutoff
+ * re-decided in FP64 near r_cut (tpp, size = block size; kind 1 runs when an axis has fewer
+ * than 3 or more than 255 cells). */
 int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned int size);
+/* Autotuner (replaces Autotuner.h, hoomd/Autotuner.h:503-664): when on (the default, and after
+ * hm_sys_set_pair_kernel with tpp = size = 0) the next run first times every launch
+ * configuration of the pair kernel in use (5 launches each, median) and keeps the fastest.
+ * An explicit tpp / block size turns it off. */
+int hm_sys_set_autotune(hm_system* s, int on);
+/* Pair launch in use: out[0] = kind actually run, out[1] = tpp, out[2] = block size (kind 1, 2)
+ * or workgroups (kind 0), out[3] = 1 when the autotuner has chosen it. */
+int hm_sys_pair_config(const hm_system* s, unsigned int out[4]);
+/* Autotuner samples of the last sweep: rows of (tpp, size, median ms), at most max_rows
+ * written; returns the number of rows of the sweep. */
+int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows);
 /* Steps per captured hipGraph chunk (0 disables graphs; default 100). */
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
 /* Advance nsteps timesteps. Returns HM_OK or a negative error (state is left at the last good
