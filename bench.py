@@ -176,6 +176,17 @@ def main():
                                                C.byref(ms), None))
         log(f"[bench] {a.pair_only} pair launches, {ms.value * 1e3:.1f} us each")
         return 0
+    pcfg = np.zeros(5, np.uint32)
+    _lib.check(_lib.lib().hm_sys_pair_config(sim._system, pcfg.ctypes.data))
+    launch = {"kind": int(pcfg[0]), "tpp": int(pcfg[1]), "block": int(pcfg[2]),
+              "batch": int(pcfg[4]), "autotuned": bool(pcfg[3])}
+    log(f"[bench] pair launch {launch}")
+    if launch["autotuned"]:
+        tl = np.zeros((64, 4), np.float32)
+        rows = _lib.lib().hm_sys_tune_log(sim._system, tl.ctypes.data, 64)
+        for r in tl[:max(rows, 0)]:
+            log(f"[bench] autotuner tpp={int(r[0])} block={int(r[1])} batch={int(r[2])} "
+                f"{r[3] * 1e3:.1f} us")
     st0 = np.zeros(6, np.uint64)
     _lib.check(_lib.lib().hm_sys_stats(sim._system, st0.ctypes.data))
     log(f"[bench] timing {a.steps} steps")
@@ -222,6 +233,7 @@ def main():
         "config": {"workload": f"LJ liquid N={N} (FCC {n_cells}^3x4), rho=0.8442, r_cut=2.5, "
                                "buffer=0.4, Bussi kT=1.2 tau=0, dt=0.005, Hilbert sort every 500",
                    "N": N, "box_L": L, "nlist_builds_in_timed_steps": builds,
+                   "pair_launch": launch,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": PAIR_KERNELS[a.pair_kind][0],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

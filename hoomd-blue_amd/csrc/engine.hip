@@ -109,7 +109,10 @@ struct hm_system
     hm_cell_entry* entries = nullptr;
     unsigned int* cell_idx = nullptr;
     unsigned short* lrow = nullptr;
-    float4* rel = nullptr;   // reference-cell relative FP32 positions (pair kernel kind 2)
+    // pair kernel kind 2: cell-slot relative FP32 positions [ncell * cmax], packed cell and
+    // slot of every particle [N] (RelFrame)
+    float4* rel = nullptr;
+    uint32_t *pcell = nullptr, *slot_of = nullptr;
     double rcut_mat[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     double rcut_max = 0.0;
     unsigned int max_excl = 0;
@@ -196,6 +199,8 @@ struct hm_system
         dfree(cell_idx);
         dfree(lrow);
         dfree(rel);
+        dfree(pcell);
+        dfree(slot_of);
         dfree(n_excl_tag);
         dfree(excl_tag);
         dfree(n_excl);
@@ -296,8 +301,13 @@ struct hm_system
             nmax = (nmax + 7) & ~7u;
             alloc_rows();
         }
-        if (!rel)
-            rel = dalloc<float4>(N);
+        if (!pcell)
+        {
+            pcell = dalloc<uint32_t>(N);
+            slot_of = dalloc<uint32_t>(N);
+            hchk(hipMemsetAsync(pcell, 0, sizeof(uint32_t) * N, stream), "clear");
+            hchk(hipMemsetAsync(slot_of, 0, sizeof(uint32_t) * N, stream), "clear");
+        }
         if (!n_neigh)
         {
             n_neigh = dalloc<unsigned int>(N);
@@ -322,9 +332,14 @@ struct hm_system
     {
         dfree(entries);
         dfree(cell_idx);
+        dfree(rel);
         entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
         cell_idx = dalloc<unsigned int>((size_t)ncell * cmax);
+        rel = dalloc<float4>((size_t)ncell * cmax);
         hchk(hipMemsetAsync(cell_idx, 0, sizeof(unsigned int) * ncell * cmax, stream), "clear");
+        hchk(hipMemsetAsync(rel, 0, sizeof(float4) * ncell * cmax, stream), "clear");
+        if (slot_of)  // slots of the old table may exceed the new one until the next build
+            hchk(hipMemsetAsync(slot_of, 0, sizeof(uint32_t) * N, stream), "clear");
     }
 
     // neighbour rows (32-bit particle indices) and local rows (16-bit staged positions);
@@ -333,9 +348,11 @@ struct hm_system
     {
         dfree(nlist);
         dfree(lrow);
-        nlist = dalloc<unsigned int>((size_t)N * nmax);
+        nlist = dalloc<unsigned int>((size_t)N * nmax + NLIST_PAD);
         lrow = dalloc<unsigned short>((size_t)N * nmax);
-        hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * N * nmax, stream), "clear nlist");
+        hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * ((size_t)N * nmax + NLIST_PAD),
+                            stream),
+             "clear nlist");
         hchk(hipMemsetAsync(lrow, 0, sizeof(unsigned short) * N * nmax, stream), "clear nlist");
     }
 
@@ -391,7 +408,8 @@ struct hm_system
         k.nmax = nmax;
         k.nlist = nlist;
         k.n_neigh = n_neigh;
-        k.lrow = lrow;
+        k.lrow = pair_kind() == 0 ? lrow : nullptr;  // local rows feed only the cell-tiled kernel
+        k.offset_codes = pair_kind() == 2 ? 1 : 0;
         k.cond = &cond->nlist_need;
         k.gate = gated ? st : nullptr;
         k.counter = st;
@@ -402,10 +420,13 @@ struct hm_system
         for (int d = 0; d < 3; d++)
             if (dim[d] < 3 || dim[d] > 255)
                 return false;
-        return true;
+        return (size_t)ncell * cmax <= OFFSET_CODE_MAX_N;
     }
     int pair_kind() const { return pair_kernel_kind == 2 && !rel_ok() ? 1 : pair_kernel_kind; }
-    float4* rel_out() const { return pair_kind() == 2 ? rel : nullptr; }
+    RelFrame rel_frame() const
+    {
+        return make_rel_frame(pair_kind() == 2 ? rel : nullptr, pcell, slot_of, box, dim);
+    }
 
     // ------------------------------------------------------------------ kernels of one step
     void launch_pair(ForceSlot& f, int with_virial)
@@ -418,8 +439,11 @@ struct hm_system
             k.N = N;
             k.pos = pos;
             k.rel = rel;
+            k.slot_of = slot_of;
+            k.part_of = cell_idx;
+            k.nslot = ncell * cmax;
             k.box = box;
-            const RelFrame fr = make_rel_frame(rel, box, dim);
+            const RelFrame fr = rel_frame();
             for (int d = 0; d < 3; d++)
             {
                 k.dim[d] = dim[d];
@@ -551,16 +575,17 @@ struct hm_system
             a1.last = last;
             a1.maxsq = r_buff < 1e-6 ? -1.0 : distance_check_maxsq(rcut_max, r_buff);
             a1.flag_state = st;
-            a1.rel = make_rel_frame(rel_out(), box, dim);
+            a1.rel = rel_frame();
         }
         hchk(launch_step_one(a1, stream), "step one");
         if (have_nl)
         {
             hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, st, stream, rel_out()),
+                                   &cond->cell_need, st, stream),
                  "cell build");
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, st, stream),
+            const RelFrame rf = rel_frame();
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, st, stream, &box, &rf),
                  "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, true);
@@ -598,7 +623,7 @@ struct hm_system
             hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
             hchk(hipMemsetAsync(cell_size, 0, sizeof(unsigned int) * ncell, stream), "clear cells");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, nullptr, stream, rel_out()),
+                                   &cond->cell_need, nullptr, stream),
                  "cell build");
             hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
                  "read conditions");
@@ -612,7 +637,9 @@ struct hm_system
                 grow_cmax(h_cond->cell_need);
                 continue;
             }
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, nullptr, stream),
+            const RelFrame rf = rel_frame();
+            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, nullptr, stream, &box,
+                                  &rf),
                  "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, false);
@@ -656,9 +683,11 @@ struct hm_system
             for (unsigned int t : {4u, 8u, 16u})
                 cands.push_back(PairLaunch{t, 0});
         else
-            for (unsigned int t : {1u, 2u, 4u, 8u, 16u})
-                for (unsigned int b : {64u, 128u, 256u, 512u, 1024u})
-                    cands.push_back(PairLaunch{t, b});
+            for (unsigned int bt : kind == 2 ? std::vector<unsigned int>{4u, 8u}
+                                             : std::vector<unsigned int>{8u})
+                for (unsigned int t : {1u, 2u, 4u, 8u, 16u})
+                    for (unsigned int b : {64u, 128u, 256u, 512u, 1024u})
+                        cands.push_back(PairLaunch{t, b, bt});
         PairLaunch& cfg = kind == 0 ? pair_cell_cfg : pair_cfg;
         hipEvent_t ev[6];
         for (auto& e : ev)
@@ -681,7 +710,7 @@ struct hm_system
             for (int r = 0; r < 5; r++)
                 hchk(hipEventElapsedTime(&t[r], ev[r], r < 4 ? ev[r + 1] : ev[5]), "event");
             std::sort(t, t + 5);
-            tune_log.push_back({c.tpp, c.block, t[2]});
+            tune_log.push_back({c.tpp, c.block, c.batch, t[2]});
             if (t[2] < best)
             {
                 best = t[2];
@@ -695,7 +724,7 @@ struct hm_system
     }
     struct TuneSample
     {
-        unsigned int tpp, size;
+        unsigned int tpp, size, batch;
         float ms;
     };
     std::vector<TuneSample> tune_log;
@@ -1249,7 +1278,7 @@ int hm_sys_set_autotune(hm_system* s, int on)
     return HM_OK;
 }
 
-int hm_sys_pair_config(const hm_system* s, unsigned int out[4])
+int hm_sys_pair_config(const hm_system* s, unsigned int out[5])
 {
     if (!s || !out)
         return fail(HM_ERR_ARGS, "hm_sys_pair_config: invalid arguments");
@@ -1259,6 +1288,7 @@ int hm_sys_pair_config(const hm_system* s, unsigned int out[4])
     out[1] = c.tpp;
     out[2] = c.block;
     out[3] = s->pair_tuned ? 1u : 0u;
+    out[4] = c.batch;
     return HM_OK;
 }
 
@@ -1269,9 +1299,10 @@ int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows)
     unsigned int n = (unsigned int)std::min<size_t>(max_rows, s->tune_log.size());
     for (unsigned int q = 0; q < n; q++)
     {
-        out[3 * q + 0] = (float)s->tune_log[q].tpp;
-        out[3 * q + 1] = (float)s->tune_log[q].size;
-        out[3 * q + 2] = s->tune_log[q].ms;
+        out[4 * q + 0] = (float)s->tune_log[q].tpp;
+        out[4 * q + 1] = (float)s->tune_log[q].size;
+        out[4 * q + 2] = (float)s->tune_log[q].batch;
+        out[4 * q + 3] = s->tune_log[q].ms;
     }
     return (int)s->tune_log.size();
 }
@@ -1457,10 +1488,24 @@ long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pair
              "download");
         hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
              "download");
+        // kind 2 rows hold cell slots with offset codes: map slots back to particles
+        const bool slots = s->pair_kind() == 2;
+        const unsigned int jmask = slots ? OFFSET_CODE_MASK : 0xffffffffu;
+        std::vector<unsigned int> part;
+        if (slots)
+        {
+            part.resize((size_t)s->ncell * s->cmax);
+            hchk(hipMemcpy(part.data(), s->cell_idx, sizeof(unsigned int) * part.size(),
+                           hipMemcpyDeviceToHost),
+                 "download");
+        }
         for (unsigned int i = 0; i < N; i++)
             for (unsigned int k = 0; k < nn[i] && k < s->nmax; k++)
             {
-                unsigned int a = tg[i], b = tg[nl[(size_t)i * s->nmax + k]];
+                unsigned int j = nl[(size_t)i * s->nmax + k] & jmask;
+                if (slots)
+                    j = part[j];
+                unsigned int a = tg[i], b = tg[j];
                 if (a < b)
                 {
                     if (pairs && out < max_pairs)

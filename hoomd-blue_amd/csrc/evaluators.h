@@ -92,6 +92,80 @@ template<typename Real> struct EvaluatorPairLJ
     Real rsq, rcutsq, lj1, lj2;
 };
 
+// Fast form of the pair-evaluator concept for the FP32 gather kernels. An entry holds the
+// per-type-pair constants, built once on the host from param_type; eval returns force / r and
+// half the pair energy, both exactly zero when the pair is not `in` range. The generic form
+// wraps the standard concept (constructor + evalForceAndEnergy), so any evaluator works;
+// evaluators may specialise it with the constants hoisted out of the per-pair work.
+template<class Eval> struct FastPair
+{
+    struct entry
+    {
+        typename Eval::param_type p;
+        float rcutsq;
+        int shift;
+    };
+    static entry make(const typename Eval::param_type& p, double rcutsq, int shift)
+    {
+        entry e;
+        e.p = p;
+        e.rcutsq = (float)rcutsq;
+        e.shift = shift;
+        return e;
+    }
+    template<bool SHIFTED = true>
+    __device__ __forceinline__ static void eval(float rsq, bool in, const entry& t, float& fdr,
+                                                float& ehalf)
+    {
+        Eval ev(in ? rsq : 1.0f, t.rcutsq, t.p);
+        float f = 0.f, e = 0.f;
+        const bool ok = ev.evalForceAndEnergy(f, e, t.shift != 0);
+        fdr = (in && ok) ? f : 0.f;
+        ehalf = (in && ok) ? 0.5f * e : 0.f;
+    }
+};
+
+// LJ: lj1 = 4 eps sigma^12, lj2 = 4 eps sigma^6 (EvaluatorPairLJ.h:162-166) pre-multiplied by
+// 12, 6 and 1/2, and the energy shift at r_cut (EvaluatorPairLJ.h:203-209) precomputed. A pair
+// with lj1 == 0 never contributes (evalForceAndEnergy returns false); the caller gives it a zero
+// cutoff. The reciprocal of an out-of-range pair is replaced by 0 before use, which zeroes
+// force and energy without a division by zero.
+template<> struct FastPair<EvaluatorPairLJ<float>>
+{
+    struct entry
+    {
+        float c12, c6, h1, h2, hcut;
+    };
+    static entry make(const EvaluatorPairLJ<float>::param_type& p, double rcutsq, int shift)
+    {
+        const double lj1 = (double)p.epsilon_x_4 * (double)p.sigma_6 * (double)p.sigma_6;
+        const double lj2 = (double)p.epsilon_x_4 * (double)p.sigma_6;
+        entry e;
+        e.c12 = (float)(12.0 * lj1);
+        e.c6 = (float)(6.0 * lj2);
+        e.h1 = (float)(0.5 * lj1);
+        e.h2 = (float)(0.5 * lj2);
+        e.hcut = 0.f;
+        if (shift && rcutsq > 0.0)
+        {
+            const double rc6inv = 1.0 / (rcutsq * rcutsq * rcutsq);
+            e.hcut = (float)(0.5 * rc6inv * (lj1 * rc6inv - lj2));
+        }
+        return e;
+    }
+    template<bool SHIFTED = true>
+    __device__ __forceinline__ static void eval(float rsq, bool in, const entry& t, float& fdr,
+                                                float& ehalf)
+    {
+        const float r2inv = in ? recip(rsq) : 0.f;
+        const float r6inv = r2inv * r2inv * r2inv;
+        fdr = r2inv * r6inv * (t.c12 * r6inv - t.c6);
+        ehalf = r6inv * (t.h1 * r6inv - t.h2);
+        if (SHIFTED)
+            ehalf -= in ? t.hcut : 0.f;
+    }
+};
+
 // FENE + WCA bond evaluator (hoomd/md/EvaluatorBondFENE.h:30-156).
 template<typename Real> struct EvaluatorBondFENE
 {

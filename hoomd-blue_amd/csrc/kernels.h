@@ -53,18 +53,46 @@ struct PairLaunch
 {
     unsigned int tpp;
     unsigned int block;
+    unsigned int batch = 8;   // row entries per lane per gather batch (relative kernel: 4 or 8)
 };
 
-// Reference-cell relative positions: rel.xyz = position - origin of the cell the particle was
-// binned into at the last build (minimum image, FP32); rel.w = bits of
-// cx | cy << 8 | cz << 16 | type << 24. Needs 3 <= dim <= 255 per axis.
+// Cell-slot relative positions. At every cell-list build the cell sort gives particle p the
+// slot c * cmax + k (cell c, k-th particle of the cell in index order) and writes
+//   rel[slot].xyz = position - origin of cell c (minimum image, FP32),
+//   rel[slot].w   = bits of pcell[p] = cx | cy << 8 | cz << 16 | type << 24,
+//   slot_of[p]    = slot.
+// Step one refreshes rel[slot_of[p]] from the new position every step; the cell (and slot) of a
+// particle stay those of the last build. A neighbour cell's particles are therefore contiguous
+// in rel. Needs 3 <= dim <= 255 per axis and ncell * cmax <= OFFSET_CODE_MAX_N.
 struct RelFrame
 {
-    float4* rel;
+    float4* rel;              // [ncell * cmax], slot order
+    const uint32_t* pcell;    // [N]
+    const uint32_t* slot_of;  // [N]
     unsigned int dim[3];
-    double w[3];      // cell widths (FP64)
-    float wf[3];      // cell widths rounded to FP32
+    double w[3];              // cell widths (FP64)
+    float wf[3];              // cell widths rounded to FP32
 };
+
+// Neighbour-row entries with offset codes: bits 0-25 particle index, bits 26-27 / 28-29 / 30-31
+// the x / y / z offset (home cell - neighbour cell) in {-1, 0, 1} as 2-bit two's complement.
+constexpr unsigned int OFFSET_CODE_SHIFT = 26;
+constexpr unsigned int OFFSET_CODE_MASK = (1u << OFFSET_CODE_SHIFT) - 1u;
+constexpr unsigned int OFFSET_CODE_MAX_N = 1u << OFFSET_CODE_SHIFT;
+// entries allocated after the last neighbour row, so batched row reads need no clamp
+constexpr unsigned int NLIST_PAD = 256;
+
+__host__ __device__ inline unsigned int offset_code(int ox, int oy, int oz)
+{
+    return (((unsigned int)ox & 3u) | (((unsigned int)oy & 3u) << 2)
+            | (((unsigned int)oz & 3u) << 4))
+           << OFFSET_CODE_SHIFT;
+}
+
+__host__ __device__ inline int wrap_offset(int d, int dim)
+{
+    return d > 1 ? d - dim : (d < -1 ? d + dim : d);
+}
 
 __host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, unsigned int cz,
                                               unsigned int type)
@@ -72,10 +100,13 @@ __host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, 
     return cx | (cy << 8) | (cz << 16) | (type << 24);
 }
 
-inline RelFrame make_rel_frame(float4* rel, const DevBox& box, const unsigned int dim[3])
+inline RelFrame make_rel_frame(float4* rel, const uint32_t* pcell, const uint32_t* slot_of,
+                               const DevBox& box, const unsigned int dim[3])
 {
     RelFrame f;
     f.rel = rel;
+    f.pcell = pcell;
+    f.slot_of = slot_of;
     for (int d = 0; d < 3; d++)
     {
         f.dim[d] = dim[d];
@@ -91,8 +122,7 @@ hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStre
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s,
-                             float4* rel = nullptr);
+                             unsigned int* cond3, const StepState* gate, hipStream_t s);
 
 struct NlistKernelArgs
 {
@@ -118,6 +148,11 @@ struct NlistKernelArgs
     unsigned int* nlist;
     unsigned int* n_neigh;
     unsigned short* lrow;     // optional: local rows (position in the home cell's candidates)
+    // slot mode: entries are cell slots (c * cmax + k, see RelFrame) carrying the candidate
+    // cell's offset from the home cell, wrapped to {-1, 0, 1} per axis, as 2-bit two's
+    // complement fields above the slot (OFFSET_CODE_SHIFT); needs 3 <= dim per axis and
+    // ncell * cmax <= OFFSET_CODE_MAX_N
+    int offset_codes;
     unsigned int* cond;       // needed nmax
     const StepState* gate;
     StepState* counter;       // n_builds incremented by block 0 when non-NULL
@@ -157,7 +192,8 @@ hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const 
 // order each cell's slots by particle index (deterministic candidate order)
 hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
                             hm_cell_entry* entries, unsigned int* cell_idx, const StepState* gate,
-                            hipStream_t s);
+                            hipStream_t s, const DevBox* box = nullptr,
+                            const RelFrame* rel = nullptr);
 hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
 
 hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,
@@ -190,7 +226,11 @@ struct PairRelArgs
 {
     unsigned int N;
     const double4* pos;      // FP64 positions for the guard-band re-decision of the cutoff
-    const float4* rel;
+    const float4* rel;       // slot order (RelFrame)
+    const uint32_t* slot_of; // [N]
+    const uint32_t* part_of; // [ncell * cmax]: particle of each slot (the cell slot table)
+    unsigned int nslot;      // ncell * cmax
+    // nlist holds N * nmax entries followed by at least NLIST_PAD readable (zeroed) entries
     DevBox box;
     unsigned int dim[3];
     float wf[3];
@@ -209,9 +249,6 @@ struct PairRelArgs
 };
 hipError_t launch_pair_rel_lj(const PairRelArgs& a, const hm_lj_params* params, PairLaunch cfg,
                               hipStream_t s);
-// (re)write rel for every particle from its current position and the cell stored in rel.w
-hipError_t launch_rel_refresh(unsigned int N, const double4* pos, const DevBox& box,
-                              const RelFrame& f, hipStream_t s);
 
 // ---------------------------------------------------------------- bond
 struct BondKernelArgs

@@ -102,8 +102,7 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
                                                          unsigned int dz, unsigned int cmax,
                                                          unsigned int* cell_size,
                                                          hm_cell_entry* entries, d3* last,
-                                                         unsigned int* cond, const StepState* gate,
-                                                         float4* rel, RelFrame fr)
+                                                         unsigned int* cond, const StepState* gate)
 {
 #pragma clang fp contract(off)
     if (gate_closed(gate))
@@ -143,16 +142,6 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
             continue;
         }
         unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
-        if (rel)
-        {
-            // position relative to the origin of the particle's cell, minimum image
-            double ox = p.x - (box.lo[0] + (double)ib * fr.w[0]);
-            double oy = p.y - (box.lo[1] + (double)jb * fr.w[1]);
-            double oz = p.z - (box.lo[2] + (double)kb * fr.w[2]);
-            min_image(box, ox, oy, oz);
-            rel[n] = make_float4((float)ox, (float)oy, (float)oz,
-                                 __uint_as_float(pack_cell(ib, jb, kb, (unsigned int)p.w)));
-        }
         unsigned int slot = atomicAdd(&cell_size[bin], 1u);
         if (slot < cmax)
         {
@@ -172,15 +161,13 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s,
-                             float4* rel)
+                             unsigned int* cond3, const StepState* gate, hipStream_t s)
 {
     if (N == 0)
         return hipSuccess;
     unsigned int blocks = min((N + 255) / 256, GATED_BLOCKS);
-    RelFrame fr = make_rel_frame(rel, box, dim);
     hipLaunchKernelGGL(cell_build_kernel, dim3(blocks), dim3(256), 0, s, N, pos, box, dim[0],
-                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate, rel, fr);
+                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
     return hipGetLastError();
 }
 
@@ -222,11 +209,25 @@ constexpr int NLB = 256;
 constexpr int NLCAP = 1024;
 constexpr int NLHOME = 256;
 
+// Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
+// minimum image, rounded to FP32; w keeps the packed word.
+__device__ inline float4 rel_of(double px, double py, double pz, uint32_t c, const DevBox& box,
+                                const RelFrame& f)
+{
+#pragma clang fp contract(off)
+    double ox = px - (box.lo[0] + (double)(c & 255u) * f.w[0]);
+    double oy = py - (box.lo[1] + (double)((c >> 8) & 255u) * f.w[1]);
+    double oz = pz - (box.lo[2] + (double)((c >> 16) & 255u) * f.w[2]);
+    min_image(box, ox, oy, oz);
+    return make_float4((float)ox, (float)oy, (float)oz, __uint_as_float(c));
+}
+
 __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsigned int cmax,
                                                         const unsigned int* __restrict__ cell_size,
                                                         hm_cell_entry* entries,
                                                         unsigned int* cell_idx,
-                                                        const StepState* gate)
+                                                        const StepState* gate, DevBox box,
+                                                        RelFrame rf)
 {
     __shared__ unsigned int sidx[4][NLHOME];
     __shared__ hm_cell_entry sent[4][NLHOME];
@@ -251,9 +252,20 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
             unsigned int rank = 0;
             for (unsigned int m = 0; m < n; m++)
                 rank += sidx[w][m] < my ? 1u : 0u;
-            entries[(size_t)c * cmax + rank] = sent[w][k];
+            const hm_cell_entry e = sent[w][k];
+            entries[(size_t)c * cmax + rank] = e;
             if (cell_idx)
                 cell_idx[(size_t)c * cmax + rank] = my;
+            if (rf.rel)
+            {
+                // cell-slot relative position, packed cell and slot of particle my
+                const unsigned int slot = c * cmax + rank;
+                const uint32_t pc = pack_cell(c % rf.dim[0], (c / rf.dim[0]) % rf.dim[1],
+                                              c / (rf.dim[0] * rf.dim[1]), e.type);
+                rf.rel[slot] = rel_of(e.x, e.y, e.z, pc, box, rf);
+                const_cast<uint32_t*>(rf.pcell)[my] = pc;
+                const_cast<uint32_t*>(rf.slot_of)[my] = slot;
+            }
         }
         __syncthreads();
     }
@@ -261,13 +273,20 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
 
 hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigned int* cell_size,
                             hm_cell_entry* entries, unsigned int* cell_idx, const StepState* gate,
-                            hipStream_t s)
+                            hipStream_t s, const DevBox* box, const RelFrame* rel)
 {
     if (ncell == 0)
         return hipSuccess;
     unsigned int blocks = min((ncell + 3) / 4, GATED_BLOCKS);
+    DevBox b{};
+    RelFrame rf{};
+    if (box && rel)
+    {
+        b = *box;
+        rf = *rel;
+    }
     hipLaunchKernelGGL(cell_sort_kernel, dim3(blocks), dim3(256), 0, s, ncell, cmax, cell_size,
-                       entries, cell_idx, gate);
+                       entries, cell_idx, gate, b, rf);
     return hipGetLastError();
 }
 
@@ -354,7 +373,8 @@ template<bool FAST, bool MT, bool EXCL> struct NlistShared
     unsigned char stp[MT ? NLCAP : 1];
     float4 sh[NLHOME];
     unsigned char shtp[MT ? NLHOME : 1];
-    unsigned int acell[27], asize[27], aoff[27];
+    unsigned int sslot[NLCAP];  // slot mode: candidate slot | offset code
+    unsigned int acell[27], asize[27], aoff[27], acode[27];
     unsigned int hcnt[NLHOME];
     unsigned int pass_last, pass_total;
 };
@@ -376,6 +396,18 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
         S.acell[tid] = c;
         S.asize[tid] = min(a.cell_size[c], a.cmax);
+        unsigned int code = 0;
+        if (a.offset_codes)
+        {
+            const int dx = (int)a.dim[0], dy = (int)a.dim[1];
+            const int hx = (int)(cell % a.dim[0]), hy = (int)((cell / a.dim[0]) % a.dim[1]),
+                      hz = (int)(cell / (a.dim[0] * a.dim[1]));
+            const int nx = (int)(c % a.dim[0]), ny = (int)((c / a.dim[0]) % a.dim[1]),
+                      nz = (int)(c / (a.dim[0] * a.dim[1]));
+            code = offset_code(wrap_offset(hx - nx, dx), wrap_offset(hy - ny, dy),
+                               wrap_offset(hz - nz, (int)a.dim[2]));
+        }
+        S.acode[tid] = code;
     }
     for (unsigned int s = tid; s < nhome; s += NLB)
     {
@@ -403,6 +435,8 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         {
             const unsigned int q = find_segment(S.aoff, first, last, k);
             const hm_cell_entry e = a.entries[(size_t)S.acell[q] * a.cmax + (k - S.aoff[q])];
+            if (a.offset_codes)
+                S.sslot[k] = (S.acell[q] * a.cmax + (k - S.aoff[q])) | S.acode[q];
             if (FAST)
             {
                 const float3 r = rel_to_centre(a.box, e.x, e.y, e.z, cx, cy, cz);
@@ -488,7 +522,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                     const unsigned int p = cnt + lanes_below(m);
                     if (p < a.nmax)
                     {
-                        row[p] = j;
+                        row[p] = a.offset_codes ? S.sslot[k] : j;
                         if (lrow)
                             lrow[p] = (unsigned short)(base_k + k);
                     }
@@ -834,116 +868,165 @@ hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params, P
 
 // ======================================================================== pair forces, FP32 relative
 // The pair sums of pair_kernel with one 16-byte gather per neighbour: each particle carries its
-// position relative to the origin of the cell it was binned into at the last build (RelFrame).
-// Row entries were found in adjacent cells of that build, so the cell offset of a pair wraps to
-// {-1, 0, 1} per axis and the pair vector is (ri - rj) + offset * w in FP32 -- the continuous
-// displacement, i.e. the minimum image while every particle stays within r_buff / 2 of its
-// build position (the distance check's rebuild criterion). The cutoff is decided in FP32; pairs
-// whose FP32 r^2 lies within a guard band (four times the worst-case FP32 error) of r_cut^2
-// are re-decided from the FP64 positions with the FP64 minimum image, so the set of interacting
-// pairs is the one the FP64 CPU path selects.
-template<class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
-__global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, ParamTable<Eval> tab)
+// position relative to the origin of the cell it was binned into at the last build (RelFrame),
+// and every row entry carries the offset between the two cells of that build, wrapped to
+// {-1, 0, 1} per axis (offset codes, written by the neighbour-list build). The pair vector is
+// (ri - rj) + offset * w in FP32 -- the continuous displacement, i.e. the minimum image while
+// every particle stays within r_buff / 2 of its build position (the distance check's rebuild
+// criterion). The cutoff is decided in FP32; pairs whose FP32 r^2 lies within a guard band
+// (four times the worst-case FP32 error) of r_cut^2 are re-decided from the FP64 positions with
+// the FP64 minimum image, so the set of interacting pairs is the one the FP64 CPU path selects.
+struct PairAcc
 {
+    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+    float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
+};
+
+// Per type pair constants of the relative kernel: the fast evaluator entry (modes none and
+// shift) and the standard parameters (XPLOR).
+template<class Eval> struct RelTable
+{
+    typename FastPair<Eval>::entry f[HM_MAX_TYPES * HM_MAX_TYPES];
+    typename Eval::param_type p[HM_MAX_TYPES * HM_MAX_TYPES];
+};
+
+// FP64 re-decision of r < r_cut for the pair (ii, particle of slot js), as the CPU path decides
+__device__ __forceinline__ bool rel_recheck(const PairRelArgs& a, unsigned int ii,
+                                            unsigned int js, unsigned int jmax, double rcutsq)
+{
+#pragma clang fp contract(off)
+    const double4 pi = a.pos[ii];
+    const double4 pj = a.pos[min(a.part_of[js], jmax)];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    min_image(a.box, dx, dy, dz);
+    return dx * dx + dy * dy + dz * dz < rcutsq;
+}
+
+// Force / r and half energy of one pair that is `in` range.
+template<class Eval, int SHIFT>
+__device__ __forceinline__ void rel_pair_eval(const PairRelArgs& a, const RelTable<Eval>& tab,
+                                              unsigned int tp, float rsq, bool in, float& fdr,
+                                              float& eh)
+{
+    if (SHIFT != 2)
+        FastPair<Eval>::template eval<SHIFT == 1>(rsq, in, tab.f[tp], fdr, eh);
+    else
+    {
+        // XPLOR smoothing (PotentialPair.h:706-731) on the standard evaluator
+        const float rcf = a.rcutsq_f[tp];
+        const float rsqf = in ? rsq : 1.0f;
+        Eval ev(rsqf, rcf, tab.p[tp]);
+        float en = 0.f;
+        fdr = 0.f;
+        ev.evalForceAndEnergy(fdr, en, a.ronsq_f[tp] > rcf);
+        const float ron = a.ronsq_f[tp];
+        if (rsqf >= ron && rsqf < rcf)
+        {
+            float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
+            float rmc = rsqf - rcf;
+            float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
+            float ds = 12.f * (rsqf - ron) * rmc * den;
+            float e0 = en;
+            en = e0 * sw;
+            fdr = sw * fdr - ds * e0;
+        }
+        fdr = in ? fdr : 0.f;
+        eh = in ? 0.5f * en : 0.f;
+    }
+}
+
+template<int B, class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<Eval> tab)
+{
+    // shift vectors of the 64 offset codes: code bits 0-1 / 2-3 / 4-5 hold the x / y / z
+    // offset in {-1, 0, 1} as 2-bit two's complement
+    __shared__ float4 shtab[64];
+    if (threadIdx.x < 64)
+    {
+        const int c = (int)threadIdx.x;
+        const int ox = (c << 30) >> 30, oy = (c << 28) >> 30, oz = (c << 26) >> 30;
+        shtab[c] = make_float4((float)ox * a.wf[0], (float)oy * a.wf[1], (float)oz * a.wf[2], 0.f);
+    }
+    __syncthreads();
     const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
     const unsigned int gt = lb * blockDim.x + threadIdx.x;
     const unsigned int i = gt / TPP;
     const unsigned int sub = gt % TPP;
     const bool active = i < a.N;
     const unsigned int ii = active ? i : a.N - 1;
-    const float4 ri = a.rel[ii];
+    const unsigned int smax = a.nslot - 1;
+    const unsigned int si = min(a.slot_of[ii], smax);
+    const float4 ri = a.rel[si];
     const unsigned int n = active ? min(a.n_neigh[ii], a.nmax) : 0u;
-    const unsigned int ci = __float_as_uint(ri.w);
-    const int cix = (int)(ci & 255u), ciy = (int)((ci >> 8) & 255u), ciz = (int)((ci >> 16) & 255u);
-    const unsigned int ti = MT ? (ci >> 24) : 0u;
+    const unsigned int ti = MT ? (__float_as_uint(ri.w) >> 24) : 0u;
     const unsigned int jmax = a.N - 1;
-    const unsigned int elast = a.nmax - 1;
-    const int dimx = (int)a.dim[0], dimy = (int)a.dim[1], dimz = (int)a.dim[2];
+    constexpr unsigned int S = TPP * B;
 
-    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
-    float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
-    const unsigned int* row = a.nlist + (size_t)ii * a.nmax;
-
-    for (unsigned int k0 = 0; k0 < n; k0 += TPP * PAIR_BATCH)
+    PairAcc acc;
+    // rows are followed by at least TPP * B readable entries (the list is padded), so a batch
+    // reads its entries unclamped; entries past n are stale but valid slots and are masked
+    const unsigned int* row = a.nlist + (size_t)ii * a.nmax + sub;
+    for (unsigned int k0 = 0; k0 < n; k0 += S)
     {
-        unsigned int js[PAIR_BATCH];
+        unsigned int ec[B];
 #pragma unroll
-        for (int q = 0; q < PAIR_BATCH; q++)
-            js[q] = row[min(k0 + q * TPP + sub, elast)];
-        pin_loads<PAIR_BATCH>(js);
+        for (int q = 0; q < B; q++)
+            ec[q] = row[k0 + q * TPP];
+        pin_loads<B>(ec);
+        unsigned int js[B];
+        float4 rj[B];
 #pragma unroll
-        for (int q = 0; q < PAIR_BATCH; q++)
-            js[q] = k0 + q * TPP + sub < n ? min(js[q], jmax) : ii;
-        float4 rj[PAIR_BATCH];
-#pragma unroll
-        for (int q = 0; q < PAIR_BATCH; q++)
-            rj[q] = a.rel[js[q]];
-#pragma unroll
-        for (int q = 0; q < PAIR_BATCH; q++)
+        for (int q = 0; q < B; q++)
         {
-            const unsigned int cj = __float_as_uint(rj[q].w);
-            int ox = cix - (int)(cj & 255u), oy = ciy - (int)((cj >> 8) & 255u),
-                oz = ciz - (int)((cj >> 16) & 255u);
-            ox += ox > 1 ? -dimx : (ox < -1 ? dimx : 0);
-            oy += oy > 1 ? -dimy : (oy < -1 ? dimy : 0);
-            oz += oz > 1 ? -dimz : (oz < -1 ? dimz : 0);
-            float dxf, dyf, dzf, rsq;
-            {
-#pragma clang fp contract(off)
-                dxf = (ri.x - rj[q].x) + (float)ox * a.wf[0];
-                dyf = (ri.y - rj[q].y) + (float)oy * a.wf[1];
-                dzf = (ri.z - rj[q].z) + (float)oz * a.wf[2];
-                rsq = dxf * dxf + dyf * dyf + dzf * dzf;
-            }
-            const unsigned int tp = MT ? ti * a.ntypes + (cj >> 24) : 0u;
+            js[q] = min(ec[q] & OFFSET_CODE_MASK, smax);
+            rj[q] = a.rel[js[q]];
+        }
+        // phase 1: pair vectors, FP32 cutoff decisions and the guard-band flags of the batch
+        float dx[B], dy[B], dz[B], rsq[B];
+        unsigned int tp[B];
+        bool in[B];
+        unsigned int chk = 0;
+#pragma unroll
+        for (int q = 0; q < B; q++)
+        {
+            const float4 sh = shtab[ec[q] >> OFFSET_CODE_SHIFT];
+            dx[q] = (ri.x - rj[q].x) + sh.x;
+            dy[q] = (ri.y - rj[q].y) + sh.y;
+            dz[q] = (ri.z - rj[q].z) + sh.z;
+            rsq[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
+            tp[q] = MT ? ti * a.ntypes + (__float_as_uint(rj[q].w) >> 24) : 0u;
             const bool valid = k0 + q * TPP + sub < n;
-            const float rcf = a.rcutsq_f[tp];
-            bool in = valid && rsq < rcf;
-            if (valid && fabsf(rsq - rcf) <= a.guard[tp])
-            {
-#pragma clang fp contract(off)
-                // re-decide with the FP64 formula of the CPU path
-                const double4 pi = a.pos[ii];
-                const double4 pj = a.pos[js[q]];
-                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                min_image(a.box, dx, dy, dz);
-                in = dx * dx + dy * dy + dz * dz < a.rcutsq[tp];
-            }
-            const float rsqf = in ? rsq : 1.0f;
-            Eval ev(rsqf, rcf, tab.p[tp]);
-            float fdr = 0.f, e = 0.f;
-            const bool shift = (SHIFT == 1) || (SHIFT == 2 && a.ronsq_f[tp] > rcf);
-            ev.evalForceAndEnergy(fdr, e, shift);
-            if (SHIFT == 2)
-            {
-                // XPLOR smoothing (PotentialPair.h:706-731)
-                const float ron = a.ronsq_f[tp];
-                if (rsqf >= ron && rsqf < rcf)
-                {
-                    float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
-                    float rmc = rsqf - rcf;
-                    float sw = rmc * rmc * (rcf + 2.f * rsqf - 3.f * ron) * den;
-                    float ds = 12.f * (rsqf - ron) * rmc * den;
-                    float e0 = e;
-                    e = e0 * sw;
-                    fdr = sw * fdr - ds * e0;
-                }
-            }
-            fdr = in ? fdr : 0.f;
-            e = in ? e : 0.f;
-            fx += dxf * fdr;
-            fy += dyf * fdr;
-            fz += dzf * fdr;
-            pe += 0.5f * e;
+            const float t = rsq[q] - a.rcutsq_f[tp[q]];
+            in[q] = valid && t < 0.f;
+            chk |= (valid && fabsf(t) <= a.guard[tp[q]]) ? (1u << q) : 0u;
+        }
+        // phase 2 (rare): FP64 re-decision of the pairs within the guard band
+        if (chk)
+        {
+#pragma unroll
+            for (int q = 0; q < B; q++)
+                if (chk & (1u << q))
+                    in[q] = rel_recheck(a, ii, js[q], jmax, a.rcutsq[tp[q]]);
+        }
+        // phase 3: evaluation and accumulation
+#pragma unroll
+        for (int q = 0; q < B; q++)
+        {
+            float fdr, eh;
+            rel_pair_eval<Eval, SHIFT>(a, tab, tp[q], rsq[q], in[q], fdr, eh);
+            acc.fx += dx[q] * fdr;
+            acc.fy += dy[q] * fdr;
+            acc.fz += dz[q] * fdr;
+            acc.pe += eh;
             if (VIRIAL)
             {
                 const float f2 = 0.5f * fdr;
-                vxx += f2 * dxf * dxf;
-                vxy += f2 * dxf * dyf;
-                vxz += f2 * dxf * dzf;
-                vyy += f2 * dyf * dyf;
-                vyz += f2 * dyf * dzf;
-                vzz += f2 * dzf * dzf;
+                acc.vxx += f2 * dx[q] * dx[q];
+                acc.vxy += f2 * dx[q] * dy[q];
+                acc.vxz += f2 * dx[q] * dz[q];
+                acc.vyy += f2 * dy[q] * dy[q];
+                acc.vyz += f2 * dy[q] * dz[q];
+                acc.vzz += f2 * dz[q] * dz[q];
             }
         }
     }
@@ -952,69 +1035,82 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, ParamTabl
 #pragma unroll
         for (int off = TPP / 2; off > 0; off >>= 1)
         {
-            fx += __shfl_xor(fx, off, TPP);
-            fy += __shfl_xor(fy, off, TPP);
-            fz += __shfl_xor(fz, off, TPP);
-            pe += __shfl_xor(pe, off, TPP);
+            acc.fx += __shfl_xor(acc.fx, off, TPP);
+            acc.fy += __shfl_xor(acc.fy, off, TPP);
+            acc.fz += __shfl_xor(acc.fz, off, TPP);
+            acc.pe += __shfl_xor(acc.pe, off, TPP);
             if (VIRIAL)
             {
-                vxx += __shfl_xor(vxx, off, TPP);
-                vxy += __shfl_xor(vxy, off, TPP);
-                vxz += __shfl_xor(vxz, off, TPP);
-                vyy += __shfl_xor(vyy, off, TPP);
-                vyz += __shfl_xor(vyz, off, TPP);
-                vzz += __shfl_xor(vzz, off, TPP);
+                acc.vxx += __shfl_xor(acc.vxx, off, TPP);
+                acc.vxy += __shfl_xor(acc.vxy, off, TPP);
+                acc.vxz += __shfl_xor(acc.vxz, off, TPP);
+                acc.vyy += __shfl_xor(acc.vyy, off, TPP);
+                acc.vyz += __shfl_xor(acc.vyz, off, TPP);
+                acc.vzz += __shfl_xor(acc.vzz, off, TPP);
             }
         }
     }
     if (active && sub == 0)
     {
-        a.force[i] = make_float4(fx, fy, fz, pe);
+        a.force[i] = make_float4(acc.fx, acc.fy, acc.fz, acc.pe);
         if (VIRIAL)
         {
             const size_t P = a.N;
-            a.virial[0 * P + i] = vxx;
-            a.virial[1 * P + i] = vxy;
-            a.virial[2 * P + i] = vxz;
-            a.virial[3 * P + i] = vyy;
-            a.virial[4 * P + i] = vyz;
-            a.virial[5 * P + i] = vzz;
+            a.virial[0 * P + i] = acc.vxx;
+            a.virial[1 * P + i] = acc.vxy;
+            a.virial[2 * P + i] = acc.vxz;
+            a.virial[3 * P + i] = acc.vyy;
+            a.virial[4 * P + i] = acc.vyz;
+            a.virial[5 * P + i] = acc.vzz;
         }
     }
 }
 
-template<class Eval, int TPP, bool VIRIAL, bool MT>
-static void pair_rel_dispatch_shift(const PairRelArgs& a, const ParamTable<Eval>& tab, dim3 grid,
+template<int B, class Eval, int TPP, bool VIRIAL, bool MT>
+static void pair_rel_dispatch_shift(const PairRelArgs& a, const RelTable<Eval>& tab, dim3 grid,
                                     dim3 block, hipStream_t s)
 {
     switch (a.shift_mode)
     {
     case 0:
-        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a, tab);
+        hipLaunchKernelGGL((pair_rel_kernel<B, Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a,
+                           tab);
         break;
     case 1:
-        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 1>), grid, block, 0, s, a, tab);
+        hipLaunchKernelGGL((pair_rel_kernel<B, Eval, TPP, VIRIAL, MT, 1>), grid, block, 0, s, a,
+                           tab);
         break;
     default:
-        hipLaunchKernelGGL((pair_rel_kernel<Eval, TPP, VIRIAL, MT, 2>), grid, block, 0, s, a, tab);
+        hipLaunchKernelGGL((pair_rel_kernel<B, Eval, TPP, VIRIAL, MT, 2>), grid, block, 0, s, a,
+                           tab);
         break;
     }
 }
 
+template<int B, class Eval, int TPP>
+static void pair_rel_dispatch_batch(const PairRelArgs& a, const RelTable<Eval>& tab, dim3 g,
+                                    dim3 b, hipStream_t s)
+{
+    const bool mt = a.ntypes > 1;
+    if (a.compute_virial)
+        mt ? pair_rel_dispatch_shift<B, Eval, TPP, true, true>(a, tab, g, b, s)
+           : pair_rel_dispatch_shift<B, Eval, TPP, true, false>(a, tab, g, b, s);
+    else
+        mt ? pair_rel_dispatch_shift<B, Eval, TPP, false, true>(a, tab, g, b, s)
+           : pair_rel_dispatch_shift<B, Eval, TPP, false, false>(a, tab, g, b, s);
+}
+
 template<class Eval, int TPP>
-static void pair_rel_dispatch_tpp(const PairRelArgs& a, const ParamTable<Eval>& tab,
-                                  unsigned int block, hipStream_t s)
+static void pair_rel_dispatch_tpp(const PairRelArgs& a, const RelTable<Eval>& tab,
+                                  unsigned int block, unsigned int batch, hipStream_t s)
 {
     unsigned int threads = a.N * TPP;
     unsigned int nblocks = round_up((threads + block - 1) / block, 8);
     dim3 g(nblocks), b(block);
-    bool mt = a.ntypes > 1;
-    if (a.compute_virial)
-        mt ? pair_rel_dispatch_shift<Eval, TPP, true, true>(a, tab, g, b, s)
-           : pair_rel_dispatch_shift<Eval, TPP, true, false>(a, tab, g, b, s);
+    if (batch == 4)
+        pair_rel_dispatch_batch<4, Eval, TPP>(a, tab, g, b, s);
     else
-        mt ? pair_rel_dispatch_shift<Eval, TPP, false, true>(a, tab, g, b, s)
-           : pair_rel_dispatch_shift<Eval, TPP, false, false>(a, tab, g, b, s);
+        pair_rel_dispatch_batch<8, Eval, TPP>(a, tab, g, b, s);
 }
 
 hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* params,
@@ -1023,11 +1119,20 @@ hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* param
     if (a_in.N == 0)
         return hipSuccess;
     using E = EvaluatorPairLJ<float>;
-    ParamTable<E> tab;
+    RelTable<E> tab;
     PairRelArgs a = a_in;
     const unsigned int nt = a.ntypes ? a.ntypes : 1;
     for (unsigned int q = 0; q < nt * nt; q++)
+    {
         tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+        tab.f[q] = FastPair<E>::make(tab.p[q], a.rcutsq[q], a.shift_mode == 1);
+        // lj1 == 0: evalForceAndEnergy never contributes -- a zero cutoff excludes the pair
+        if ((double)tab.p[q].epsilon_x_4 * (double)tab.p[q].sigma_6 == 0.0)
+        {
+            a.rcutsq[q] = 0.0;
+            a.rcutsq_f[q] = 0.f;
+        }
+    }
     // worst-case FP32 error of r^2 near r_cut^2: |rel| <= R = 1.5 w per axis (cell width plus
     // the r_buff / 2 drift allowance, r_buff <= w), each rel rounded (u R), their difference
     // (2 u R), the cell offset w rounded (u w) and the final sum (u (2 R + w))
@@ -1044,22 +1149,23 @@ hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* param
         a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rc * e_dx + 6.0 * u * rc2) + 1e-12);
     }
     const unsigned int block = cfg.block ? cfg.block : 256;
+    const unsigned int batch = cfg.batch == 4 ? 4u : 8u;
     switch (cfg.tpp ? cfg.tpp : 4)
     {
     case 1:
-        pair_rel_dispatch_tpp<E, 1>(a, tab, block, s);
+        pair_rel_dispatch_tpp<E, 1>(a, tab, block, batch, s);
         break;
     case 2:
-        pair_rel_dispatch_tpp<E, 2>(a, tab, block, s);
+        pair_rel_dispatch_tpp<E, 2>(a, tab, block, batch, s);
         break;
     case 8:
-        pair_rel_dispatch_tpp<E, 8>(a, tab, block, s);
+        pair_rel_dispatch_tpp<E, 8>(a, tab, block, batch, s);
         break;
     case 16:
-        pair_rel_dispatch_tpp<E, 16>(a, tab, block, s);
+        pair_rel_dispatch_tpp<E, 16>(a, tab, block, batch, s);
         break;
     default:
-        pair_rel_dispatch_tpp<E, 4>(a, tab, block, s);
+        pair_rel_dispatch_tpp<E, 4>(a, tab, block, batch, s);
         break;
     }
     return hipGetLastError();
@@ -1439,38 +1545,6 @@ hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* param
 // TwoStepConstantVolume::integrateStepOne (hoomd/md/TwoStepConstantVolume.cc:7-80), fused with
 // NeighborList::distanceCheck for step t + 1 (NeighborList.cc:1220-1289) and with clearing the
 // cell occupancy counters for a possible rebuild.
-// Position relative to the origin of the cell packed in cellw (the cell the particle was binned
-// into at the last cell-list build), minimum image, rounded to FP32; cellw is kept.
-__device__ inline float4 rel_of(const double4& p, float cellw, const DevBox& box,
-                                const RelFrame& f)
-{
-#pragma clang fp contract(off)
-    const unsigned int c = __float_as_uint(cellw);
-    double ox = p.x - (box.lo[0] + (double)(c & 255u) * f.w[0]);
-    double oy = p.y - (box.lo[1] + (double)((c >> 8) & 255u) * f.w[1]);
-    double oz = p.z - (box.lo[2] + (double)((c >> 16) & 255u) * f.w[2]);
-    min_image(box, ox, oy, oz);
-    return make_float4((float)ox, (float)oy, (float)oz, cellw);
-}
-
-__global__ __launch_bounds__(256) void rel_refresh_kernel(unsigned int N,
-                                                          const double4* __restrict__ pos,
-                                                          DevBox box, RelFrame f)
-{
-    const unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < N)
-        f.rel[n] = rel_of(pos[n], f.rel[n].w, box, f);
-}
-
-hipError_t launch_rel_refresh(unsigned int N, const double4* pos, const DevBox& box,
-                              const RelFrame& f, hipStream_t s)
-{
-    if (N == 0 || !f.rel)
-        return hipSuccess;
-    hipLaunchKernelGGL(rel_refresh_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, box, f);
-    return hipGetLastError();
-}
-
 __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
 {
 #pragma clang fp contract(off)
@@ -1510,7 +1584,7 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
         a.pos[n] = p;
         a.image[n] = img;
         if (a.rel.rel)
-            a.rel.rel[n] = rel_of(p, a.rel.rel[n].w, a.box, a.rel);
+            a.rel.rel[a.rel.slot_of[n]] = rel_of(px, py, pz, a.rel.pcell[n], a.box, a.rel);
         if (a.last)
         {
             const d3 l = a.last[n];

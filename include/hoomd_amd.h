@@ -331,9 +331,10 @@ int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned in
  * An explicit tpp / block size turns it off. */
 int hm_sys_set_autotune(hm_system* s, int on);
 /* Pair launch in use: out[0] = kind actually run, out[1] = tpp, out[2] = block size (kind 1, 2)
- * or workgroups (kind 0), out[3] = 1 when the autotuner has chosen it. */
-int hm_sys_pair_config(const hm_system* s, unsigned int out[4]);
-/* Autotuner samples of the last sweep: rows of (tpp, size, median ms), at most max_rows
+ * or workgroups (kind 0), out[3] = 1 when the autotuner has chosen it, out[4] = row entries per
+ * lane per gather batch (kind 2: 4 or 8). */
+int hm_sys_pair_config(const hm_system* s, unsigned int out[5]);
+/* Autotuner samples of the last sweep: rows of (tpp, size, batch, median ms), at most max_rows
  * written; returns the number of rows of the sweep. */
 int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows);
 /* Steps per captured hipGraph chunk (0 disables graphs; default 100). */

@@ -11,7 +11,7 @@
  * tests round exactly as written (the reference is compiled without FMA contraction on
  * x86-64 baseline targets).
  *
- * Pinning status (what checks each function; see tests/test_oracle_*.py):
+ * Pinning status (what checks each function; the checks are in tests/test_oracle.py):
  *   ob_* box functions ......... pinned: reference BoxDim.h compiled in oracle/_ref + closed forms
  *   oe_lj / oe_fene ............ pinned: forces_and_energies.json, test_bond.py vectors,
  *                                 reference evaluator headers compiled in oracle/_ref
@@ -23,7 +23,10 @@
  *                                 time reversibility
  *   orng_philox ................ pinned: Random123 published known-answer vectors
  *   orng_seed / counter / u01 .. pinned: random_numbers_test.cc known answers
- *   orng_normal / gamma ........ pinned: moments (statistical); stream itself unpinned
+ *   orng_normal / gamma ........ PARITY UNPINNED as streams: only their moments are checked
+ *                                 (the reference's RandomNumbers.h does not build here and its
+ *                                 tests hold no normal/gamma vectors); oi_bussi_alpha and
+ *                                 oi_thermalize inherit this
  *   os_hilbert_* ............... pinned: permutation/adjacency invariants of the Hilbert curve
  */
 #ifndef HOOMD_ORACLE_H

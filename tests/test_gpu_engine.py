@@ -317,16 +317,16 @@ def test_autotuner_sweeps_and_keeps_results(hoomd):
             sim.run(0)
             _lib.check(_lib.lib().hm_sys_set_pair_kernel(sim._system, 2, 16, 64))
         sim.run(30)
-        cfg = np.zeros(4, np.uint32)
+        cfg = np.zeros(5, np.uint32)
         _lib.check(_lib.lib().hm_sys_pair_config(sim._system, cfg.ctypes.data))
         if tuned:
             assert sim.operations.is_tuning_complete
-            log = np.zeros((32, 3), np.float32)
-            rows = _lib.lib().hm_sys_tune_log(sim._system, log.ctypes.data, 32)
-            assert rows == 25
-            best = log[np.argmin(log[:rows, 2])]
-            assert (cfg[1], cfg[2]) == (int(best[0]), int(best[1]))
-            assert np.all(log[:rows, 2] > 0)
+            log = np.zeros((64, 4), np.float32)
+            rows = _lib.lib().hm_sys_tune_log(sim._system, log.ctypes.data, 64)
+            assert rows == 50  # tpp x block size x batch (4, 8)
+            best = log[np.argmin(log[:rows, 3])]
+            assert (cfg[1], cfg[2], cfg[4]) == (int(best[0]), int(best[1]), int(best[2]))
+            assert np.all(log[:rows, 3] > 0)
         else:
             assert not sim.operations.is_tuning_complete
             assert (cfg[1], cfg[2]) == (16, 64)
