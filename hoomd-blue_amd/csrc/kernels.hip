@@ -206,6 +206,7 @@ hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const 
 // threshold, the band being four times the worst-case FP32 error; rows therefore equal the CPU
 // rows entry for entry. Grids with fewer than 4 cells along an axis use the FP64 kernel.
 constexpr int NLB = 256;
+constexpr int NLU = 4;  // candidate chunks of 64 per sweep iteration (FP32 path)
 constexpr int NLCAP = 1024;
 constexpr int NLHOME = 256;
 
@@ -473,35 +474,75 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             unsigned int cnt = S.hcnt[s];
             unsigned int* row = a.nlist + (size_t)i * a.nmax;
             unsigned short* lrow = a.lrow ? a.lrow + (size_t)i * a.nmax : nullptr;
-            for (unsigned int b = 0; b < total; b += 64)
+            if (FAST)
             {
-                const unsigned int k = b + lane;
-                bool hit = false;
-                unsigned int j = 0;
-                if (k < total)
+                // NLU chunks of 64 candidates per iteration: the LDS reads and distance tests of
+                // the chunks are independent, the rare guard-band re-decisions are taken out of
+                // the main path, then the hits are compacted chunk by chunk in candidate order
+                for (unsigned int b = 0; b < total; b += 64 * NLU)
                 {
-                    unsigned int tp = 0;
-                    if (FAST)
+                    float4 c4[NLU];
+                    unsigned int tp[NLU];
+                    bool hit[NLU];
+                    unsigned int chk = 0;
+#pragma unroll
+                    for (int u = 0; u < NLU; u++)
+                        c4[u] = S.sc[min(b + 64u * u + lane, total - 1)];
+#pragma unroll
+                    for (int u = 0; u < NLU; u++)
                     {
-                        const float4 c4 = S.sc[k];
-                        j = __float_as_uint(c4.w);
-                        const float dx = h4.x - c4.x, dy = h4.y - c4.y, dz = h4.z - c4.z;
+                        const bool valid = b + 64u * u + lane < total;
+                        const float dx = h4.x - c4[u].x, dy = h4.y - c4[u].y, dz = h4.z - c4[u].z;
                         const float dr2 = dx * dx + dy * dy + dz * dz;
-                        tp = MT ? ti * a.ntypes + S.stp[k] : 0u;
-                        const float rl = a.rlistsq_f[tp];
-                        hit = dr2 <= rl;
-                        if (fabsf(dr2 - rl) <= a.guard[tp])
-                        {
-                            // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
-                            const double4 pi = a.pos[i];
-                            const double4 pj = a.pos[j];
-                            double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
-                            min_image(a.box, ddx, ddy, ddz);
-                            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            hit = d2 <= a.rlistsq[tp];
-                        }
+                        tp[u] = MT ? ti * a.ntypes + S.stp[min(b + 64u * u + lane, total - 1)] : 0u;
+                        const float t = dr2 - a.rlistsq_f[tp[u]];
+                        hit[u] = valid && t <= 0.f;
+                        chk |= (valid && fabsf(t) <= a.guard[tp[u]]) ? (1u << u) : 0u;
                     }
-                    else
+                    if (chk)
+                    {
+#pragma unroll
+                        for (int u = 0; u < NLU; u++)
+                            if (chk & (1u << u))
+                            {
+                                // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
+                                const double4 pi = a.pos[i];
+                                const double4 pj = a.pos[__float_as_uint(c4[u].w)];
+                                double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                                min_image(a.box, ddx, ddy, ddz);
+                                hit[u] = ddx * ddx + ddy * ddy + ddz * ddz <= a.rlistsq[tp[u]];
+                            }
+                    }
+#pragma unroll
+                    for (int u = 0; u < NLU; u++)
+                    {
+                        const unsigned int j = __float_as_uint(c4[u].w);
+                        bool h = hit[u] && (j != i) && a.enabled[tp[u]];
+                        if (EXCL)
+                            h = h && j != ex0 && j != ex1 && j != ex2 && j != ex3;
+                        const uint64_t m = __ballot(h);
+                        if (h)
+                        {
+                            const unsigned int k = b + 64u * u + lane;
+                            const unsigned int p = cnt + lanes_below(m);
+                            if (p < a.nmax)
+                            {
+                                row[p] = a.offset_codes ? S.sslot[k] : j;
+                                if (lrow)
+                                    lrow[p] = (unsigned short)(base_k + k);
+                            }
+                        }
+                        cnt += (unsigned int)__popcll(m);
+                    }
+                }
+            }
+            else
+                for (unsigned int b = 0; b < total; b += 64)
+                {
+                    const unsigned int k = b + lane;
+                    bool hit = false;
+                    unsigned int j = 0;
+                    if (k < total)
                     {
                         j = S.sj[k];
                         double dx = hx - S.sx[k];
@@ -509,26 +550,25 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         double dz = hz - S.sz[k];
                         min_image(a.box, dx, dy, dz);
                         const double dr2 = dx * dx + dy * dy + dz * dz;
-                        tp = MT ? ti * a.ntypes + S.stp[k] : 0u;
+                        const unsigned int tp = MT ? ti * a.ntypes + S.stp[k] : 0u;
                         hit = dr2 <= a.rlistsq[tp];
+                        hit = hit && (j != i) && a.enabled[tp];
+                        if (EXCL)
+                            hit = hit && j != ex0 && j != ex1 && j != ex2 && j != ex3;
                     }
-                    hit = hit && (j != i) && a.enabled[tp];
-                    if (EXCL)
-                        hit = hit && j != ex0 && j != ex1 && j != ex2 && j != ex3;
-                }
-                const uint64_t m = __ballot(hit);
-                if (hit)
-                {
-                    const unsigned int p = cnt + lanes_below(m);
-                    if (p < a.nmax)
+                    const uint64_t m = __ballot(hit);
+                    if (hit)
                     {
-                        row[p] = a.offset_codes ? S.sslot[k] : j;
-                        if (lrow)
-                            lrow[p] = (unsigned short)(base_k + k);
+                        const unsigned int p = cnt + lanes_below(m);
+                        if (p < a.nmax)
+                        {
+                            row[p] = a.offset_codes ? S.sslot[k] : j;
+                            if (lrow)
+                                lrow[p] = (unsigned short)(base_k + k);
+                        }
                     }
+                    cnt += (unsigned int)__popcll(m);
                 }
-                cnt += (unsigned int)__popcll(m);
-            }
             if (lane == 0)
                 S.hcnt[s] = cnt;
         }
