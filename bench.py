@@ -67,6 +67,8 @@ PAIR_KERNELS = {
     1: ("pair_kernel<EvaluatorPairLJ>", 4, 4 + 32 + 16),
     # 32-bit rows; n_neigh + FP32 cell-relative position + force
     2: ("pair_rel_kernel<EvaluatorPairLJ>", 4, 4 + 16 + 16),
+    # 16-bit local rows; n_neigh + FP32 cell-relative position + cell slot table + force
+    3: ("pair_tile_kernel<EvaluatorPairLJ>", 2, 4 + 16 + 4 + 16),
 }
 
 
@@ -129,8 +131,9 @@ def main():
     ap.add_argument("--pair-only", type=int, default=0,
                     help="profiling mode: after warmup launch only the pair kernel this many "
                          "times (for rocprofv3 counter passes) and exit")
-    ap.add_argument("--pair-kind", type=int, default=2, choices=(0, 1, 2),
-                    help="pair kernel: 0 cell-tiled, 1 FP64 gather, 2 FP32 cell-relative gather")
+    ap.add_argument("--pair-kind", type=int, default=2, choices=(0, 1, 2, 3),
+                    help="pair kernel: 0 FP64 cell tiles, 1 FP64 gather, 2 FP32 cell-relative "
+                         "gather, 3 FP32 cell tiles")
     ap.add_argument("--pair-tpp", type=int, default=0, help="pair kernel lanes per particle")
     ap.add_argument("--pair-block", type=int, default=0, help="pair kernel block size")
     ap.add_argument("--copy-calibration", type=int, default=0,

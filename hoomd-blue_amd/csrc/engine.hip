@@ -150,8 +150,10 @@ struct hm_system
     int compute_virial = 0;
     PairLaunch pair_cfg{4, 256};      // gather kernels: lanes per particle, block size
     PairLaunch pair_cell_cfg{8, 0};   // cell-tiled kernel: lanes per particle, workgroups
-    // 0 cell-tiled, 1 neighbour-index gather of FP64 positions, 2 gather of FP32 positions
-    // relative to the build cell (needs 3..255 cells per axis, else kind 1 runs)
+    PairLaunch pair_tile_cfg{8, 2048, 8};  // FP32 cell tiles: lanes, workgroups, row batch
+    // 0 cell-tiled (FP64 staging), 1 neighbour-index gather of FP64 positions, 2 gather of FP32
+    // cell-slot relative positions, 3 cell-tiled FP32 relative positions (kinds 2 and 3 need
+    // 3..255 cells per axis, else kind 1 runs)
     int pair_kernel_kind = 2;
     bool autotune = true;             // sweep the pair launch configuration at the next prep
     bool pair_tuned = false;
@@ -349,11 +351,13 @@ struct hm_system
         dfree(nlist);
         dfree(lrow);
         nlist = dalloc<unsigned int>((size_t)N * nmax + NLIST_PAD);
-        lrow = dalloc<unsigned short>((size_t)N * nmax);
+        lrow = dalloc<unsigned short>((size_t)N * nmax + NLIST_PAD);
         hchk(hipMemsetAsync(nlist, 0, sizeof(unsigned int) * ((size_t)N * nmax + NLIST_PAD),
                             stream),
              "clear nlist");
-        hchk(hipMemsetAsync(lrow, 0, sizeof(unsigned short) * N * nmax, stream), "clear nlist");
+        hchk(hipMemsetAsync(lrow, 0, sizeof(unsigned short) * ((size_t)N * nmax + NLIST_PAD),
+                            stream),
+             "clear nlist");
     }
 
     void grow_nmax(unsigned int need)
@@ -408,7 +412,9 @@ struct hm_system
         k.nmax = nmax;
         k.nlist = nlist;
         k.n_neigh = n_neigh;
-        k.lrow = pair_kind() == 0 ? lrow : nullptr;  // local rows feed only the cell-tiled kernel
+        // local rows feed the cell-tiled kernels, which need no index rows
+        k.lrow = (pair_kind() == 0 || pair_kind() == 3) ? lrow : nullptr;
+        k.skip_index_rows = pair_kind() == 3 ? 1 : 0;
         k.offset_codes = pair_kind() == 2 ? 1 : 0;
         k.cond = &cond->nlist_need;
         k.gate = gated ? st : nullptr;
@@ -422,16 +428,110 @@ struct hm_system
                 return false;
         return (size_t)ncell * cmax <= OFFSET_CODE_MAX_N;
     }
-    int pair_kind() const { return pair_kernel_kind == 2 && !rel_ok() ? 1 : pair_kernel_kind; }
+    int pair_kind() const
+    {
+        return (pair_kernel_kind == 2 || pair_kernel_kind == 3) && !rel_ok() ? 1 : pair_kernel_kind;
+    }
+    bool uses_rel() const { return pair_kind() == 2 || pair_kind() == 3; }
+    // Pair list (tags, a < b) from the 16-bit local rows: entry l of a particle in home cell c
+    // is candidate l of c's adjacency-ordered sequence (cells in cell_adj order, slots in order)
+    long long pairs_from_local_rows(unsigned int* pairs, long long max_pairs)
+    {
+        std::vector<unsigned int> csize(ncell), adj((size_t)ncell * n_adj), cidx((size_t)ncell * cmax),
+            nn(N), tg(N), pc(N);
+        std::vector<unsigned short> lr((size_t)N * nmax);
+        hchk(hipStreamSynchronize(stream), "sync");
+        auto dl = [&](void* d, const void* sp, size_t b) {
+            hchk(hipMemcpy(d, sp, b, hipMemcpyDeviceToHost), "download");
+        };
+        dl(csize.data(), cell_size, sizeof(unsigned int) * csize.size());
+        dl(adj.data(), cell_adj, sizeof(unsigned int) * adj.size());
+        dl(cidx.data(), cell_idx, sizeof(unsigned int) * cidx.size());
+        dl(nn.data(), n_neigh, sizeof(unsigned int) * N);
+        dl(tg.data(), tag, sizeof(unsigned int) * N);
+        dl(pc.data(), pcell, sizeof(uint32_t) * N);
+        dl(lr.data(), lrow, sizeof(unsigned short) * lr.size());
+        long long out = 0;
+        std::vector<unsigned int> seq;
+        for (unsigned int i = 0; i < N; i++)
+        {
+            const uint32_t p = pc[i];
+            const unsigned int c = (p & 255u) + dim[0] * (((p >> 8) & 255u) + dim[1] * ((p >> 16) & 255u));
+            seq.clear();
+            for (unsigned int q = 0; q < n_adj; q++)
+            {
+                const unsigned int nc = adj[(size_t)c * n_adj + q];
+                const unsigned int sz = std::min(csize[nc], cmax);
+                for (unsigned int k = 0; k < sz; k++)
+                    seq.push_back(cidx[(size_t)nc * cmax + k]);
+            }
+            for (unsigned int k = 0; k < nn[i] && k < nmax; k++)
+            {
+                const unsigned int l = lr[(size_t)i * nmax + k];
+                if (l >= seq.size())
+                    throw HmError{HM_ERR_STATE, "local row entry outside the candidate sequence"};
+                const unsigned int a = tg[i], b = tg[seq[l]];
+                if (a < b)
+                {
+                    if (pairs && out < max_pairs)
+                    {
+                        pairs[2 * out] = a;
+                        pairs[2 * out + 1] = b;
+                    }
+                    out++;
+                }
+            }
+        }
+        return out;
+    }
+
     RelFrame rel_frame() const
     {
-        return make_rel_frame(pair_kind() == 2 ? rel : nullptr, pcell, slot_of, box, dim);
+        return make_rel_frame(uses_rel() ? rel : nullptr, pcell, slot_of, box, dim);
     }
 
     // ------------------------------------------------------------------ kernels of one step
     void launch_pair(ForceSlot& f, int with_virial)
     {
         const int kind = pair_kind();
+        if (kind == 3)
+        {
+            PairTileArgs k;
+            memset(&k, 0, sizeof(k));
+            k.N = N;
+            k.pos = pos;
+            k.rel = rel;
+            k.cell_idx = cell_idx;
+            k.cell_size = cell_size;
+            k.cell_adj = cell_adj;
+            k.n_adj = n_adj;
+            k.ncell = ncell;
+            k.cmax = cmax;
+            const RelFrame fr = rel_frame();
+            for (int d = 0; d < 3; d++)
+            {
+                k.dim[d] = dim[d];
+                k.wf[d] = fr.wf[d];
+            }
+            k.box = box;
+            k.lrow = lrow;
+            k.nmax = nmax;
+            k.n_neigh = n_neigh;
+            k.ntypes = ntypes;
+            for (unsigned int q = 0; q < ntypes * ntypes; q++)
+            {
+                const double rc = f.rcut[q] > 0.0 ? f.rcut[q] : 0.0;
+                k.rcutsq[q] = rc * rc;
+                k.rcutsq_f[q] = (float)(rc * rc);
+                k.ronsq_f[q] = (float)(f.ron[q] * f.ron[q]);
+            }
+            k.shift_mode = f.mode;
+            k.compute_virial = with_virial;
+            k.force = f.f;
+            k.virial = f.v;
+            hchk(launch_pair_tile_lj(k, f.p, pair_tile_cfg, stream), "pair force");
+            return;
+        }
         if (kind == 2)
         {
             PairRelArgs k;
@@ -682,13 +782,18 @@ struct hm_system
         if (kind == 0)
             for (unsigned int t : {4u, 8u, 16u})
                 cands.push_back(PairLaunch{t, 0});
+        else if (kind == 3)
+            for (unsigned int bt : {4u, 8u})
+                for (unsigned int t : {4u, 8u, 16u})
+                    for (unsigned int g : {1024u, 2048u, 4096u})
+                        cands.push_back(PairLaunch{t, g, bt});
         else
             for (unsigned int bt : kind == 2 ? std::vector<unsigned int>{4u, 8u}
                                              : std::vector<unsigned int>{8u})
                 for (unsigned int t : {1u, 2u, 4u, 8u, 16u})
                     for (unsigned int b : {64u, 128u, 256u, 512u, 1024u})
                         cands.push_back(PairLaunch{t, b, bt});
-        PairLaunch& cfg = kind == 0 ? pair_cell_cfg : pair_cfg;
+        PairLaunch& cfg = kind == 0 ? pair_cell_cfg : (kind == 3 ? pair_tile_cfg : pair_cfg);
         hipEvent_t ev[6];
         for (auto& e : ev)
             hchk(hipEventCreate(&e), "event");
@@ -1246,9 +1351,9 @@ int hm_sys_set_pair_launch(hm_system* s, unsigned int tpp, unsigned int block_si
 
 int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned int size)
 {
-    if (!s || kind < 0 || kind > 2)
+    if (!s || kind < 0 || kind > 3)
         return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: invalid arguments");
-    if (kind == 0 && tpp && tpp != 4 && tpp != 8 && tpp != 16)
+    if ((kind == 0 || kind == 3) && tpp && tpp != 4 && tpp != 8 && tpp != 16)
         return fail(HM_ERR_ARGS, "hm_sys_set_pair_kernel: cell kernel tpp must be 4, 8 or 16");
     s->pair_kernel_kind = kind;
     // the relative positions of kind 2 are written by the cell-list build: build again
@@ -1261,6 +1366,15 @@ int hm_sys_set_pair_kernel(hm_system* s, int kind, unsigned int tpp, unsigned in
         if (tpp)
             s->pair_cell_cfg.tpp = tpp;
         s->pair_cell_cfg.block = size;
+        s->clear_graphs();
+        return HM_OK;
+    }
+    if (kind == 3)
+    {
+        if (tpp)
+            s->pair_tile_cfg.tpp = tpp;
+        if (size)
+            s->pair_tile_cfg.block = size;
         s->clear_graphs();
         return HM_OK;
     }
@@ -1283,7 +1397,7 @@ int hm_sys_pair_config(const hm_system* s, unsigned int out[5])
     if (!s || !out)
         return fail(HM_ERR_ARGS, "hm_sys_pair_config: invalid arguments");
     const int kind = s->pair_kind();
-    const PairLaunch& c = kind == 0 ? s->pair_cell_cfg : s->pair_cfg;
+    const PairLaunch& c = kind == 0 ? s->pair_cell_cfg : (kind == 3 ? s->pair_tile_cfg : s->pair_cfg);
     out[0] = (unsigned int)kind;
     out[1] = c.tpp;
     out[2] = c.block;
@@ -1488,6 +1602,11 @@ long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pair
              "download");
         hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
              "download");
+        if (s->pair_kind() == 3)
+        {
+            out = s->pairs_from_local_rows(pairs, max_pairs);
+            return HM_OK;
+        }
         // kind 2 rows hold cell slots with offset codes: map slots back to particles
         const bool slots = s->pair_kind() == 2;
         const unsigned int jmask = slots ? OFFSET_CODE_MASK : 0xffffffffu;

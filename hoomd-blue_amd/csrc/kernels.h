@@ -153,6 +153,7 @@ struct NlistKernelArgs
     // complement fields above the slot (OFFSET_CODE_SHIFT); needs 3 <= dim per axis and
     // ncell * cmax <= OFFSET_CODE_MAX_N
     int offset_codes;
+    int skip_index_rows;      // write only the local rows (nlist untouched)
     unsigned int* cond;       // needed nmax
     const StepState* gate;
     StepState* counter;       // n_builds incremented by block 0 when non-NULL
@@ -249,6 +250,42 @@ struct PairRelArgs
 };
 hipError_t launch_pair_rel_lj(const PairRelArgs& a, const hm_lj_params* params, PairLaunch cfg,
                               hipStream_t s);
+
+// Cell-tiled form of the relative kernel: one workgroup per home cell stages the adjacent
+// cells' cell-slot relative positions in LDS, shifted into the home cell's frame, and reads
+// each particle's neighbours through its 16-bit local row (the position of the neighbour in
+// the home cell's adjacency-ordered candidate sequence, as the neighbour-list build writes it).
+struct PairTileArgs
+{
+    unsigned int N;
+    const double4* pos;       // FP64 positions for the guard-band re-decision of the cutoff
+    const float4* rel;        // slot order (RelFrame)
+    const uint32_t* cell_idx; // [ncell * cmax] particle of each slot
+    const unsigned int* cell_size;
+    const unsigned int* cell_adj;
+    unsigned int n_adj;       // 27 (3 <= dim per axis)
+    unsigned int ncell;
+    unsigned int cmax;
+    unsigned int dim[3];
+    float wf[3];
+    DevBox box;
+    const uint16_t* lrow;     // [N * nmax + NLIST_PAD]
+    unsigned int nmax;
+    const unsigned int* n_neigh;
+    unsigned int ntypes;
+    double rcutsq[HM_MAX_TYPES * HM_MAX_TYPES];
+    float rcutsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float ronsq_f[HM_MAX_TYPES * HM_MAX_TYPES];
+    float guard[HM_MAX_TYPES * HM_MAX_TYPES];
+    int shift_mode;
+    int compute_virial;
+    float4* force;
+    float* virial;
+};
+// cfg.tpp: lanes per particle (4, 8, 16); cfg.batch: entries per lane per row load (4, 8);
+// cfg.block: workgroups of the persistent cell loop (0 = 2048)
+hipError_t launch_pair_tile_lj(const PairTileArgs& a, const hm_lj_params* params,
+                               PairLaunch cfg, hipStream_t s);
 
 // ---------------------------------------------------------------- bond
 struct BondKernelArgs

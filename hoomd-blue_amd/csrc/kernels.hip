@@ -527,7 +527,8 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                             const unsigned int p = cnt + lanes_below(m);
                             if (p < a.nmax)
                             {
-                                row[p] = a.offset_codes ? S.sslot[k] : j;
+                                if (!a.skip_index_rows)
+                                    row[p] = a.offset_codes ? S.sslot[k] : j;
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
                             }
@@ -562,7 +563,8 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         const unsigned int p = cnt + lanes_below(m);
                         if (p < a.nmax)
                         {
-                            row[p] = a.offset_codes ? S.sslot[k] : j;
+                            if (!a.skip_index_rows)
+                                row[p] = a.offset_codes ? S.sslot[k] : j;
                             if (lrow)
                                 lrow[p] = (unsigned short)(base_k + k);
                         }
@@ -944,22 +946,22 @@ __device__ __forceinline__ bool rel_recheck(const PairRelArgs& a, unsigned int i
 
 // Force / r and half energy of one pair that is `in` range.
 template<class Eval, int SHIFT>
-__device__ __forceinline__ void rel_pair_eval(const PairRelArgs& a, const RelTable<Eval>& tab,
-                                              unsigned int tp, float rsq, bool in, float& fdr,
-                                              float& eh)
+__device__ __forceinline__ void rel_pair_eval(const float* rcutsq_f, const float* ronsq_f,
+                                              const RelTable<Eval>& tab, unsigned int tp,
+                                              float rsq, bool in, float& fdr, float& eh)
 {
     if (SHIFT != 2)
         FastPair<Eval>::template eval<SHIFT == 1>(rsq, in, tab.f[tp], fdr, eh);
     else
     {
         // XPLOR smoothing (PotentialPair.h:706-731) on the standard evaluator
-        const float rcf = a.rcutsq_f[tp];
+        const float rcf = rcutsq_f[tp];
         const float rsqf = in ? rsq : 1.0f;
         Eval ev(rsqf, rcf, tab.p[tp]);
         float en = 0.f;
         fdr = 0.f;
-        ev.evalForceAndEnergy(fdr, en, a.ronsq_f[tp] > rcf);
-        const float ron = a.ronsq_f[tp];
+        ev.evalForceAndEnergy(fdr, en, ronsq_f[tp] > rcf);
+        const float ron = ronsq_f[tp];
         if (rsqf >= ron && rsqf < rcf)
         {
             float den = 1.f / ((rcf - ron) * (rcf - ron) * (rcf - ron));
@@ -1053,7 +1055,7 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
         for (int q = 0; q < B; q++)
         {
             float fdr, eh;
-            rel_pair_eval<Eval, SHIFT>(a, tab, tp[q], rsq[q], in[q], fdr, eh);
+            rel_pair_eval<Eval, SHIFT>(a.rcutsq_f, a.ronsq_f, tab, tp[q], rsq[q], in[q], fdr, eh);
             acc.fx += dx[q] * fdr;
             acc.fy += dy[q] * fdr;
             acc.fz += dz[q] * fdr;
@@ -1206,6 +1208,386 @@ hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* param
         break;
     default:
         pair_rel_dispatch_tpp<E, 4>(a, tab, block, batch, s);
+        break;
+    }
+    return hipGetLastError();
+}
+
+// ======================================================================== pair forces, FP32 cell tiles
+// PotentialPair::computeForces (PotentialPair.h:587-794) organised per home cell. The workgroup
+// stages the cell-slot relative positions of the 27 adjacent cells (adjacency order, slots in
+// order -- the candidate sequence of the neighbour-list build) in LDS, each shifted by
+// -offset * w into the home cell's frame, so a pair vector is r_i - r_j with both operands in
+// LDS. Rows are the build's 16-bit local rows, read as B contiguous entries per lane. The
+// cutoff is decided as in pair_rel_kernel: FP32 with FP64 re-decision inside the guard band.
+constexpr unsigned int TCAP = 1024;
+constexpr unsigned int TBLOCK = 256;
+
+struct TileShared
+{
+    float4 sp[TCAP];          // staged home-frame positions; w = packed cell | type
+    unsigned int sidx[TCAP];  // particle index of each staged candidate
+    float4 ash[27];           // -offset * w per adjacency slot
+    unsigned int aoff[28], acell[27];
+    unsigned int qhome;
+};
+
+// candidate l of the home cell's sequence; staged in LDS when l < TCAP
+template<bool ALL>
+__device__ __forceinline__ void tile_cand(const PairTileArgs& a, const TileShared& S,
+                                          unsigned int l, float4& r, unsigned int& j)
+{
+    if (ALL || l < TCAP)
+    {
+        r = S.sp[l];
+        j = S.sidx[l];
+        return;
+    }
+    const unsigned int q = find_segment(S.aoff, 0, a.n_adj, l);
+    const unsigned int slot = S.acell[q] * a.cmax + (l - S.aoff[q]);
+    const float4 rr = a.rel[slot];
+    const float4 sh = S.ash[q];
+    r = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z, rr.w);
+    j = a.cell_idx[slot];
+}
+
+// CH contiguous 16-bit row entries of one lane, loaded as CH / 8 16-byte vectors
+template<int CH> struct RowChunk
+{
+    uint4 v[CH / 8];
+};
+
+template<int CH>
+__device__ __forceinline__ void load_chunk(const uint16_t* p, RowChunk<CH>& c)
+{
+#pragma unroll
+    for (int w = 0; w < CH / 8; w++)
+        c.v[w] = reinterpret_cast<const uint4*>(p)[w];
+}
+
+template<int CH>
+__device__ __forceinline__ unsigned int chunk_entry(const RowChunk<CH>& c, int q)
+{
+    const uint4 v = c.v[q >> 3];
+    const int d = (q & 7) >> 1;
+    const unsigned int w = d == 0 ? v.x : (d == 1 ? v.y : (d == 2 ? v.z : v.w));
+    return (q & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// the entries [k0 + sub * CH, k0 + sub * CH + CH) of one home particle's row
+template<bool ALL, int CH, class Eval, bool VIRIAL, bool MT, int SHIFT>
+__device__ __forceinline__ void tile_entries(const PairTileArgs& a, const RelTable<Eval>& tab,
+                                             const TileShared& S, const float4& ri,
+                                             unsigned int i, unsigned int ti, unsigned int n,
+                                             unsigned int kb, const RowChunk<CH>& c,
+                                             unsigned int lmax, unsigned int jmax, PairAcc& acc)
+{
+#pragma unroll
+    for (int g = 0; g < CH; g += 4)
+    {
+        float dx[4], dy[4], dz[4], rsq[4];
+        unsigned int tp[4], jj[4];
+        bool in[4];
+        unsigned int chk = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            float4 rj;
+            tile_cand<ALL>(a, S, min(chunk_entry<CH>(c, g + u), lmax), rj, jj[u]);
+            dx[u] = ri.x - rj.x;
+            dy[u] = ri.y - rj.y;
+            dz[u] = ri.z - rj.z;
+            rsq[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            tp[u] = MT ? ti * a.ntypes + (__float_as_uint(rj.w) >> 24) : 0u;
+            const bool valid = kb + g + u < n;
+            const float t = rsq[u] - a.rcutsq_f[tp[u]];
+            in[u] = valid && t < 0.f;
+            chk |= (valid && fabsf(t) <= a.guard[tp[u]]) ? (1u << u) : 0u;
+        }
+        if (chk)
+        {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (chk & (1u << u))
+                {
+#pragma clang fp contract(off)
+                    // re-decide with the FP64 formula of the CPU path
+                    const double4 pi = a.pos[i];
+                    const double4 pj = a.pos[min(jj[u], jmax)];
+                    double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                    min_image(a.box, ddx, ddy, ddz);
+                    in[u] = ddx * ddx + ddy * ddy + ddz * ddz < a.rcutsq[tp[u]];
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            float fdr, eh;
+            rel_pair_eval<Eval, SHIFT>(a.rcutsq_f, a.ronsq_f, tab, tp[u], rsq[u], in[u], fdr,
+                                       eh);
+            acc.fx += dx[u] * fdr;
+            acc.fy += dy[u] * fdr;
+            acc.fz += dz[u] * fdr;
+            acc.pe += eh;
+            if (VIRIAL)
+            {
+                const float f2 = 0.5f * fdr;
+                acc.vxx += f2 * dx[u] * dx[u];
+                acc.vxy += f2 * dx[u] * dy[u];
+                acc.vxz += f2 * dx[u] * dz[u];
+                acc.vyy += f2 * dy[u] * dy[u];
+                acc.vyz += f2 * dy[u] * dz[u];
+                acc.vzz += f2 * dz[u] * dz[u];
+            }
+        }
+    }
+}
+
+// One home particle (TPP lanes): chunk 0 of its row has been loaded into c0 already.
+template<bool ALL, int CH, class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__device__ __forceinline__ void tile_particle(const PairTileArgs& a, const RelTable<Eval>& tab,
+                                              const TileShared& S, bool active, unsigned int hs,
+                                              unsigned int i, unsigned int n,
+                                              const RowChunk<CH>& c0, unsigned int total)
+{
+    const unsigned int sub = threadIdx.x % TPP;
+    const unsigned int jmax = a.N - 1;
+    const unsigned int lmax = total - 1;
+    float4 ri;
+    unsigned int idum;
+    tile_cand<ALL>(a, S, S.aoff[S.qhome] + (active ? hs : 0u), ri, idum);
+    const unsigned int ti = MT ? (__float_as_uint(ri.w) >> 24) : 0u;
+    PairAcc acc;
+    tile_entries<ALL, CH, Eval, VIRIAL, MT, SHIFT>(a, tab, S, ri, i, ti, n, sub * CH, c0, lmax,
+                                                   jmax, acc);
+    const uint16_t* row = a.lrow + (size_t)i * a.nmax + sub * CH;
+    for (unsigned int k0 = TPP * CH; k0 < n; k0 += TPP * CH)
+    {
+        RowChunk<CH> c;
+        load_chunk<CH>(row + k0, c);
+        tile_entries<ALL, CH, Eval, VIRIAL, MT, SHIFT>(a, tab, S, ri, i, ti, n, k0 + sub * CH, c,
+                                                       lmax, jmax, acc);
+    }
+#pragma unroll
+    for (int off = TPP / 2; off > 0; off >>= 1)
+    {
+        acc.fx += __shfl_xor(acc.fx, off, TPP);
+        acc.fy += __shfl_xor(acc.fy, off, TPP);
+        acc.fz += __shfl_xor(acc.fz, off, TPP);
+        acc.pe += __shfl_xor(acc.pe, off, TPP);
+        if (VIRIAL)
+        {
+            acc.vxx += __shfl_xor(acc.vxx, off, TPP);
+            acc.vxy += __shfl_xor(acc.vxy, off, TPP);
+            acc.vxz += __shfl_xor(acc.vxz, off, TPP);
+            acc.vyy += __shfl_xor(acc.vyy, off, TPP);
+            acc.vyz += __shfl_xor(acc.vyz, off, TPP);
+            acc.vzz += __shfl_xor(acc.vzz, off, TPP);
+        }
+    }
+    if (active && sub == 0)
+    {
+        a.force[i] = make_float4(acc.fx, acc.fy, acc.fz, acc.pe);
+        if (VIRIAL)
+        {
+            const size_t P = a.N;
+            a.virial[0 * P + i] = acc.vxx;
+            a.virial[1 * P + i] = acc.vxy;
+            a.virial[2 * P + i] = acc.vxz;
+            a.virial[3 * P + i] = acc.vyy;
+            a.virial[4 * P + i] = acc.vyz;
+            a.virial[5 * P + i] = acc.vzz;
+        }
+    }
+}
+
+template<int B, class Eval, int TPP, bool VIRIAL, bool MT, int SHIFT>
+__global__ __launch_bounds__(TBLOCK) void pair_tile_kernel(PairTileArgs a, RelTable<Eval> tab)
+{
+    constexpr int CH = 2 * B;  // row entries per lane per chunk
+    __shared__ TileShared S;
+    const unsigned int tid = threadIdx.x;
+    const unsigned int sub = tid % TPP;
+    const XcdRange r = xcd_range(a.ncell);
+    for (unsigned int cell = r.begin; cell < r.end; cell += r.stride)
+    {
+        const unsigned int nhome = min(a.cell_size[cell], a.cmax);  // the same for every lane
+        if (nhome == 0)
+            continue;
+        // the first pass's particles and row chunks are requested before the staging, so
+        // their latency overlaps it
+        const unsigned int hs0 = tid / TPP;
+        const bool active0 = hs0 < nhome;
+        const unsigned int i0 = active0 ? min(a.cell_idx[(size_t)cell * a.cmax + hs0], a.N - 1)
+                                        : 0u;
+        const unsigned int n0 = active0 ? min(a.n_neigh[i0], a.nmax) : 0u;
+        RowChunk<CH> c0;
+        load_chunk<CH>(a.lrow + (size_t)i0 * a.nmax + sub * CH, c0);
+        if (tid < 64)
+        {
+            unsigned int size = 0;
+            if (tid < a.n_adj)
+            {
+                const unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
+                size = min(a.cell_size[c], a.cmax);
+                S.acell[tid] = c;
+                // -(home - neighbour) * w, the home-cell frame shift of this neighbour cell
+                const int hx = (int)(cell % a.dim[0]), hy = (int)((cell / a.dim[0]) % a.dim[1]),
+                          hz = (int)(cell / (a.dim[0] * a.dim[1]));
+                const int nx = (int)(c % a.dim[0]), ny = (int)((c / a.dim[0]) % a.dim[1]),
+                          nz = (int)(c / (a.dim[0] * a.dim[1]));
+                const int ox = wrap_offset(hx - nx, (int)a.dim[0]);
+                const int oy = wrap_offset(hy - ny, (int)a.dim[1]);
+                const int oz = wrap_offset(hz - nz, (int)a.dim[2]);
+                S.ash[tid] = make_float4(-(float)ox * a.wf[0], -(float)oy * a.wf[1],
+                                         -(float)oz * a.wf[2], 0.f);
+                if (c == cell)
+                    S.qhome = tid;
+            }
+            // inclusive prefix sum of the cell sizes over the wave
+            unsigned int incl = size;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1)
+            {
+                const unsigned int v = __shfl_up(incl, off, 64);
+                if (tid >= (unsigned int)off)
+                    incl += v;
+            }
+            if (tid < a.n_adj)
+                S.aoff[tid + 1] = incl;
+            if (tid == 0)
+                S.aoff[0] = 0;
+        }
+        __syncthreads();
+        const unsigned int total = S.aoff[a.n_adj];
+        const unsigned int nst = min(total, TCAP);
+        for (unsigned int t = tid; t < nst; t += TBLOCK)
+        {
+            const unsigned int q = find_segment(S.aoff, 0, a.n_adj, t);
+            const unsigned int slot = S.acell[q] * a.cmax + (t - S.aoff[q]);
+            const float4 rr = a.rel[slot];
+            const float4 sh = S.ash[q];
+            S.sp[t] = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z, rr.w);
+            S.sidx[t] = a.cell_idx[slot];
+        }
+        __syncthreads();
+        const bool all = total <= TCAP;
+        if (all)
+            tile_particle<true, CH, Eval, TPP, VIRIAL, MT, SHIFT>(a, tab, S, active0, hs0, i0, n0,
+                                                                  c0, total);
+        else
+            tile_particle<false, CH, Eval, TPP, VIRIAL, MT, SHIFT>(a, tab, S, active0, hs0, i0,
+                                                                   n0, c0, total);
+        // cells holding more than TBLOCK / TPP particles: further passes, rows loaded inline
+        for (unsigned int s0 = TBLOCK / TPP; s0 < nhome; s0 += TBLOCK / TPP)
+        {
+            const unsigned int hs = s0 + tid / TPP;
+            const bool active = hs < nhome;
+            const unsigned int i = active ? min(a.cell_idx[(size_t)cell * a.cmax + hs], a.N - 1)
+                                          : 0u;
+            const unsigned int n = active ? min(a.n_neigh[i], a.nmax) : 0u;
+            RowChunk<CH> c;
+            load_chunk<CH>(a.lrow + (size_t)i * a.nmax + sub * CH, c);
+            if (all)
+                tile_particle<true, CH, Eval, TPP, VIRIAL, MT, SHIFT>(a, tab, S, active, hs, i, n,
+                                                                      c, total);
+            else
+                tile_particle<false, CH, Eval, TPP, VIRIAL, MT, SHIFT>(a, tab, S, active, hs, i,
+                                                                       n, c, total);
+        }
+        __syncthreads();
+    }
+}
+
+template<int B, class Eval, int TPP, bool VIRIAL, bool MT>
+static void pair_tile_dispatch_shift(const PairTileArgs& a, const RelTable<Eval>& tab,
+                                     unsigned int grid, hipStream_t s)
+{
+    switch (a.shift_mode)
+    {
+    case 0:
+        hipLaunchKernelGGL((pair_tile_kernel<B, Eval, TPP, VIRIAL, MT, 0>), dim3(grid),
+                           dim3(TBLOCK), 0, s, a, tab);
+        break;
+    case 1:
+        hipLaunchKernelGGL((pair_tile_kernel<B, Eval, TPP, VIRIAL, MT, 1>), dim3(grid),
+                           dim3(TBLOCK), 0, s, a, tab);
+        break;
+    default:
+        hipLaunchKernelGGL((pair_tile_kernel<B, Eval, TPP, VIRIAL, MT, 2>), dim3(grid),
+                           dim3(TBLOCK), 0, s, a, tab);
+        break;
+    }
+}
+
+template<int B, class Eval, int TPP>
+static void pair_tile_dispatch_batch(const PairTileArgs& a, const RelTable<Eval>& tab,
+                                     unsigned int grid, hipStream_t s)
+{
+    const bool mt = a.ntypes > 1;
+    if (a.compute_virial)
+        mt ? pair_tile_dispatch_shift<B, Eval, TPP, true, true>(a, tab, grid, s)
+           : pair_tile_dispatch_shift<B, Eval, TPP, true, false>(a, tab, grid, s);
+    else
+        mt ? pair_tile_dispatch_shift<B, Eval, TPP, false, true>(a, tab, grid, s)
+           : pair_tile_dispatch_shift<B, Eval, TPP, false, false>(a, tab, grid, s);
+}
+
+template<class Eval, int TPP>
+static void pair_tile_dispatch_tpp(const PairTileArgs& a, const RelTable<Eval>& tab,
+                                   unsigned int batch, unsigned int grid, hipStream_t s)
+{
+    if (batch == 4)
+        pair_tile_dispatch_batch<4, Eval, TPP>(a, tab, grid, s);
+    else
+        pair_tile_dispatch_batch<8, Eval, TPP>(a, tab, grid, s);
+}
+
+hipError_t launch_pair_tile_lj(const PairTileArgs& a_in, const hm_lj_params* params,
+                               PairLaunch cfg, hipStream_t s)
+{
+    if (a_in.N == 0 || a_in.ncell == 0)
+        return hipSuccess;
+    using E = EvaluatorPairLJ<float>;
+    RelTable<E> tab;
+    PairTileArgs a = a_in;
+    const unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+        tab.f[q] = FastPair<E>::make(tab.p[q], a.rcutsq[q], a.shift_mode == 1);
+        if ((double)tab.p[q].epsilon_x_4 * (double)tab.p[q].sigma_6 == 0.0)
+        {
+            a.rcutsq[q] = 0.0;
+            a.rcutsq_f[q] = 0.f;
+        }
+    }
+    // worst-case FP32 error of r^2 near r_cut^2 (rel rounding, the home-frame shift and the
+    // difference, |rel| <= 1.5 w): below that of the gather form, whose bound is used
+    const double u = 1.0 / 16777216.0;
+    double wmax = 0.0;
+    for (int d = 0; d < 3; d++)
+        wmax = fmax(wmax, (double)a.box.L[d] / (double)a.dim[d]);
+    const double R = 1.5 * wmax;
+    const double e_dx = u * (2.0 * R + 2.0 * R + wmax + 2.0 * R + wmax);
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        const double rc2 = a.rcutsq[q];
+        const double rc = sqrt(rc2);
+        a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rc * e_dx + 6.0 * u * rc2) + 1e-12);
+    }
+    const unsigned int grid = round_up(min(cfg.block ? cfg.block : 2048u, a.ncell), 8);
+    const unsigned int batch = cfg.batch == 4 ? 4u : 8u;
+    switch (cfg.tpp ? cfg.tpp : 8)
+    {
+    case 4:
+        pair_tile_dispatch_tpp<E, 4>(a, tab, batch, grid, s);
+        break;
+    case 16:
+        pair_tile_dispatch_tpp<E, 16>(a, tab, batch, grid, s);
+        break;
+    default:
+        pair_tile_dispatch_tpp<E, 8>(a, tab, batch, grid, s);
         break;
     }
     return hipGetLastError();

@@ -80,9 +80,15 @@ def test_first_forces_match_fp64(hoomd):
     assert abs(lj.energy - rf[:, 3].sum()) / abs(rf[:, 3].sum()) < 1e-6
 
 
-def test_pair_list_matches_oracle(hoomd):
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_pair_list_matches_oracle(hoomd, kind):
+    """The pair list decoded from each row format (particle indices, cell slots with offset
+    codes, 16-bit local rows) is the oracle's half list, bit for bit."""
+    from hoomd_amd import _lib
     pos, vel, L = lj_state(8)
     sim, lj = make_sim(hoomd, pos, vel, L)
+    sim.run(0)
+    _lib.check(_lib.lib().hm_sys_set_pair_kernel(sim._system, kind, 0, 0))
     sim.run(0)
     got = lj.nlist.pair_list.astype(np.int64)
     got = got[np.lexsort((got[:, 1], got[:, 0]))]
@@ -261,12 +267,13 @@ def _lj_types_fp64(pos, L, types, rc, params, r_buff=0.4):
     return ref
 
 
-@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
 @pytest.mark.parametrize("n", [6, 8])
 def test_pair_kernel_kinds_after_rebuilds(hoomd, kind, n):
-    """Every engine pair kernel (cell-tiled, FP64 gather, FP32 cell-relative gather) gives the
-    FP64 forces of the state reached after many rebuilds and Hilbert re-sorts; n = 6 gives 3
-    cells per axis (the smallest grid of the relative kernel), n = 8 gives 4."""
+    """Every engine pair kernel (FP64 cell tiles, FP64 gather, FP32 cell-relative gather, FP32
+    cell tiles) gives the FP64 forces of the state reached after many rebuilds and Hilbert
+    re-sorts; n = 6 gives 3 cells per axis (the smallest grid of the relative kernels), n = 8
+    gives 4."""
     from hoomd_amd import _lib
     pos, vel, L = lj_state(n, kT=1.5)
     N = len(pos)
@@ -299,6 +306,9 @@ def test_pair_kernel_kinds_after_rebuilds(hoomd, kind, n):
     _lib.check(_lib.lib().hm_sys_set_pair_kernel(sim._system, kind, 0, 0))
     sim.run(150)
     assert lj.nlist.num_builds >= 3
+    cfg = np.zeros(5, np.uint32)
+    _lib.check(_lib.lib().hm_sys_pair_config(sim._system, cfg.ctypes.data))
+    assert cfg[0] == kind
     s = sim.state.get_snapshot()
     ref = _lj_types_fp64(s.particles.position, L, s.particles.typeid, rc, params)
     assert rel_rms(lj.forces, ref[:, :3]) < 1e-5
@@ -323,7 +333,8 @@ def test_autotuner_sweeps_and_keeps_results(hoomd):
             assert sim.operations.is_tuning_complete
             log = np.zeros((64, 4), np.float32)
             rows = _lib.lib().hm_sys_tune_log(sim._system, log.ctypes.data, 64)
-            assert rows == 50  # tpp x block size x batch (4, 8)
+            # tpp x workgroups x batch (cell-tiled default) or tpp x block size x batch (gather)
+            assert rows == {3: 18, 2: 50}[int(cfg[0])]
             best = log[np.argmin(log[:rows, 3])]
             assert (cfg[1], cfg[2], cfg[4]) == (int(best[0]), int(best[1]), int(best[2]))
             assert np.all(log[:rows, 3] > 0)
