@@ -113,48 +113,74 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
         double4 p = pos[n];
         if (last)
             last[n] = d3{p.x, p.y, p.z};
+        bool ok = true;
+        unsigned int bin = 0;
         if (isnan(p.x) || isnan(p.y) || isnan(p.z))
         {
             atomicMax(&cond[1], n + 1);
-            continue;
-        }
-        double fx = (p.x - box.lo[0]) * box.Linv[0];
-        double fy = (p.y - box.lo[1]) * box.Linv[1];
-        double fz = (p.z - box.lo[2]) * box.Linv[2];
-        int ib = (int)(fx * (double)dx);
-        int jb = (int)(fy * (double)dy);
-        int kb = (int)(fz * (double)dz);
-        if (fx < -0.00001 || fx >= 1.00001 || fy < -0.00001 || fy >= 1.00001 || fz < -0.00001
-            || fz >= 1.00001)
-        {
-            atomicMax(&cond[2], n + 1);
-            continue;
-        }
-        if (ib == (int)dx)
-            ib = 0;
-        if (jb == (int)dy)
-            jb = 0;
-        if (kb == (int)dz)
-            kb = 0;
-        if (ib < 0 || ib >= (int)dx || jb < 0 || jb >= (int)dy || kb < 0 || kb >= (int)dz)
-        {
-            atomicMax(&cond[2], n + 1);
-            continue;
-        }
-        unsigned int bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
-        unsigned int slot = atomicAdd(&cell_size[bin], 1u);
-        if (slot < cmax)
-        {
-            hm_cell_entry e;
-            e.x = p.x;
-            e.y = p.y;
-            e.z = p.z;
-            e.idx = n;
-            e.type = (uint32_t)p.w;
-            entries[(size_t)bin * cmax + slot] = e;
+            ok = false;
         }
         else
-            atomicMax(&cond[0], slot + 1);
+        {
+            double fx = (p.x - box.lo[0]) * box.Linv[0];
+            double fy = (p.y - box.lo[1]) * box.Linv[1];
+            double fz = (p.z - box.lo[2]) * box.Linv[2];
+            int ib = (int)(fx * (double)dx);
+            int jb = (int)(fy * (double)dy);
+            int kb = (int)(fz * (double)dz);
+            if (ib == (int)dx)
+                ib = 0;
+            if (jb == (int)dy)
+                jb = 0;
+            if (kb == (int)dz)
+                kb = 0;
+            if (fx < -0.00001 || fx >= 1.00001 || fy < -0.00001 || fy >= 1.00001 || fz < -0.00001
+                || fz >= 1.00001 || ib < 0 || ib >= (int)dx || jb < 0 || jb >= (int)dy || kb < 0
+                || kb >= (int)dz)
+            {
+                atomicMax(&cond[2], n + 1);
+                ok = false;
+            }
+            else
+                bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
+        }
+        // slot allocation aggregated over the wave: one atomic per distinct cell among the
+        // active lanes (Hilbert-ordered particles put a wave into a few cells)
+        unsigned int slot = 0;
+        bool todo = ok;
+        uint64_t pend = __ballot(todo);
+        while (pend)
+        {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const unsigned int lb = __shfl(bin, leader);
+            const bool mine = todo && bin == lb;
+            const uint64_t m = __ballot(mine);
+            unsigned int base = 0;
+            if ((int)lane_id() == leader)
+                base = atomicAdd(&cell_size[lb], (unsigned int)__popcll(m));
+            base = __shfl(base, leader);
+            if (mine)
+            {
+                slot = base + lanes_below(m);
+                todo = false;
+            }
+            pend = __ballot(todo);
+        }
+        if (ok)
+        {
+            if (slot < cmax)
+            {
+                hm_cell_entry e;
+                e.x = p.x;
+                e.y = p.y;
+                e.z = p.z;
+                e.idx = n;
+                e.type = (uint32_t)p.w;
+                entries[(size_t)bin * cmax + slot] = e;
+            }
+            else
+                atomicMax(&cond[0], slot + 1);
+        }
     }
 }
 
@@ -207,7 +233,8 @@ hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const 
 // rows entry for entry. Grids with fewer than 4 cells along an axis use the FP64 kernel.
 constexpr int NLB = 256;
 constexpr int NLU = 4;  // candidate chunks of 64 per sweep iteration (FP32 path)
-constexpr int NLCAP = 1024;
+constexpr int NLCAP = 768;
+constexpr int RBUF = 160;  // row entries of one home particle buffered per wave in LDS
 constexpr int NLHOME = 256;
 
 // Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
@@ -375,6 +402,8 @@ template<bool FAST, bool MT, bool EXCL> struct NlistShared
     float4 sh[NLHOME];
     unsigned char shtp[MT ? NLHOME : 1];
     unsigned int sslot[NLCAP];  // slot mode: candidate slot | offset code
+    unsigned int rb[NLB / 64][RBUF];     // hits of the wave's home particle in this pass
+    unsigned short rbl[NLB / 64][RBUF];
     unsigned int acell[27], asize[27], aoff[27], acode[27];
     unsigned int hcnt[NLHOME];
     unsigned int pass_last, pass_total;
@@ -479,6 +508,8 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                 // NLU chunks of 64 candidates per iteration: the LDS reads and distance tests of
                 // the chunks are independent, the rare guard-band re-decisions are taken out of
                 // the main path, then the hits are compacted chunk by chunk in candidate order
+                // into an LDS row buffer that is written out coalesced after the sweep
+                const unsigned int cnt0 = cnt;
                 for (unsigned int b = 0; b < total; b += 64 * NLU)
                 {
                     float4 c4[NLU];
@@ -525,10 +556,16 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         {
                             const unsigned int k = b + 64u * u + lane;
                             const unsigned int p = cnt + lanes_below(m);
-                            if (p < a.nmax)
+                            const unsigned int v = a.offset_codes ? S.sslot[k] : j;
+                            if (p - cnt0 < (unsigned int)RBUF)
+                            {
+                                S.rb[wave][p - cnt0] = v;
+                                S.rbl[wave][p - cnt0] = (unsigned short)(base_k + k);
+                            }
+                            else if (p < a.nmax)
                             {
                                 if (!a.skip_index_rows)
-                                    row[p] = a.offset_codes ? S.sslot[k] : j;
+                                    row[p] = v;
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
                             }
@@ -536,6 +573,17 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         cnt += (unsigned int)__popcll(m);
                     }
                 }
+                // write the buffered part of this pass's hits, [cnt0, min(cnt, nmax))
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const unsigned int cend = min(min(cnt, a.nmax), cnt0 + (unsigned int)RBUF);
+                for (unsigned int t = cnt0 + lane; t < cend; t += 64)
+                {
+                    if (!a.skip_index_rows)
+                        row[t] = S.rb[wave][t - cnt0];
+                    if (lrow)
+                        lrow[t] = S.rbl[wave][t - cnt0];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             else
                 for (unsigned int b = 0; b < total; b += 64)
