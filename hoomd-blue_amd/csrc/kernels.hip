@@ -4,6 +4,8 @@
 // cutoff test are exact FP64 comparisons; pair and bond arithmetic is FP32 with FP32
 // per-particle sums. Wavefronts are 64 lanes; particle-parallel gathers use an XCD-aware block
 // remap so spatially close (Hilbert-sorted) particles share an XCD's L2.
+#include <type_traits>
+
 #include "evaluators.h"
 #include "kernels.h"
 
@@ -232,9 +234,14 @@ hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const 
 // threshold, the band being four times the worst-case FP32 error; rows therefore equal the CPU
 // rows entry for entry. Grids with fewer than 4 cells along an axis use the FP64 kernel.
 constexpr int NLB = 256;
-constexpr int NLU = 4;  // candidate chunks of 64 per sweep iteration (FP32 path)
-constexpr int NLCAP = 768;
-constexpr int RBUF = 160;  // row entries of one home particle buffered per wave in LDS
+#ifndef HM_NLU
+#define HM_NLU 4
+#endif
+constexpr int NLU = HM_NLU;  // candidate chunks of 64 per sweep iteration (FP32 path)
+#ifndef HM_NLCAP
+#define HM_NLCAP 768
+#endif
+constexpr int NLCAP = HM_NLCAP;
 constexpr int NLHOME = 256;
 
 // Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
@@ -402,8 +409,6 @@ template<bool FAST, bool MT, bool EXCL> struct NlistShared
     float4 sh[NLHOME];
     unsigned char shtp[MT ? NLHOME : 1];
     unsigned int sslot[NLCAP];  // slot mode: candidate slot | offset code
-    unsigned int rb[NLB / 64][RBUF];     // hits of the wave's home particle in this pass
-    unsigned short rbl[NLB / 64][RBUF];
     unsigned int acell[27], asize[27], aoff[27], acode[27];
     unsigned int hcnt[NLHOME];
     unsigned int pass_last, pass_total;
@@ -508,19 +513,17 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                 // NLU chunks of 64 candidates per iteration: the LDS reads and distance tests of
                 // the chunks are independent, the rare guard-band re-decisions are taken out of
                 // the main path, then the hits are compacted chunk by chunk in candidate order
-                // into an LDS row buffer that is written out coalesced after the sweep
-                const unsigned int cnt0 = cnt;
-                for (unsigned int b = 0; b < total; b += 64 * NLU)
-                {
-                    float4 c4[NLU];
-                    unsigned int tp[NLU];
-                    bool hit[NLU];
+                auto chunks = [&](auto uc, unsigned int b) {
+                    constexpr int U = decltype(uc)::value;
+                    float4 c4[U];
+                    unsigned int tp[U];
+                    bool hit[U];
                     unsigned int chk = 0;
 #pragma unroll
-                    for (int u = 0; u < NLU; u++)
+                    for (int u = 0; u < U; u++)
                         c4[u] = S.sc[min(b + 64u * u + lane, total - 1)];
 #pragma unroll
-                    for (int u = 0; u < NLU; u++)
+                    for (int u = 0; u < U; u++)
                     {
                         const bool valid = b + 64u * u + lane < total;
                         const float dx = h4.x - c4[u].x, dy = h4.y - c4[u].y, dz = h4.z - c4[u].z;
@@ -533,7 +536,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                     if (chk)
                     {
 #pragma unroll
-                        for (int u = 0; u < NLU; u++)
+                        for (int u = 0; u < U; u++)
                             if (chk & (1u << u))
                             {
                                 // exact FP64 decision as NeighborListBinned.cc:151-157 computes it
@@ -545,7 +548,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                             }
                     }
 #pragma unroll
-                    for (int u = 0; u < NLU; u++)
+                    for (int u = 0; u < U; u++)
                     {
                         const unsigned int j = __float_as_uint(c4[u].w);
                         bool h = hit[u] && (j != i) && a.enabled[tp[u]];
@@ -556,34 +559,23 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         {
                             const unsigned int k = b + 64u * u + lane;
                             const unsigned int p = cnt + lanes_below(m);
-                            const unsigned int v = a.offset_codes ? S.sslot[k] : j;
-                            if (p - cnt0 < (unsigned int)RBUF)
-                            {
-                                S.rb[wave][p - cnt0] = v;
-                                S.rbl[wave][p - cnt0] = (unsigned short)(base_k + k);
-                            }
-                            else if (p < a.nmax)
+                            if (p < a.nmax)
                             {
                                 if (!a.skip_index_rows)
-                                    row[p] = v;
+                                    row[p] = a.offset_codes ? S.sslot[k] : j;
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
                             }
                         }
                         cnt += (unsigned int)__popcll(m);
                     }
-                }
-                // write the buffered part of this pass's hits, [cnt0, min(cnt, nmax))
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                const unsigned int cend = min(min(cnt, a.nmax), cnt0 + (unsigned int)RBUF);
-                for (unsigned int t = cnt0 + lane; t < cend; t += 64)
-                {
-                    if (!a.skip_index_rows)
-                        row[t] = S.rb[wave][t - cnt0];
-                    if (lrow)
-                        lrow[t] = S.rbl[wave][t - cnt0];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                };
+                // whole groups of NLU chunks, then the remaining chunks one at a time
+                unsigned int b = 0;
+                for (; b + 64u * NLU <= total; b += 64u * NLU)
+                    chunks(std::integral_constant<int, NLU>(), b);
+                for (; b < total; b += 64u)
+                    chunks(std::integral_constant<int, 1>(), b);
             }
             else
                 for (unsigned int b = 0; b < total; b += 64)
