@@ -561,8 +561,10 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                             const unsigned int p = cnt + lanes_below(m);
                             if (p < a.nmax)
                             {
+                                // rows are streamed out: keep them from evicting the candidates
                                 if (!a.skip_index_rows)
-                                    row[p] = a.offset_codes ? S.sslot[k] : j;
+                                    __builtin_nontemporal_store(a.offset_codes ? S.sslot[k] : j,
+                                                                row + p);
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
                             }
