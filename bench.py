@@ -190,7 +190,7 @@ def main():
         for r in tl[:max(rows, 0)]:
             log(f"[bench] autotuner tpp={int(r[0])} block={int(r[1])} batch={int(r[2])} "
                 f"{r[3] * 1e3:.1f} us")
-    st0 = np.zeros(6, np.uint64)
+    st0 = np.zeros(9, np.uint64)
     _lib.check(_lib.lib().hm_sys_stats(sim._system, st0.ctypes.data))
     log(f"[bench] timing {a.steps} steps")
     _lib.check(_lib.lib().hm_synchronize())
@@ -198,7 +198,7 @@ def main():
     sim.run(a.steps)
     _lib.check(_lib.lib().hm_synchronize())
     elapsed = time.perf_counter() - t0
-    st1 = np.zeros(6, np.uint64)
+    st1 = np.zeros(9, np.uint64)
     _lib.check(_lib.lib().hm_sys_stats(sim._system, st1.ctypes.data))
     tps = a.steps / elapsed
     builds = int(st1[0] - st0[0])

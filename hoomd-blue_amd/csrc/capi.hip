@@ -349,7 +349,8 @@ int hm_thermo(unsigned int N, const double* vel, const float* const* forces,
         return fail(HM_ERR_ARGS, "hm_thermo: invalid arguments");
     hipStream_t s = (hipStream_t)stream;
     double* partial = nullptr;
-    if (hipMallocAsync((void**)&partial, sizeof(double) * 9 * THERMO_BLOCKS, s) != hipSuccess)
+    if (hipMallocAsync((void**)&partial, sizeof(double) * THERMO_SUMS * THERMO_BLOCKS, s)
+        != hipSuccess)
         return fail(HM_ERR_ALLOC, "hm_thermo: scratch allocation failed");
     int r = check(launch_thermo(N, (const double4*)vel, (const float4* const*)forces, virials,
                                 nforces, partial, out, s),

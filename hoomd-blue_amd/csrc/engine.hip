@@ -103,6 +103,9 @@ struct hm_system
     // neighbour list
     double r_buff = 0.4;
     int exclude_bonds = 1;
+    unsigned int check_delay = 1;  // NeighborList rebuild_check_delay
+    int dist_check = 1;            // NeighborList check_dist
+    unsigned long long run_builds_base = 0;
     unsigned int dim[3] = {0, 0, 0};
     unsigned int ncell = 0, n_adj = 0, cmax = 0, nmax = 0;
     unsigned int *cell_size = nullptr, *cell_adj = nullptr, *nlist = nullptr, *n_neigh = nullptr;
@@ -419,6 +422,8 @@ struct hm_system
         k.cond = &cond->nlist_need;
         k.gate = gated ? st : nullptr;
         k.counter = st;
+        k.check_delay = check_delay;
+        k.dist_check = dist_check;
     }
 
     bool rel_ok() const
@@ -675,6 +680,8 @@ struct hm_system
             a1.last = last;
             a1.maxsq = r_buff < 1e-6 ? -1.0 : distance_check_maxsq(rcut_max, r_buff);
             a1.flag_state = st;
+            a1.check_delay = check_delay;
+            a1.dist_check = dist_check;
             a1.rel = rel_frame();
         }
         hchk(launch_step_one(a1, stream), "step one");
@@ -851,6 +858,11 @@ struct hm_system
         s0.timestep = timestep;
         s0.rebuild_at = 0;
         s0.alpha = 1.0;
+        // the build below is the forced build of the first step (NeighborList.cc:1358-1368)
+        s0.last_update = timestep;
+        s0.forced_at = timestep;
+        s0.shortest = ~0ull;
+        s0.n_builds = rcut_max > 0.0 ? 1 : 0;
         hchk(hipMemcpyAsync(st, &s0, sizeof(StepState), hipMemcpyHostToDevice, stream), "state");
         update_exclusion_index();
         if (rcut_max > 0.0)
@@ -997,6 +1009,18 @@ struct hm_system
 
     int run(uint64_t nsteps)
     {
+        // build statistics count from the start of each run (System::run -> resetStats,
+        // System.cc:90; NeighborList::resetStats, NeighborList.cc:1420-1426)
+        run_builds_base = 0;
+        if (prepared && st)
+        {
+            StepState h;
+            hchk(hipStreamSynchronize(stream), "sync");
+            hchk(hipMemcpy(&h, st, sizeof(StepState), hipMemcpyDeviceToHost), "download");
+            run_builds_base = h.n_builds;
+            const unsigned long long reset[2] = {~0ull, 0ull};
+            hchk(hipMemcpy(&st->shortest, reset, sizeof(reset), hipMemcpyHostToDevice), "reset");
+        }
         if (!prepared)
             prep();
         uint64_t end = timestep + nsteps;
@@ -1122,9 +1146,9 @@ hm_system* hm_sys_create(unsigned int N, hm_box box, unsigned int ntypes, const 
         s->rtag = dalloc<unsigned int>(N);
         s->last = dalloc<d3>(N);
         s->ke_partial = dalloc<double>((N + 255) / 256);
-        s->thermo_partial = dalloc<double>(9 * THERMO_BLOCKS);
-        s->thermo_out = dalloc<double>(9);
-        s->h_thermo = halloc<double>(9);
+        s->thermo_partial = dalloc<double>(THERMO_SUMS * THERMO_BLOCKS);
+        s->thermo_out = dalloc<double>(THERMO_SUMS);
+        s->h_thermo = halloc<double>(THERMO_SUMS);
         s->d_gamma = dalloc<double>(HM_MAX_TYPES);
         hchk(hipMemcpy(s->pos, p.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
         hchk(hipMemcpy(s->vel, v.data(), sizeof(double4) * N, hipMemcpyHostToDevice), "upload");
@@ -1640,9 +1664,9 @@ long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pair
     return r == HM_OK ? out : r;
 }
 
-int hm_sys_thermo(hm_system* s, double* out9)
+int hm_sys_thermo(hm_system* s, double* out15)
 {
-    if (!s || !out9)
+    if (!s || !out15)
         return fail(HM_ERR_ARGS, "hm_sys_thermo: invalid arguments");
     return guarded([&]() {
         if (!s->prepared)
@@ -1656,37 +1680,50 @@ int hm_sys_thermo(hm_system* s, double* out9)
             vp[q] = s->forces[q].v;
         hchk(launch_thermo(s->N, s->vel, fp, vp, nf, s->thermo_partial, s->thermo_out, s->stream),
              "thermo");
-        hchk(hipMemcpyAsync(s->h_thermo, s->thermo_out, sizeof(double) * 9, hipMemcpyDeviceToHost,
+        hchk(hipMemcpyAsync(s->h_thermo, s->thermo_out, sizeof(double) * THERMO_SUMS, hipMemcpyDeviceToHost,
                             s->stream),
              "thermo");
         hchk(hipStreamSynchronize(s->stream), "sync");
-        for (int k = 0; k < 9; k++)
-            out9[k] = s->h_thermo[k];
+        for (unsigned int k = 0; k < THERMO_SUMS; k++)
+            out15[k] = s->h_thermo[k];
         return HM_OK;
     });
 }
 
-int hm_sys_stats(hm_system* s, uint64_t* out6)
+int hm_sys_stats(hm_system* s, uint64_t* out9)
 {
-    if (!s || !out6)
+    if (!s || !out9)
         return fail(HM_ERR_ARGS, "hm_sys_stats: invalid arguments");
     return guarded([&]() {
-        uint64_t builds = 0;
+        StepState h{};
+        h.shortest = ~0ull;
         if (s->st)
         {
-            StepState h;
             hchk(hipStreamSynchronize(s->stream), "sync");
             hchk(hipMemcpy(&h, s->st, sizeof(StepState), hipMemcpyDeviceToHost), "download");
-            builds = h.n_builds;
         }
-        out6[0] = builds;
-        out6[1] = s->nmax;
-        out6[2] = s->cmax;
-        out6[3] = s->n_sorts;
-        out6[4] = s->steps_run;
-        out6[5] = s->rollbacks;
+        out9[0] = h.n_builds;
+        out9[1] = s->nmax;
+        out9[2] = s->cmax;
+        out9[3] = s->n_sorts;
+        out9[4] = s->steps_run;
+        out9[5] = s->rollbacks;
+        out9[6] = h.n_builds - std::min<uint64_t>(h.n_builds, s->run_builds_base);
+        // NeighborList::getSmallestRebuild: periods are tracked below 100 (NeighborList.cc:259)
+        out9[7] = std::min<uint64_t>(h.shortest, 100);
+        out9[8] = h.n_dangerous;
         return HM_OK;
     });
+}
+
+int hm_sys_set_nlist_policy(hm_system* s, unsigned int rebuild_check_delay, int check_dist)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_set_nlist_policy: NULL system");
+    s->check_delay = rebuild_check_delay;
+    s->dist_check = check_dist != 0;
+    s->clear_graphs();
+    return HM_OK;
 }
 
 int hm_sys_thermalize(hm_system* s, double kT)

@@ -26,6 +26,11 @@ struct StepState
     double alpha;                   // Bussi translational factor of the current step
     unsigned int error;             // sticky error bits (see ERR_*)
     unsigned int pad;
+    // rebuild policy and statistics (NeighborList::needsUpdating, NeighborList.cc:1325-1418)
+    unsigned long long last_update; // step of the last build
+    unsigned long long forced_at;   // step of a forced build (sort, overflow recovery)
+    unsigned long long shortest;    // shortest period between distance-check builds this run
+    unsigned long long n_dangerous; // builds on the first check after a build with delay > 1
 };
 
 enum : unsigned int
@@ -156,7 +161,9 @@ struct NlistKernelArgs
     int skip_index_rows;      // write only the local rows (nlist untouched)
     unsigned int* cond;       // needed nmax
     const StepState* gate;
-    StepState* counter;       // n_builds incremented by block 0 when non-NULL
+    StepState* counter;       // n_builds and the rebuild statistics updated by block 0
+    unsigned int check_delay;
+    int dist_check;
 };
 
 // Cell-tiled pair kernel: neighbours read from the home cell's LDS-staged candidates through
@@ -323,7 +330,9 @@ struct StepOneArgs
     // fused distance check (NeighborList::distanceCheck for step t + 1)
     const d3* last;
     double maxsq;
-    StepState* flag_state;     // rebuild_at = t + 1 when a particle moved too far
+    StepState* flag_state;     // rebuild_at = t + 1 when the rebuild policy asks for a build
+    unsigned int check_delay;  // rebuild_check_delay (>= 0)
+    int dist_check;            // check_dist
     // fused refresh of the reference-cell relative positions (rel.rel may be NULL)
     RelFrame rel;
     // fused clearing of the cell occupancy counters
@@ -365,8 +374,9 @@ hipError_t launch_accel_init(unsigned int N, const double4* vel, d3* accel, cons
                              unsigned int nforces, hipStream_t s);
 
 // thermo sums: out[0] KE, out[1] PE, out[2] W, out[3..8] virial tensor; partial scratch of
-// THERMO_BLOCKS * 9 doubles
+// THERMO_BLOCKS * THERMO_SUMS doubles of partial sums; THERMO_SUMS doubles of output
 constexpr unsigned int THERMO_BLOCKS = 512;
+constexpr unsigned int THERMO_SUMS = 15;
 hipError_t launch_thermo(unsigned int N, const double4* vel, const float4* const* f,
                          const float* const* vir, unsigned int nforces, double* partial,
                          double* out, hipStream_t s);

@@ -638,7 +638,20 @@ __global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
     if (gate_closed(a.gate))
         return;
     if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
-        a.counter->n_builds++;
+    {
+        // build statistics of NeighborList::needsUpdating (NeighborList.cc:1349-1415)
+        StepState* st = a.counter;
+        const unsigned long long t1 = st->timestep + 1;
+        if (st->forced_at != t1)
+        {
+            if (t1 > st->last_update && t1 - st->last_update < st->shortest)
+                st->shortest = t1 - st->last_update;
+            if (a.dist_check && a.check_delay > 1 && t1 == st->last_update + a.check_delay)
+                st->n_dangerous++;
+        }
+        st->last_update = t1;
+        st->n_builds++;
+    }
     const XcdRange r = xcd_range(a.ncell);
     for (unsigned int cell = r.begin; cell < r.end; cell += r.stride)
         nlist_cell<FAST, MT, EXCL>(a, cell, S);
@@ -2057,8 +2070,23 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
             moved = dx * dx + dy * dy + dz * dz >= a.maxsq;
         }
     }
-    if (a.flag_state && __ballot(moved) != 0 && (threadIdx.x & 63u) == 0)
-        atomicMax(&a.flag_state->rebuild_at, a.flag_state->timestep + 1);
+    if (a.flag_state)
+    {
+        // NeighborList::needsUpdating (NeighborList.cc:1325-1418) for step t1: no check before
+        // rebuild_check_delay steps have passed since the last build; an unconditional build when
+        // the buffer is zero or distance checks are off and the period is hit; else the distance
+        // check. The conditions are the same on every lane, so the ballot stays uniform.
+        const StepState* st = a.flag_state;
+        const unsigned long long t1 = st->timestep + 1;
+        const bool should = !(t1 < st->last_update + a.check_delay);
+        const bool always = a.maxsq < 0.0
+                            || (!a.dist_check
+                                && (a.check_delay == 0
+                                    || (a.check_delay > 1
+                                        && t1 == st->last_update + a.check_delay)));
+        if (should && (always || __ballot(moved) != 0) && (threadIdx.x & 63u) == 0)
+            atomicMax(&a.flag_state->rebuild_at, t1);
+    }
 }
 
 hipError_t launch_step_one(const StepOneArgs& a, hipStream_t s)
@@ -2243,6 +2271,7 @@ hipError_t launch_step_begin(StepState* st, const double* ke_partial, unsigned i
 __global__ void set_rebuild_kernel(StepState* st)
 {
     st->rebuild_at = st->next + 1;
+    st->forced_at = st->next + 1;
 }
 
 hipError_t launch_set_rebuild(StepState* st, hipStream_t s)
@@ -2301,12 +2330,19 @@ __global__ __launch_bounds__(256) void thermo_partial_kernel(unsigned int N, con
 {
 #pragma clang fp contract(off)
     __shared__ double sh[4];
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[THERMO_SUMS] = {};
     for (unsigned int n = blockIdx.x * blockDim.x + threadIdx.x; n < N;
          n += gridDim.x * blockDim.x)
     {
         double4 v = vel[n];
         acc[0] += v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+        // kinetic part of the pressure tensor, m v_a v_b (ComputeThermo.cc:236-262)
+        acc[9] += v.w * v.x * v.x;
+        acc[10] += v.w * v.x * v.y;
+        acc[11] += v.w * v.x * v.z;
+        acc[12] += v.w * v.y * v.y;
+        acc[13] += v.w * v.y * v.z;
+        acc[14] += v.w * v.z * v.z;
         for (unsigned int q = 0; q < fp.n; q++)
         {
             acc[1] += (double)fp.f[q][n].w;
@@ -2315,11 +2351,11 @@ __global__ __launch_bounds__(256) void thermo_partial_kernel(unsigned int N, con
                     acc[3 + m] += (double)fp.v[q][(size_t)m * N + n];
         }
     }
-    for (int k = 0; k < 9; k++)
+    for (unsigned int k = 0; k < THERMO_SUMS; k++)
     {
         double t = block_sum_256(acc[k], sh);
         if (threadIdx.x == 0)
-            partial[blockIdx.x * 9 + k] = t;
+            partial[blockIdx.x * THERMO_SUMS + k] = t;
     }
 }
 
@@ -2328,11 +2364,11 @@ __global__ __launch_bounds__(256) void thermo_final_kernel(const double* partial
 {
 #pragma clang fp contract(off)
     __shared__ double sh[4];
-    for (int k = 0; k < 9; k++)
+    for (unsigned int k = 0; k < THERMO_SUMS; k++)
     {
         double acc = 0.0;
         for (unsigned int b = threadIdx.x; b < nblocks; b += 256)
-            acc += partial[b * 9 + k];
+            acc += partial[b * THERMO_SUMS + k];
         double t = block_sum_256(acc, sh);
         if (threadIdx.x == 0)
         {

@@ -104,6 +104,7 @@ SIGNATURES = {
     "hm_sys_create": (_p, [_u, HmBox, _u, _p, _p, _p, _p, _p, _u16]),
     "hm_sys_destroy": (None, [_p]),
     "hm_sys_set_nlist": (_i, [_p, _d, _i]),
+    "hm_sys_set_nlist_policy": (_i, [_p, _u, _i]),
     "hm_sys_add_pair_lj": (_i, [_p, _p, _p, _p, _i]),
     "hm_sys_add_bond_fene": (_i, [_p, _u, _p, _p, _u, _p]),
     "hm_sys_set_method": (_i, [_p, _i, _d, _d, _p]),

@@ -17,7 +17,7 @@ class ThermodynamicQuantities:
         sim = self._simulation
         if sim is None or sim._system is None:
             raise RuntimeError("ThermodynamicQuantities needs an attached simulation")
-        out = np.zeros(9)
+        out = np.zeros(15)
         _lib.check(_lib.lib().hm_sys_thermo(sim._system, out.ctypes.data), "thermo")
         return out
 
@@ -62,10 +62,7 @@ class ThermodynamicQuantities:
 
     @property
     def pressure_tensor(self):
+        """(xx, xy, xz, yy, yz, zz) of (sum m v_a v_b + virial_ab) / V, summed on the device
+        (ComputeThermo.cc:236-262)."""
         o = self._thermo()
-        snap = self._simulation.state.get_snapshot()
-        v = snap.particles.velocity
-        m = snap.particles.mass
-        kin = [np.sum(m * v[:, a] * v[:, b]) for a, b in
-               ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
-        return tuple(float((k + w) / self.volume) for k, w in zip(kin, o[3:9]))
+        return tuple(float((k + w) / self.volume) for k, w in zip(o[9:15], o[3:9]))

@@ -30,6 +30,8 @@ class Integrator:
         if nlists:
             nl = next(iter(nlists.values()))
             _lib.check(L.hm_sys_set_nlist(h, nl.buffer, int("bond" in nl.exclusions)), "nlist")
+            _lib.check(L.hm_sys_set_nlist_policy(h, int(nl.rebuild_check_delay),
+                                                 int(bool(nl.check_dist))), "nlist policy")
         for f in self.forces:
             f._attach(sim, h)
         m = self.methods[0]

@@ -9,18 +9,17 @@ class NeighborList:
     def __init__(self, buffer, exclusions=("bond",), rebuild_check_delay=1, check_dist=True):
         self.buffer = float(buffer)
         self.exclusions = tuple(exclusions)
-        if rebuild_check_delay != 1 or not check_dist:
-            raise NotImplementedError(
-                "hoomd_amd checks distances every step (rebuild_check_delay=1, check_dist=True)")
+        if int(rebuild_check_delay) < 0:
+            raise ValueError("rebuild_check_delay must be non-negative")
         for e in self.exclusions:
             if e != "bond":
                 raise NotImplementedError(f"exclusion '{e}' is not supported (only 'bond')")
-        self.rebuild_check_delay = rebuild_check_delay
-        self.check_dist = check_dist
+        self.rebuild_check_delay = int(rebuild_check_delay)
+        self.check_dist = bool(check_dist)
         self._simulation = None
 
     def _signature(self):
-        return (self.buffer, self.exclusions)
+        return (self.buffer, self.exclusions, self.rebuild_check_delay, self.check_dist)
 
     @property
     def pair_list(self):
@@ -35,14 +34,33 @@ class NeighborList:
         _lib.check(_lib.lib().hm_sys_get_pairs(sim._system, out.ctypes.data, n), "pair_list")
         return out[:n]
 
-    @property
-    def num_builds(self):
+    def _stats(self):
         sim = self._simulation
         if sim is None or sim._system is None:
-            return 0
-        st = np.zeros(6, np.uint64)
+            return None
+        st = np.zeros(9, np.uint64)
         _lib.check(_lib.lib().hm_sys_stats(sim._system, st.ctypes.data), "stats")
-        return int(st[0])
+        return st
+
+    @property
+    def num_builds(self):
+        """Neighbour-list builds in the last Simulation.run (nlist.py:317-324)."""
+        st = self._stats()
+        return 0 if st is None else int(st[6])
+
+    @property
+    def shortest_rebuild(self):
+        """Shortest number of steps between distance-check rebuilds in the last run
+        (nlist.py:308-315, NeighborList::getSmallestRebuild)."""
+        st = self._stats()
+        return 100 if st is None else int(st[7])
+
+    @property
+    def num_dangerous_builds(self):
+        """Builds taken on the first check after a build with rebuild_check_delay > 1
+        (the reference's \"Dangerous neighborlist build\" notice, NeighborList.cc:1405-1414)."""
+        st = self._stats()
+        return 0 if st is None else int(st[8])
 
 
 class Cell(NeighborList):

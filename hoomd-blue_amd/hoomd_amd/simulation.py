@@ -6,6 +6,7 @@ only describes the system, exactly as hoomd.Simulation hands it to System::run
 """
 
 import time
+import warnings
 
 import numpy as np
 
@@ -206,6 +207,19 @@ class Simulation:
         dt = time.perf_counter() - t0
         self._walltime += dt
         self._tps = steps / dt if dt > 0 and steps else self._tps
+        self._warn_dangerous_builds()
+
+    def _warn_dangerous_builds(self):
+        """The reference's notice for builds taken on the first distance check after a build
+        (NeighborList.cc:1405-1414)."""
+        integ = self.operations.integrator
+        for f in (integ.forces if integ is not None else []):
+            nl = getattr(f, "nlist", None)
+            if nl is not None and nl.num_dangerous_builds:
+                warnings.warn("nlist: Dangerous neighborlist build occurred. Continuing this "
+                              "simulation may produce incorrect results and/or program crashes. "
+                              "Decrease the neighborlist check_period and rerun.", RuntimeWarning)
+                return
 
     def __del__(self):
         h = getattr(self, "_system", None)

@@ -263,9 +263,11 @@ int hm_langevin_step_two(unsigned int N, double* vel, double* accel, const doubl
                          unsigned int nforces, const double* gamma_by_type, unsigned int ntypes,
                          double kT, double dt, uint64_t timestep, uint16_t seed,
                          hm_stream_t stream);
-/* Thermodynamic sums in FP64 (deterministic order): out[0] = KE, out[1] = PE, out[2] = trace of
- * the virial / 3 (W), out[3..8] = virial tensor sums. Replaces gpu_compute_thermo_partial_sums +
- * final_sums (hoomd/md/ComputeThermoGPU.cu:46,141). forces may hold 0..4 arrays. */
+/* Thermodynamic sums in FP64 (deterministic order), out[15]: out[0] = KE, out[1] = PE,
+ * out[2] = trace of the virial / 3 (W), out[3..8] = virial tensor sums (xx, xy, xz, yy, yz, zz),
+ * out[9..14] = kinetic tensor sums m v_a v_b in the same order. Replaces
+ * gpu_compute_thermo_partial_sums + final_sums (hoomd/md/ComputeThermoGPU.cu:46,141). forces may
+ * hold 0..4 arrays. Allocates its partial-sum scratch on the stream (hipMallocAsync). */
 int hm_thermo(unsigned int N, const double* vel, const float* const* forces,
               const float* const* virials, unsigned int nforces, double* out, hm_stream_t stream);
 /* Bussi rescale factor for the translational DOF from the kinetic energy K, drawn on the host
@@ -300,6 +302,10 @@ void hm_sys_destroy(hm_system* s);
 
 /* Neighbour list (md.nlist.Cell): buffer, bond exclusions on/off. */
 int hm_sys_set_nlist(hm_system* s, double r_buff, int exclude_bonds);
+/* Rebuild policy of NeighborList::needsUpdating (hoomd/md/NeighborList.cc:1325-1418): no check
+ * until rebuild_check_delay steps after a build; with check_dist the list is rebuilt when a
+ * particle moved by r_buff / 2, without it every rebuild_check_delay steps. Defaults 1, 1. */
+int hm_sys_set_nlist_policy(hm_system* s, unsigned int rebuild_check_delay, int check_dist);
 /* LJ pair force (md.pair.LJ): r_cut/params per type pair (ntypes^2, row-major, symmetric);
  * r_cut <= 0 disables a pair. mode: 0 none, 1 shift, 2 xplor. Returns the force id (>= 0). */
 int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
@@ -354,11 +360,14 @@ int hm_sys_get_forces(hm_system* s, int force_id, double* force4, double* virial
 /* Neighbour list of the last build as (tag_i, tag_j) pairs with tag_i < tag_j; returns the
  * number of pairs (<= max_pairs) or a negative error. */
 long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pairs);
-/* Thermo sums of the current state: see hm_thermo. Evaluates forces first if stale. */
-int hm_sys_thermo(hm_system* s, double* out9);
-/* Statistics: out[0] = nlist builds, out[1] = nmax, out[2] = cmax, out[3] = sorts, out[4] = steps
- * run, out[5] = rollbacks after overflow. */
-int hm_sys_stats(hm_system* s, uint64_t* out6);
+/* Thermo sums of the current state into out[15]: see hm_thermo. Evaluates the virials first
+ * when they are not computed every step. */
+int hm_sys_thermo(hm_system* s, double* out15);
+/* Statistics, out[9]: out[0] = nlist builds since creation, out[1] = nmax, out[2] = cmax,
+ * out[3] = sorts, out[4] = steps run, out[5] = rollbacks after overflow, out[6] = builds in the
+ * last run (NeighborList num_builds), out[7] = shortest period between distance-check builds in
+ * the last run, 100 when none (getSmallestRebuild), out[8] = dangerous builds in the last run. */
+int hm_sys_stats(hm_system* s, uint64_t* out9);
 /* Thermalize velocities of all particles (ParticleGroup::thermalizeParticleMomenta). */
 int hm_sys_thermalize(hm_system* s, double kT);
 /* Launch stream used by the engine (a hipStream_t). */

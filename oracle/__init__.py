@@ -77,6 +77,9 @@ def lib():
             "os_get_forces": (None, [_p, _p, _p, _p]),
             "os_timestep": (C.c_uint64, [_p]),
             "os_n_builds": (C.c_uint64, [_p]),
+            "os_set_nlist_policy": (None, [_p, C.c_uint, C.c_int]),
+            "os_n_dangerous": (C.c_uint64, [_p]),
+            "os_shortest_rebuild": (C.c_uint, [_p]),
             "os_kinetic_energy": (_d, [_p]),
             "os_potential_energy": (_d, [_p]),
         }
@@ -448,6 +451,17 @@ class System:
     @property
     def n_builds(self):
         return lib().os_n_builds(self.h)
+
+    def set_nlist_policy(self, rebuild_check_delay=1, check_dist=True):
+        lib().os_set_nlist_policy(self.h, int(rebuild_check_delay), int(bool(check_dist)))
+
+    @property
+    def n_dangerous(self):
+        return lib().os_n_dangerous(self.h)
+
+    @property
+    def shortest_rebuild(self):
+        return lib().os_shortest_rebuild(self.h)
 
     def kinetic_energy(self):
         return lib().os_kinetic_energy(self.h)

@@ -966,6 +966,11 @@ struct os_system
     unsigned int *nlist, *n_neigh;
     int force_update;
     uint64_t n_builds;
+    /* rebuild policy and statistics (NeighborList.cc:1313-1426) */
+    unsigned int check_delay;
+    int dist_check;
+    uint64_t last_update, n_dangerous;
+    uint64_t update_periods[100];
     /* bonds (by tag) */
     size_t nb;
     unsigned int* bond_tags;
@@ -985,6 +990,8 @@ os_system* os_create(size_t N, double Lx, double Ly, double Lz, const double* po
 {
     os_system* s = (os_system*)calloc(1, sizeof(os_system));
     s->N = N;
+    s->check_delay = 1;
+    s->dist_check = 1;
     ob_box_init(&s->box, Lx, Ly, Lz);
     s->pos = (double*)malloc(sizeof(double) * 3 * N);
     s->vel = (double*)malloc(sizeof(double) * 3 * N);
@@ -1156,13 +1163,39 @@ static void os_apply_sort(os_system* s)
     s->force_update = 1;
 }
 
-static void os_nlist_compute(os_system* s)
+static void os_nlist_compute(os_system* s, uint64_t timestep)
 {
-    int rebuild = s->force_update;
-    if (!rebuild)
-        rebuild = on_distance_check(&s->box, s->N, s->pos, s->last, s->rcut, s->r_buff);
+    /* NeighborList::needsUpdating (hoomd/md/NeighborList.cc:1325-1418), check once per step */
+    int rebuild = 0, dangerous = 0;
+    if (s->force_update)
+    {
+        /* a forced build: no statistics, never dangerous (:1358-1368) */
+        rebuild = 1;
+    }
+    else if (!(timestep < s->last_update + s->check_delay)) /* shouldCheckDistance (:1313-1316) */
+    {
+        dangerous = s->dist_check && s->check_delay > 1
+                    && timestep == s->last_update + s->check_delay;
+        if (s->r_buff < 1e-6
+            || (!s->dist_check
+                && (s->check_delay == 0
+                    || (s->check_delay > 1 && timestep == s->last_update + s->check_delay))))
+            rebuild = 1;
+        else
+            rebuild = on_distance_check(&s->box, s->N, s->pos, s->last, s->rcut, s->r_buff);
+        if (rebuild && timestep > s->last_update)
+        {
+            uint64_t period = timestep - s->last_update;
+            if (period >= 100)
+                period = 99;
+            s->update_periods[period]++;
+        }
+    }
     if (!rebuild)
         return;
+    if (dangerous)
+        s->n_dangerous++;
+    s->last_update = timestep;
     s->force_update = 0;
     os_build_exclusions(s);
     for (;;)
@@ -1181,10 +1214,10 @@ static void os_nlist_compute(os_system* s)
     s->n_builds++;
 }
 
-static int os_compute_net_force(os_system* s)
+static int os_compute_net_force(os_system* s, uint64_t timestep)
 {
     size_t N = s->N;
-    os_nlist_compute(s);
+    os_nlist_compute(s, timestep);
     op_pair_lj(&s->box, N, s->pos, s->nlist, s->n_neigh, s->nmax, 1, s->sigma, s->epsilon, s->rcut,
                0.0, s->shift_mode, 1, s->pair_f, s->pair_v);
     memcpy(s->net_f, s->pair_f, sizeof(double) * 4 * N);
@@ -1210,7 +1243,7 @@ int os_run(os_system* s, uint64_t nsteps)
     double ndof = 3.0 * (double)N - (s->thermostat == 2 ? 0.0 : 3.0);
     if (!s->prepared)
     {
-        int r = os_compute_net_force(s);
+        int r = os_compute_net_force(s, s->timestep);
         if (r)
             return r;
         for (size_t j = 0; j < N; j++)
@@ -1231,7 +1264,7 @@ int os_run(os_system* s, uint64_t nsteps)
             alpha = oi_bussi_alpha(K, ndof, s->dt, s->tau_gamma, s->kT, t, s->seed, 0);
         }
         oi_nvt_step_one(&s->box, N, s->pos, s->vel, s->accel, s->image, s->dt, alpha);
-        int r = os_compute_net_force(s);
+        int r = os_compute_net_force(s, t + 1);
         if (r)
             return r;
         if (s->thermostat == 2)
@@ -1285,6 +1318,23 @@ void os_get_forces(const os_system* s, double* pf, double* pv, double* bf)
 
 uint64_t os_timestep(const os_system* s) { return s->timestep; }
 uint64_t os_n_builds(const os_system* s) { return s->n_builds; }
+
+void os_set_nlist_policy(os_system* s, unsigned int rebuild_check_delay, int check_dist)
+{
+    s->check_delay = rebuild_check_delay;
+    s->dist_check = check_dist != 0;
+}
+
+uint64_t os_n_dangerous(const os_system* s) { return s->n_dangerous; }
+
+unsigned int os_shortest_rebuild(const os_system* s)
+{
+    /* NeighborList::getSmallestRebuild (hoomd/md/NeighborList.cc:1428-1436) */
+    for (unsigned int i = 0; i < 100; i++)
+        if (s->update_periods[i] != 0)
+            return i;
+    return 100;
+}
 double os_kinetic_energy(const os_system* s) { return oi_kinetic_energy(s->N, s->vel, s->mass); }
 
 double os_potential_energy(const os_system* s)

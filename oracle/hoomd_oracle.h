@@ -177,6 +177,10 @@ void os_get_forces(const os_system* s, double* pair_force4, double* pair_virial6
                    double* bond_force4);
 uint64_t os_timestep(const os_system* s);
 uint64_t os_n_builds(const os_system* s);
+/* NeighborList rebuild_check_delay / check_dist (NeighborList.cc:1313-1418); defaults 1, 1 */
+void os_set_nlist_policy(os_system* s, unsigned int rebuild_check_delay, int check_dist);
+uint64_t os_n_dangerous(const os_system* s);
+unsigned int os_shortest_rebuild(const os_system* s);
 double os_kinetic_energy(const os_system* s);
 double os_potential_energy(const os_system* s);
 

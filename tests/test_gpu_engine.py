@@ -167,6 +167,12 @@ def test_thermo_matches_oracle(hoomd):
     W = (rv[0].sum() + rv[3].sum() + rv[5].sum()) / 3
     P = (2 * ke / 3 + W) / L**3
     assert abs(thermo.pressure - P) < 1e-5 * abs(P) + 1e-8
+    # pressure tensor: kinetic m v_a v_b (FP64 closed form) + pair virial (FP64 oracle)
+    kin = [np.sum(vel[:, a] * vel[:, b]) for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+    ref = (np.array(kin) + rv.sum(axis=1)) / L**3
+    got = np.array(thermo.pressure_tensor)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    assert abs((got[0] + got[3] + got[5]) / 3 - thermo.pressure) < 1e-9 * abs(P) + 1e-12
 
 
 def test_langevin_bead_spring_matches_oracle(hoomd):
@@ -344,3 +350,32 @@ def test_autotuner_sweeps_and_keeps_results(hoomd):
         out.append(sim.state.get_snapshot().particles.position.copy())
     # different launch shapes change only the FP32 summation order of the forces
     assert np.abs(out[0] - out[1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("delay,dist,kT", [(3, False, 1.2), (0, False, 1.2), (4, True, 40.0),
+                                           (1, True, 40.0)])
+def test_rebuild_policy_matches_oracle(hoomd, delay, dist, kT):
+    """rebuild_check_delay / check_dist (NeighborList::needsUpdating, NeighborList.cc:1325-1418):
+    the engine rebuilds on exactly the oracle's steps, counts the same dangerous builds and the
+    same shortest period, and the trajectories agree."""
+    pos, vel, L = lj_state(8, kT=kT)
+    sim, lj = make_sim(hoomd, pos, vel, L, dt=0.002)
+    lj.nlist.rebuild_check_delay = delay
+    lj.nlist.check_dist = dist
+    sim.run(40)
+    ref = O.System(pos, vel, L, dt=0.002)
+    ref.set_nlist_policy(delay, dist)
+    ref.run(40)
+    assert lj.nlist.num_builds == ref.n_builds
+    assert lj.nlist.num_dangerous_builds == ref.n_dangerous
+    assert lj.nlist.shortest_rebuild == ref.shortest_rebuild
+    rp, rv, ri = ref.state()
+    snap = sim.state.get_snapshot()
+    gp = snap.particles.position + snap.particles.image * L
+    # FP32 pair forces: the hot system's larger forces carry larger rounding
+    assert np.abs(gp - (rp + ri * L)).max() < (1e-6 if kT < 10 else 2e-5)
+    # a second run counts its own builds (System::run resets the statistics)
+    sim.run(10)
+    ref2_builds = ref.n_builds
+    ref.run(10)
+    assert lj.nlist.num_builds == ref.n_builds - ref2_builds

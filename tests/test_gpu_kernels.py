@@ -369,7 +369,7 @@ def test_thermo_sums(H):
     f = rng.normal(size=(N, 4)).astype(np.float32)
     vir = rng.normal(size=(6, N)).astype(np.float32)
     d_v, d_f, d_vir = dev(v4), dev(f), dev(vir)
-    out = H.DeviceArray((9,), np.float64, fill=0)
+    out = H.DeviceArray((15,), np.float64, fill=0)
     H.check(H.lib().hm_thermo(N, d_v.ptr, H.pointer_array([d_f.ptr]), H.pointer_array([d_vir.ptr]),
                               1, out.ptr, None))
     H.check(H.lib().hm_synchronize())
@@ -378,8 +378,12 @@ def test_thermo_sums(H):
     assert math.isclose(o[0], ke, rel_tol=1e-12)
     assert math.isclose(o[1], f[:, 3].astype(float).sum(), rel_tol=1e-9, abs_tol=1e-9)
     vs = vir.astype(float).sum(axis=1)
-    assert np.allclose(o[3:], vs, rtol=1e-9, atol=1e-9)
+    assert np.allclose(o[3:9], vs, rtol=1e-9, atol=1e-9)
     assert math.isclose(o[2], (vs[0] + vs[3] + vs[5]) / 3, rel_tol=1e-9, abs_tol=1e-9)
+    m, v = v4[:, 3], v4[:, :3]
+    kin = [np.sum(m * v[:, a] * v[:, b]) for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+    assert np.allclose(o[9:15], kin, rtol=1e-12, atol=1e-9)
+    assert math.isclose(o[9] + o[12] + o[14], 2 * o[0], rel_tol=1e-12)
 
 
 # ------------------------------------------------------------------ Hilbert sort

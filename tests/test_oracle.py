@@ -479,3 +479,34 @@ def test_fcc_lattice_inputs():
     pos, L = fcc_lattice(16)
     assert len(pos) == 16384 and math.isclose(L, 26.8735, rel_tol=1e-5)
     assert (pos >= -L / 2).all() and (pos < L / 2).all()
+
+
+def test_rebuild_policy_counts():
+    """NeighborList::needsUpdating (NeighborList.cc:1325-1418): without distance checks the list
+    is rebuilt exactly every rebuild_check_delay steps (every step for 0); with them, no check
+    happens before the delay has passed, a build on the first check after a build is counted as
+    dangerous, and the shortest period is the delay or longer."""
+    from lattices import maxwell_boltzmann, perturbed_fcc
+    pos, L = perturbed_fcc(4, 0.8442, r=0.05, seed=2)
+    vel = maxwell_boltzmann(len(pos), 1.2, seed=2)
+    for delay, dist, steps, expect in ((5, False, 50, 1 + 10), (0, False, 20, 1 + 20),
+                                       (3, False, 31, 1 + 10)):
+        s = O.System(pos, vel, L, dt=0.005)
+        s.set_nlist_policy(delay, dist)
+        s.run(steps)
+        assert s.n_builds == expect
+        assert s.n_dangerous == 0
+        assert s.shortest_rebuild == max(delay, 1) if delay else s.shortest_rebuild == 1
+    # hot system, distance checks from the 4th step after a build: every build is dangerous
+    hot = maxwell_boltzmann(len(pos), 40.0, seed=3)
+    s = O.System(pos, hot, L, dt=0.005)
+    s.set_nlist_policy(4, True)
+    s.run(40)
+    assert s.n_builds > 3
+    assert s.n_dangerous == s.n_builds - 1
+    assert s.shortest_rebuild == 4
+    # default policy: the first check after a build is never dangerous
+    s = O.System(pos, hot, L, dt=0.005)
+    s.run(40)
+    assert s.n_dangerous == 0
+    assert 1 <= s.shortest_rebuild < 100
