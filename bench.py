@@ -119,6 +119,53 @@ def cpu_baseline(n_cells, rho, seed, steps):
     return steps / dt, dt, int(s.n_builds - b0)
 
 
+HOWTO_A100_TPS = {1: 15955.0, 4: 7276.5}  # sphinx-doc/howto/determine-the-most-efficient-device.rst
+
+
+def howto(replicate, steps, warmup, seed):
+    """The reference's how-to device benchmark (sphinx-doc/howto/determine-the-most-efficient-
+    device.py): spheres.gsd (2048 particles at density 1) replicated replicate^3 times, momenta
+    thermalized at kT = 1.2, WCA (LJ sigma = epsilon = 1, r_cut = 2^(1/6)), Cell(buffer = 0.2),
+    ConstantVolume + Bussi(kT = 1.2), dt = 0.001; run(100) until tuning completes, warm up, then
+    time. Prints one JSON line, the reference's published A100 figure beside it."""
+    import hoomd_amd as hoomd
+    from hoomd_amd import _lib
+    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
+    sim.create_state_from_gsd(os.path.join(ROOT, "tests", "golden", "spheres.gsd"))
+    sim.state.replicate(replicate, replicate, replicate)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.2)
+    cell = hoomd.md.nlist.Cell(buffer=0.2)
+    lj = hoomd.md.pair.LJ(nlist=cell)
+    lj.params[("A", "A")] = dict(sigma=1, epsilon=1)
+    lj.r_cut[("A", "A")] = 2 ** (1 / 6)
+    bussi = hoomd.md.methods.thermostats.Bussi(kT=1.2)
+    method = hoomd.md.methods.ConstantVolume(filter=hoomd.filter.All(), thermostat=bussi)
+    sim.operations.integrator = hoomd.md.Integrator(dt=0.001, methods=[method], forces=[lj])
+    N = sim.state.N_particles
+    while not sim.operations.is_tuning_complete:
+        sim.run(100)
+    log(f"[bench] how-to N={N}: warmup {warmup} steps")
+    sim.run(warmup)
+    _lib.check(_lib.lib().hm_synchronize())
+    t0 = time.perf_counter()
+    sim.run(steps)
+    _lib.check(_lib.lib().hm_synchronize())
+    elapsed = time.perf_counter() - t0
+    tps = steps / elapsed
+    ref = HOWTO_A100_TPS.get(replicate)
+    print(json.dumps({
+        "metric": "timesteps/s (reference how-to device benchmark, WCA)", "value": tps,
+        "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+        "reference_a100_tps": ref, "ratio_to_reference_a100": tps / ref if ref else None,
+        "dtype": "f32 pair arithmetic, f64 positions/integration",
+        "data": "spheres.gsd (the reference's how-to input) replicated, thermalized",
+        "config": {"workload": f"how-to WCA N={N} (spheres.gsd x {replicate}^3), buffer 0.2, "
+                               "Bussi kT=1.2, dt=0.001", "N": N,
+                   "nlist_builds_in_timed_steps": cell.num_builds}}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +186,15 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="profiling mode: device-to-device copy of this many MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--howto", type=int, default=0, metavar="R",
+                    help="run the reference's how-to device benchmark instead: spheres.gsd "
+                         "replicated R x R x R (1: N = 2048, 4: N = 131072)")
     a = ap.parse_args()
+    if a.howto:
+        if a.gpus != 1:
+            print("bench.py runs on exactly one GPU", file=sys.stderr)
+            return 2
+        return howto(a.howto, a.steps, a.warmup, a.seed)
     if a.copy_calibration:
         from hoomd_amd import _lib
         nb = a.copy_calibration << 20

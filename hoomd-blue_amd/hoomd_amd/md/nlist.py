@@ -40,6 +40,8 @@ class NeighborList:
             return None
         st = np.zeros(9, np.uint64)
         _lib.check(_lib.lib().hm_sys_stats(sim._system, st.ctypes.data), "stats")
+        if sim._run_stats is not None:  # a run split at writer steps
+            st[6], st[7], st[8] = sim._run_stats
         return st
 
     @property

@@ -38,6 +38,10 @@ class Operations:
     def add(self, op):
         from .md.compute import ThermodynamicQuantities
         from .tune import ParticleSorter
+        from .write import GSD
+        if isinstance(op, GSD):
+            self.writers.append(op)
+            return
         if isinstance(op, ParticleSorter):
             self.tuners.append(op)
         elif isinstance(op, ThermodynamicQuantities):
@@ -70,6 +74,10 @@ class Operations:
         _lib.check(_lib.lib().hm_sys_set_autotune(sim._system, 1), "autotune")
 
     def remove(self, op):
+        if op in self.writers:
+            self.writers.remove(op)
+            op.close()
+            return
         for lst in (self.tuners, self.computes):
             if op in lst:
                 lst.remove(op)
@@ -86,9 +94,11 @@ class Simulation:
         self._system = None
         self._attached_key = None
         self._timestep = 0
+        self._timestep_set = False
         self._tps = None
         self._walltime = 0.0
         self._configured = False
+        self._run_stats = None
         self.operations = Operations(self)
 
     # -------------------------------------------------------------- properties
@@ -116,6 +126,7 @@ class Simulation:
         if self._state is not None:
             raise RuntimeError("Cannot set timestep after the state is initialised")
         self._timestep = int(v)
+        self._timestep_set = True
 
     @property
     def tps(self):
@@ -130,6 +141,18 @@ class Simulation:
         if self._state is not None:
             raise RuntimeError("Cannot initialize more than once")
         self._state = State(self, snapshot)
+
+    def create_state_from_gsd(self, filename, frame=-1):
+        """Simulation.create_state_from_gsd (hoomd/simulation.py:188-240): the state from one
+        frame of a GSD file. Unless the timestep was set before, it becomes the frame's
+        configuration/step."""
+        if self._state is not None:
+            raise RuntimeError("Cannot initialize more than once")
+        from . import gsd
+        snap = gsd.read_snapshot(filename, frame)
+        if not self._timestep_set:
+            self._timestep = snap._step
+        self._state = State(self, snap)
 
     def _invalidate(self):
         """Operations changed: the native system is rebuilt (from the current state) on next use."""
@@ -197,17 +220,48 @@ class Simulation:
 
     # -------------------------------------------------------------- run
     def run(self, steps, write_at_start=False):
+        """Simulation.run (System::run, System.cc:80-181). The engine runs the steps between two
+        writer steps in one call; writers act after the step counter reaches their step, and on
+        the starting step too when write_at_start is set."""
         steps = int(steps)
         if steps < 0:
             raise ValueError("steps must be non-negative")
         self._attach()
+        L = _lib.lib()
+        writers = self.operations.writers
+        self._run_stats = None
         t0 = time.perf_counter()
-        _lib.check(_lib.lib().hm_sys_run(self._system, steps), "run")
-        _lib.check(_lib.lib().hm_synchronize(), "run")
+        if not writers:
+            _lib.check(L.hm_sys_run(self._system, steps), "run")
+        else:
+            # the engine restarts its build statistics at each call: keep them over the run
+            agg = np.array([0, 100, 0], np.uint64)
+            st = np.zeros(9, np.uint64)
+            t = self.timestep
+            end = t + steps
+            if write_at_start:
+                self._write(t)
+            while t < end:
+                nxt = [w.trigger.next_after(t) for w in writers]
+                stop = min([s for s in nxt if s is not None] + [end])
+                _lib.check(L.hm_sys_run(self._system, stop - t), "run")
+                _lib.check(L.hm_sys_stats(self._system, st.ctypes.data), "stats")
+                agg[0] += st[6]
+                agg[1] = min(agg[1], st[7])
+                agg[2] += st[8]
+                t = stop
+                self._write(t)
+            self._run_stats = agg
+        _lib.check(L.hm_synchronize(), "run")
         dt = time.perf_counter() - t0
         self._walltime += dt
         self._tps = steps / dt if dt > 0 and steps else self._tps
         self._warn_dangerous_builds()
+
+    def _write(self, t):
+        for w in self.operations.writers:
+            if w.trigger(t):
+                w._write(self._state, t)
 
     def _warn_dangerous_builds(self):
         """The reference's notice for builds taken on the first distance check after a build

@@ -65,19 +65,35 @@ class Snapshot:
         self.bonds = _Bonds()
         self.communicator = _Communicator()
 
-    def replicate(self, nx, ny, nz):
-        """Snapshot.replicate: tile the box nx x ny x nz (positions wrapped into the new box)."""
-        Lx, Ly, Lz = self.configuration.box[:3]
+    def replicate(self, nx, ny, nz=1):
+        """Snapshot.replicate (SnapshotParticleData::replicate, ParticleData.cc:3742-3811): each
+        particle is unwrapped with its image, and copy j (loops x, y, z, z fastest) gets tag
+        j*N + i at fraction f/n + l/n of the new box, with a fresh image flag. Bond tags shift by
+        j*N (BondedGroupData.cc:1366-1400)."""
+        nx, ny, nz = int(nx), int(ny), int(nz)
+        if min(nx, ny, nz) < 1:
+            raise ValueError("replicate counts must be positive")
+        L = np.asarray(self.configuration.box[:3], float)
+        n = np.array([nx, ny, nz], float)
+        newL = L * n
         p = self.particles
         N = p.N
-        shifts = np.array([[i, j, k] for i in range(nx) for j in range(ny) for k in range(nz)],
-                          float) * [Lx, Ly, Lz]
-        pos = (p.position[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
-        newL = np.array([nx * Lx, ny * Ly, nz * Lz])
-        pos = pos - (newL - [Lx, Ly, Lz]) / 2
-        pos = (pos + newL / 2) % newL - newL / 2
-        reps = len(shifts)
+        frac = (p.position + p.image * L + L / 2) / L
+        lmn = np.array([[l, m, k] for l in range(nx) for m in range(ny) for k in range(nz)], float)
+        f_new = (frac[None, :, :] / n + lmn[:, None, :] / n).reshape(-1, 3)
+        q = -newL / 2 + f_new * newL
+        g = (q + newL / 2) / newL - 0.5
+        img = np.where(g >= 0, g + 0.5, g - 0.5).astype(np.int64)
+        q = q - img * newL
+        hi = q >= newL / 2
+        q = np.where(hi, q - newL, q)
+        img = img + hi
+        lo = q < -newL / 2
+        q = np.where(lo, q + newL, q)
+        img = img - lo
+        reps = len(lmn)
         vel = np.tile(p.velocity, (reps, 1))
+        acc = np.tile(p.acceleration, (reps, 1))
         typeid = np.tile(p.typeid, reps)
         mass = np.tile(p.mass, reps)
         groups = [self.bonds.group + r * N for r in range(reps)]
@@ -85,8 +101,10 @@ class Snapshot:
         types = list(p.types)
         p.N = N * reps
         p.types = types
-        p.position[:] = pos
+        p.position[:] = q
+        p.image[:] = img
         p.velocity[:] = vel
+        p.acceleration[:] = acc
         p.typeid[:] = typeid
         p.mass[:] = mass
         nb = self.bonds.N

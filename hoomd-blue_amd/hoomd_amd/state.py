@@ -60,10 +60,25 @@ class State:
         return copy.deepcopy(self._snapshot)
 
     def set_snapshot(self, snapshot):
-        if snapshot.particles.N != self.N_particles:
-            raise RuntimeError("set_snapshot: the number of particles must not change")
+        """Replace the system's particles (hoomd/state.py set_snapshot). Keeping N, the box and
+        the types writes positions, velocities and images into the running system; any other
+        change rebuilds the device system on next use, as initializeFromSnapshot does."""
+        box = snapshot.configuration.box
+        if any(abs(t) > 0 for t in box[3:6]):
+            raise ValueError("hoomd_amd supports orthorhombic boxes only (tilt factors must be 0)")
+        sim = self._simulation
+        same = (snapshot.particles.N == self.N_particles
+                and list(box[:3]) == list(self._snapshot.configuration.box[:3])
+                and list(snapshot.particles.types) == self.particle_types
+                and snapshot.bonds.N == self._snapshot.bonds.N
+                and np.array_equal(snapshot.particles.typeid, self._snapshot.particles.typeid)
+                and np.array_equal(snapshot.particles.mass, self._snapshot.particles.mass)
+                and np.array_equal(snapshot.bonds.group, self._snapshot.bonds.group))
+        if not same:
+            sim._destroy_system()
+            sim._invalidate()
         self._snapshot = copy.deepcopy(snapshot)
-        sysh = self._simulation._system
+        sysh = sim._system
         if sysh is not None:
             p = self._snapshot.particles
             pos = np.ascontiguousarray(p.position, np.float64)
@@ -71,6 +86,12 @@ class State:
             img = np.ascontiguousarray(p.image, np.int32)
             _lib.check(_lib.lib().hm_sys_set_state(sysh, pos.ctypes.data, vel.ctypes.data,
                                                    img.ctypes.data), "set_state")
+
+    def replicate(self, nx, ny, nz=1):
+        """State.replicate (hoomd/state.py:580-609): the snapshot replicated nx x ny x nz."""
+        snap = self.get_snapshot()
+        snap.replicate(nx, ny, nz)
+        self.set_snapshot(snap)
 
     def thermalize_particle_momenta(self, filter, kT):
         """Draw Maxwell-Boltzmann velocities (ParticleGroup::thermalizeParticleMomenta) on the
