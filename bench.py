@@ -119,10 +119,12 @@ def cpu_baseline(n_cells, rho, seed, steps):
     return steps / dt, dt, int(s.n_builds - b0)
 
 
-HOWTO_A100_TPS = {1: 15955.0, 4: 7276.5}  # sphinx-doc/howto/determine-the-most-efficient-device.rst
+# sphinx-doc/howto/determine-the-most-efficient-device.rst: one A100; one EPYC 7742 MPI rank
+HOWTO_A100_TPS = {1: 15955.0, 4: 7276.5}
+HOWTO_EPYC_1RANK_TPS = {1: 2699.0, 4: 36.072}
 
 
-def howto(replicate, steps, warmup, seed):
+def howto(replicate, steps, warmup, seed, cpu_seconds):
     """The reference's how-to device benchmark (sphinx-doc/howto/determine-the-most-efficient-
     device.py): spheres.gsd (2048 particles at density 1) replicated replicate^3 times, momenta
     thermalized at kT = 1.2, WCA (LJ sigma = epsilon = 1, r_cut = 2^(1/6)), Cell(buffer = 0.2),
@@ -142,6 +144,7 @@ def howto(replicate, steps, warmup, seed):
     method = hoomd.md.methods.ConstantVolume(filter=hoomd.filter.All(), thermostat=bussi)
     sim.operations.integrator = hoomd.md.Integrator(dt=0.001, methods=[method], forces=[lj])
     N = sim.state.N_particles
+    snap0 = sim.state.get_snapshot()
     while not sim.operations.is_tuning_complete:
         sim.run(100)
     log(f"[bench] how-to N={N}: warmup {warmup} steps")
@@ -153,6 +156,16 @@ def howto(replicate, steps, warmup, seed):
     elapsed = time.perf_counter() - t0
     tps = steps / elapsed
     ref = HOWTO_A100_TPS.get(replicate)
+    cpu = None
+    if cpu_seconds > 0:
+        cps, cdt, csteps = howto_cpu(snap0, seed, cpu_seconds)
+        ref_cpu = HOWTO_EPYC_1RANK_TPS.get(replicate)
+        cpu = {"value": cps, "unit": "timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"N={N} how-to WCA state after thermalize, {csteps} steps in {cdt:.1f} s "
+                         "on one host core",
+               "reference_epyc7742_1rank_tps": ref_cpu,
+               "ratio_to_reference_1rank": cps / ref_cpu if ref_cpu else None}
+        log(f"[bench] CPU oracle {cps:.1f} steps/s")
     print(json.dumps({
         "metric": "timesteps/s (reference how-to device benchmark, WCA)", "value": tps,
         "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -162,8 +175,28 @@ def howto(replicate, steps, warmup, seed):
         "data": "spheres.gsd (the reference's how-to input) replicated, thermalized",
         "config": {"workload": f"how-to WCA N={N} (spheres.gsd x {replicate}^3), buffer 0.2, "
                                "Bussi kT=1.2, dt=0.001", "N": N,
-                   "nlist_builds_in_timed_steps": cell.num_builds}}), flush=True)
+                   "nlist_builds_in_timed_steps": cell.num_builds},
+        "cpu_baseline": cpu}), flush=True)
     return 0
+
+
+def howto_cpu(snap, seed, seconds):
+    """The oracle (FP64 C restatement of the reference CPU path) on the how-to state, one host
+    core: a 20-step probe sizes a sample of about ``seconds``."""
+    import oracle as O
+    L = np.array(snap.configuration.box[:3])
+    s = O.System(snap.particles.position, snap.particles.velocity, L, r_cut=2 ** (1 / 6),
+                 mode="none", buffer=0.2, dt=0.001, seed=seed)
+    s.set_bussi(1.2, 0.0)
+    s.set_sorter(200, 256)
+    t0 = time.perf_counter()
+    s.run(20)
+    probe = time.perf_counter() - t0
+    steps = max(20, int(seconds * 20 / max(probe, 1e-9)))
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    return steps / dt, dt, steps
 
 
 def main():
@@ -194,7 +227,7 @@ def main():
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
             return 2
-        return howto(a.howto, a.steps, a.warmup, a.seed)
+        return howto(a.howto, a.steps, a.warmup, a.seed, 0 if a.no_cpu else 8.0)
     if a.copy_calibration:
         from hoomd_amd import _lib
         nb = a.copy_calibration << 20

@@ -256,6 +256,31 @@ int hm_pair_lj(const hm_pair_args* a, const hm_lj_params* params, hm_stream_t st
     return check(launch_pair_lj(k, params, cfg, (hipStream_t)stream), "hm_pair_lj");
 }
 
+int hm_pair_table(const hm_pair_args* a, const hm_table_params* params, hm_stream_t stream)
+{
+    if (!a || !params || !a->pos || !a->nlist || !a->n_neigh || !a->force
+        || (a->compute_virial && !a->virial) || a->ntypes == 0 || a->ntypes > HM_MAX_TYPES
+        || (a->nmax & 7u) || a->nmax == 0)
+        return fail(HM_ERR_ARGS, "hm_pair_table: invalid arguments");
+    if (a->tpp && a->tpp != 1 && a->tpp != 2 && a->tpp != 4 && a->tpp != 8 && a->tpp != 16)
+        return fail(HM_ERR_ARGS, "hm_pair_table: tpp must be 1, 2, 4, 8 or 16");
+    if (a->block_size && (a->block_size % 64 || a->block_size > 1024))
+        return fail(HM_ERR_ARGS, "hm_pair_table: block_size must be a multiple of 64 <= 1024");
+    if (a->shift_mode != 0)
+        return fail(HM_ERR_ARGS, "hm_pair_table: tables support shift_mode 0 (none) only");
+    const unsigned int nt = a->ntypes;
+    for (unsigned int q = 0; q < nt * nt; q++)
+        if (params[q].width && a->r_cut[q] > 0.0
+            && (!params[q].U || !params[q].F || !(params[q].r_min < a->r_cut[q])))
+            return fail(HM_ERR_ARGS, "hm_pair_table: a table needs U, F and r_min < r_cut");
+    PairKernelArgs k;
+    fill_pair_args(k, a->N, (const double4*)a->pos, a->box, a->nlist, a->nmax, a->n_neigh,
+                   a->ntypes, a->r_cut, a->r_on, 0, a->compute_virial, (float4*)a->force,
+                   a->virial);
+    PairLaunch cfg{a->tpp, a->block_size};
+    return check(launch_pair_table(k, params, cfg, (hipStream_t)stream), "hm_pair_table");
+}
+
 int hm_bond_fene(const hm_bond_args* a, const hm_fene_params* params, hm_stream_t stream)
 {
     if (!a || !params || !a->pos || !a->tag || !a->rtag || !a->bond_row || !a->bond_other

@@ -66,14 +66,17 @@ template<class T> T* halloc(size_t n)
 
 struct ForceSlot
 {
-    int kind = 0; // 0 LJ pair, 1 FENE bond
+    int kind = 0; // 0 LJ pair, 1 FENE bond, 2 Table pair
     float4* f = nullptr;
     float* v = nullptr;
-    // LJ
+    // pair forces
     double rcut[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     double ron[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     hm_lj_params p[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     int mode = 0;
+    // Table: U then F of every type pair in one device array
+    float* tab = nullptr;
+    hm_table_params tp[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     // FENE, CSR by tag
     unsigned int* row = nullptr;
     unsigned int* other = nullptr;
@@ -82,6 +85,10 @@ struct ForceSlot
     hm_fene_params fp[HM_MAX_TYPES] = {};
     unsigned int* flag = nullptr;
 };
+bool is_pair(const ForceSlot& f)
+{
+    return f.kind == 0 || f.kind == 2;
+}
 } // namespace
 
 struct hm_system
@@ -194,6 +201,7 @@ struct hm_system
             dfree(f.other);
             dfree(f.btype);
             dfree(f.flag);
+            dfree(f.tab);
         }
         forces.clear();
         dfree(cell_size);
@@ -261,7 +269,7 @@ struct hm_system
             v = 0.0;
         rcut_max = 0.0;
         for (auto& f : forces)
-            if (f.kind == 0)
+            if (is_pair(f))
                 for (unsigned int q = 0; q < ntypes * ntypes; q++)
                     rcut_mat[q] = std::max(rcut_mat[q], f.rcut[q]);
         for (unsigned int q = 0; q < ntypes * ntypes; q++)
@@ -433,9 +441,16 @@ struct hm_system
                 return false;
         return (size_t)ncell * cmax <= OFFSET_CODE_MAX_N;
     }
+    // the requested kernel form, where it applies: the cell-tiled forms (0, 3) are built for LJ
+    // only, so a Table force runs the relative gather (2); the relative forms need rel_ok()
     int pair_kind() const
     {
-        return (pair_kernel_kind == 2 || pair_kernel_kind == 3) && !rel_ok() ? 1 : pair_kernel_kind;
+        int k = pair_kernel_kind;
+        if (k == 0 || k == 3)
+            for (const auto& f : forces)
+                if (f.kind == 2)
+                    k = 2;
+        return (k == 2 || k == 3) && !rel_ok() ? 1 : k;
     }
     bool uses_rel() const { return pair_kind() == 2 || pair_kind() == 3; }
     // Pair list (tags, a < b) from the 16-bit local rows: entry l of a particle in home cell c
@@ -569,7 +584,10 @@ struct hm_system
             k.compute_virial = with_virial;
             k.force = f.f;
             k.virial = f.v;
-            hchk(launch_pair_rel_lj(k, f.p, pair_cfg, stream), "pair force");
+            if (f.kind == 2)
+                hchk(launch_pair_rel_table(k, f.tp, pair_cfg, stream), "pair force");
+            else
+                hchk(launch_pair_rel_lj(k, f.p, pair_cfg, stream), "pair force");
             return;
         }
         if (kind == 1)
@@ -577,7 +595,10 @@ struct hm_system
             PairKernelArgs k;
             fill_pair_args(k, N, pos, hbox, nlist, nmax, n_neigh, ntypes, f.rcut, f.ron, f.mode,
                            with_virial, f.f, f.v);
-            hchk(launch_pair_lj(k, f.p, pair_cfg, stream), "pair force");
+            if (f.kind == 2)
+                hchk(launch_pair_table(k, f.tp, pair_cfg, stream), "pair force");
+            else
+                hchk(launch_pair_lj(k, f.p, pair_cfg, stream), "pair force");
             return;
         }
         PairCellArgs c;
@@ -616,7 +637,7 @@ struct hm_system
     {
         for (auto& f : forces)
         {
-            if (f.kind == 0)
+            if (is_pair(f))
                 launch_pair(f, with_virial);
             else
             {
@@ -776,7 +797,7 @@ struct hm_system
     {
         ForceSlot* f = nullptr;
         for (auto& x : forces)
-            if (x.kind == 0)
+            if (is_pair(x))
             {
                 f = &x;
                 break;
@@ -1206,6 +1227,57 @@ int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
             f.p[q] = params[q];
         }
         f.mode = mode;
+        f.f = dalloc<float4>(s->N);
+        f.v = dalloc<float>((size_t)6 * s->N);
+        hchk(hipMemset(f.f, 0, sizeof(float4) * s->N), "clear");
+        hchk(hipMemset(f.v, 0, sizeof(float) * 6 * s->N), "clear");
+        s->forces.push_back(f);
+        s->prepared = false;
+        s->nmax = 0;
+        s->clear_graphs();
+        return (int)s->forces.size() - 1;
+    });
+}
+
+int hm_sys_add_pair_table(hm_system* s, const double* r_cut, const double* r_min,
+                          const unsigned int* width, const double* U, const double* F)
+{
+    if (!s || !r_cut || !r_min || !width || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_pair_table: invalid arguments");
+    const unsigned int nt = s->ntypes;
+    size_t total = 0;
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        if (width[q] && r_cut[q] > 0.0 && !(r_min[q] < r_cut[q]))
+            return fail(HM_ERR_ARGS, "hm_sys_add_pair_table: r_min must be below r_cut");
+        total += width[q];
+    }
+    if (total && (!U || !F))
+        return fail(HM_ERR_ARGS, "hm_sys_add_pair_table: U and F are required");
+    return guarded([&]() {
+        ForceSlot f;
+        f.kind = 2;
+        std::vector<float> h(2 * total + 1, 0.f);
+        for (size_t k = 0; k < total; k++)
+        {
+            h[k] = (float)U[k];
+            h[total + k] = (float)F[k];
+        }
+        f.tab = dalloc<float>(h.size());
+        hchk(hipMemcpy(f.tab, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice),
+             "upload");
+        size_t off = 0;
+        for (unsigned int q = 0; q < nt * nt; q++)
+        {
+            f.rcut[q] = r_cut[q];
+            f.ron[q] = 0.0;
+            f.tp[q].r_min = r_min[q];
+            f.tp[q].width = width[q];
+            f.tp[q].U = f.tab + off;
+            f.tp[q].F = f.tab + total + off;
+            off += width[q];
+        }
+        f.mode = 0;
         f.f = dalloc<float4>(s->N);
         f.v = dalloc<float>((size_t)6 * s->N);
         hchk(hipMemset(f.f, 0, sizeof(float4) * s->N), "clear");
@@ -1750,7 +1822,7 @@ hm_stream_t hm_sys_stream(hm_system* s)
 int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg_ms,
                      uint64_t* nlist_entries)
 {
-    if (!s || force_id < 0 || force_id >= (int)s->forces.size() || s->forces[force_id].kind != 0
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size() || !is_pair(s->forces[force_id])
         || iters == 0 || !avg_ms)
         return fail(HM_ERR_ARGS, "hm_sys_time_pair: invalid arguments");
     return guarded([&]() {

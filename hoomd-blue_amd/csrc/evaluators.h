@@ -89,7 +89,87 @@ template<typename Real> struct EvaluatorPairLJ
         return "lj";
     }
 
+    static constexpr int shift_modes = 3; // none, shift, xplor
+
     Real rsq, rcutsq, lj1, lj2;
+};
+
+// Tabulated pair potential (hoomd/md/EvaluatorPairTable.h:30-196): U and F = -dU/dr on width
+// points r_min + k dr, dr = (r_cut - r_min) / width, interpolated linearly; the point after the
+// last is zero, and the pair does not interact below r_min or from r_cut on. The tables are
+// device arrays (the reference keeps them in ManagedArrays); param_type carries their address.
+template<typename Real> struct EvaluatorPairTable
+{
+    struct param_type
+    {
+        const Real* U;
+        const Real* F;
+        Real rmin;
+        unsigned int width;
+
+        static param_type make(const Real* U, const Real* F, double rmin, unsigned int width)
+        {
+            param_type p;
+            p.U = U;
+            p.F = F;
+            p.rmin = Real(rmin);
+            p.width = width;
+            return p;
+        }
+    };
+
+    __host__ __device__ __forceinline__ EvaluatorPairTable(Real _rsq, Real _rcutsq,
+                                                            const param_type& p)
+        : rsq(_rsq), rcutsq(_rcutsq), rmin(p.rmin), width(p.width), U(p.U), F(p.F)
+    {
+    }
+
+    __host__ __device__ static constexpr bool needsCharge()
+    {
+        return false;
+    }
+    __host__ __device__ void setCharge(Real, Real) { }
+
+    // EvaluatorPairTable::evalForceAndEnergy (EvaluatorPairTable.h:152-196); tables do not shift.
+    // The index stays below width: rounding can put r just under r_cut at width dr, where the
+    // reference would read one element past its table.
+    __host__ __device__ __forceinline__ bool evalForceAndEnergy(Real& force_divr, Real& pair_eng,
+                                                                bool) const
+    {
+        const Real r = sqrt(rsq);
+        if (!(rsq < rcutsq) || r < rmin || width == 0)
+            return false;
+        const Real rcut = sqrt(rcutsq);
+        const Real delta_r = (rcut - rmin) / Real(width);
+        const Real value_f = (r - rmin) / delta_r;
+        unsigned int k = (unsigned int)floor(value_f);
+        k = k < width ? k : width - 1;
+        const Real U0 = U[k], F0 = F[k];
+        Real U1 = Real(0), F1 = Real(0);
+        if (k + 1 < width)
+        {
+            U1 = U[k + 1];
+            F1 = F[k + 1];
+        }
+        const Real f = value_f - Real(k);
+        const Real Ur = U0 + f * (U1 - U0);
+        const Real Fr = F0 + f * (F1 - F0);
+        if (rsq > Real(0))
+            force_divr = Fr / r;
+        pair_eng = Ur;
+        return true;
+    }
+
+    static const char* getName()
+    {
+        return "table";
+    }
+
+    static constexpr int shift_modes = 1; // none only
+
+    Real rsq, rcutsq, rmin;
+    unsigned int width;
+    const Real *U, *F;
 };
 
 // Fast form of the pair-evaluator concept for the FP32 gather kernels. An entry holds the

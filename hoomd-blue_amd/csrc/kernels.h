@@ -228,6 +228,9 @@ struct PairKernelArgs
 };
 hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params,
                           PairLaunch cfg, hipStream_t s);
+// The same kernel with the Table evaluator (mode none)
+hipError_t launch_pair_table(const PairKernelArgs& a, const hm_table_params* params,
+                             PairLaunch cfg, hipStream_t s);
 
 // Pair kernel reading one 16-byte relative position per neighbour (see RelFrame).
 struct PairRelArgs
@@ -257,6 +260,8 @@ struct PairRelArgs
 };
 hipError_t launch_pair_rel_lj(const PairRelArgs& a, const hm_lj_params* params, PairLaunch cfg,
                               hipStream_t s);
+hipError_t launch_pair_rel_table(const PairRelArgs& a, const hm_table_params* params,
+                                 PairLaunch cfg, hipStream_t s);
 
 // Cell-tiled form of the relative kernel: one workgroup per home cell stages the adjacent
 // cells' cell-slot relative positions in LDS, shifted into the home cell's frame, and reads

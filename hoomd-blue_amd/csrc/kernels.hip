@@ -900,6 +900,11 @@ template<class Eval, int TPP, bool VIRIAL, bool MT>
 static hipError_t pair_dispatch_shift(const PairKernelArgs& a, const ParamTable<Eval>& tab,
                                       dim3 grid, dim3 block, hipStream_t s)
 {
+    if constexpr (Eval::shift_modes == 1)
+    {
+        hipLaunchKernelGGL((pair_kernel<Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a, tab);
+        return hipGetLastError();
+    }
     switch (a.shift_mode)
     {
     case 0:
@@ -960,6 +965,18 @@ hipError_t launch_pair_lj(const PairKernelArgs& a, const hm_lj_params* params, P
     unsigned int nt = a.ntypes ? a.ntypes : 1;
     for (unsigned int q = 0; q < nt * nt; q++)
         tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+    return pair_dispatch<E>(a, tab, cfg, s);
+}
+
+hipError_t launch_pair_table(const PairKernelArgs& a, const hm_table_params* params,
+                             PairLaunch cfg, hipStream_t s)
+{
+    using E = EvaluatorPairTable<float>;
+    ParamTable<E> tab;
+    unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+        tab.p[q] = E::param_type::make(params[q].U, params[q].F, params[q].r_min,
+                                       params[q].width);
     return pair_dispatch<E>(a, tab, cfg, s);
 }
 
@@ -1167,6 +1184,12 @@ template<int B, class Eval, int TPP, bool VIRIAL, bool MT>
 static void pair_rel_dispatch_shift(const PairRelArgs& a, const RelTable<Eval>& tab, dim3 grid,
                                     dim3 block, hipStream_t s)
 {
+    if constexpr (Eval::shift_modes == 1)
+    {
+        hipLaunchKernelGGL((pair_rel_kernel<B, Eval, TPP, VIRIAL, MT, 0>), grid, block, 0, s, a,
+                           tab);
+        return;
+    }
     switch (a.shift_mode)
     {
     case 0:
@@ -1210,26 +1233,12 @@ static void pair_rel_dispatch_tpp(const PairRelArgs& a, const RelTable<Eval>& ta
         pair_rel_dispatch_batch<8, Eval, TPP>(a, tab, g, b, s);
 }
 
-hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* params,
-                              PairLaunch cfg, hipStream_t s)
+// Guard bands of the FP32 cutoff decisions, then the launch with the tuned configuration
+template<class E>
+static hipError_t pair_rel_launch(PairRelArgs a, const RelTable<E>& tab, PairLaunch cfg,
+                                  hipStream_t s)
 {
-    if (a_in.N == 0)
-        return hipSuccess;
-    using E = EvaluatorPairLJ<float>;
-    RelTable<E> tab;
-    PairRelArgs a = a_in;
     const unsigned int nt = a.ntypes ? a.ntypes : 1;
-    for (unsigned int q = 0; q < nt * nt; q++)
-    {
-        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
-        tab.f[q] = FastPair<E>::make(tab.p[q], a.rcutsq[q], a.shift_mode == 1);
-        // lj1 == 0: evalForceAndEnergy never contributes -- a zero cutoff excludes the pair
-        if ((double)tab.p[q].epsilon_x_4 * (double)tab.p[q].sigma_6 == 0.0)
-        {
-            a.rcutsq[q] = 0.0;
-            a.rcutsq_f[q] = 0.f;
-        }
-    }
     // worst-case FP32 error of r^2 near r_cut^2: |rel| <= R = 1.5 w per axis (cell width plus
     // the r_buff / 2 drift allowance, r_buff <= w), each rel rounded (u R), their difference
     // (2 u R), the cell offset w rounded (u w) and the final sum (u (2 R + w))
@@ -1266,6 +1275,54 @@ hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* param
         break;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_pair_rel_lj(const PairRelArgs& a_in, const hm_lj_params* params,
+                              PairLaunch cfg, hipStream_t s)
+{
+    if (a_in.N == 0)
+        return hipSuccess;
+    using E = EvaluatorPairLJ<float>;
+    RelTable<E> tab;
+    PairRelArgs a = a_in;
+    const unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        tab.p[q] = E::param_type::make(params[q].sigma, params[q].epsilon);
+        tab.f[q] = FastPair<E>::make(tab.p[q], a.rcutsq[q], a.shift_mode == 1);
+        // lj1 == 0: evalForceAndEnergy never contributes -- a zero cutoff excludes the pair
+        if ((double)tab.p[q].epsilon_x_4 * (double)tab.p[q].sigma_6 == 0.0)
+        {
+            a.rcutsq[q] = 0.0;
+            a.rcutsq_f[q] = 0.f;
+        }
+    }
+    return pair_rel_launch<E>(a, tab, cfg, s);
+}
+
+hipError_t launch_pair_rel_table(const PairRelArgs& a_in, const hm_table_params* params,
+                                 PairLaunch cfg, hipStream_t s)
+{
+    if (a_in.N == 0)
+        return hipSuccess;
+    using E = EvaluatorPairTable<float>;
+    RelTable<E> tab;
+    PairRelArgs a = a_in;
+    a.shift_mode = 0;
+    const unsigned int nt = a.ntypes ? a.ntypes : 1;
+    for (unsigned int q = 0; q < nt * nt; q++)
+    {
+        tab.p[q] = E::param_type::make(params[q].U, params[q].F, params[q].r_min,
+                                       params[q].width);
+        tab.f[q] = FastPair<E>::make(tab.p[q], a.rcutsq[q], 0);
+        // an empty table never contributes -- a zero cutoff excludes the pair
+        if (params[q].width == 0)
+        {
+            a.rcutsq[q] = 0.0;
+            a.rcutsq_f[q] = 0.f;
+        }
+    }
+    return pair_rel_launch<E>(a, tab, cfg, s);
 }
 
 // ======================================================================== pair forces, FP32 cell tiles

@@ -45,6 +45,10 @@ class HmLJParams(C.Structure):
     _fields_ = [("sigma", _d), ("epsilon", _d)]
 
 
+class HmTableParams(C.Structure):
+    _fields_ = [("r_min", _d), ("width", _u), ("U", _p), ("F", _p)]
+
+
 class HmPairArgs(C.Structure):
     _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("nlist", _p), ("nmax", _u),
                 ("n_neigh", _p), ("ntypes", _u), ("r_cut", _d * (MAX_TYPES * MAX_TYPES)),
@@ -91,6 +95,7 @@ SIGNATURES = {
     "hm_nlist_build": (_i, [C.POINTER(HmNlistArgs), _p]),
     "hm_nlist_check": (_i, [_u, _p, _p, HmBox, _d, _d, _p, _p]),
     "hm_pair_lj": (_i, [C.POINTER(HmPairArgs), _p, _p]),
+    "hm_pair_table": (_i, [C.POINTER(HmPairArgs), _p, _p]),
     "hm_pair_lj_cells": (_i, [C.POINTER(HmPairCellArgs), _p, _p]),
     "hm_bond_fene": (_i, [C.POINTER(HmBondArgs), _p, _p]),
     "hm_nvt_step_one": (_i, [_u, _p, _p, _p, _p, HmBox, _d, _d, _p]),
@@ -106,6 +111,7 @@ SIGNATURES = {
     "hm_sys_set_nlist": (_i, [_p, _d, _i]),
     "hm_sys_set_nlist_policy": (_i, [_p, _u, _i]),
     "hm_sys_add_pair_lj": (_i, [_p, _p, _p, _p, _i]),
+    "hm_sys_add_pair_table": (_i, [_p, _p, _p, _p, _p, _p]),
     "hm_sys_add_bond_fene": (_i, [_p, _u, _p, _p, _u, _p]),
     "hm_sys_set_method": (_i, [_p, _i, _d, _d, _p]),
     "hm_sys_set_sorter": (_i, [_p, _u, _u]),

@@ -1,4 +1,5 @@
-"""Pair forces (hoomd/md/pair/pair.py). ``LJ`` drives the EvaluatorPairLJ kernel."""
+"""Pair forces (hoomd/md/pair/pair.py). ``LJ`` and ``Table`` drive the pair kernels with
+the EvaluatorPairLJ and EvaluatorPairTable evaluators."""
 
 import numpy as np
 
@@ -68,6 +69,68 @@ class LJ(Pair):
         fid = _lib.lib().hm_sys_add_pair_lj(h, rcf.ctypes.data, rof.ctypes.data, P,
                                             self._modes[self.mode])
         _lib.check(fid, "pair.LJ")
+        self._force_id = fid
+        self._simulation = sim
+        self.nlist._simulation = sim
+
+
+class Table(Pair):
+    """hoomd.md.pair.Table(nlist, default_r_cut=None) (pair.py:520-618): params[(a, b)] =
+    dict(r_min=..., U=array, F=array), U and F (= -dU/dr) on the points
+    numpy.linspace(r_min, r_cut, len(U), endpoint=False), interpolated linearly
+    (EvaluatorPairTable.h:152-196). Mode 'none' only."""
+
+    _modes = {"none": 0}
+
+    def __init__(self, nlist, default_r_cut=None):
+        super().__init__(nlist, default_r_cut, 0.0, "none")
+        self.params = TypePairParameter()
+
+    @staticmethod
+    def _table(p):
+        missing = {"r_min", "U", "F"} - set(p)
+        if missing:
+            raise ValueError(f"Table params need {sorted(missing)}")
+        U = np.asarray(p["U"], np.float64).ravel()
+        F = np.asarray(p["F"], np.float64).ravel()
+        if U.size != F.size:
+            raise RuntimeError("The length of V and F arrays must be equal.")
+        if U.size == 0:
+            raise RuntimeError("The length of V and F must not be zero.")
+        return float(p["r_min"]), U, F
+
+    def _signature(self):
+        tabs = []
+        for k, v in sorted(self.params._d.items()):
+            r_min, U, F = self._table(v)
+            tabs.append((k, r_min, U.tobytes(), F.tobytes()))
+        return ("Table", tuple(tabs), self.r_cut.signature(), self.nlist._signature())
+
+    def _attach(self, sim, h):
+        types = sim.state.particle_types
+        nt = len(types)
+        rc, _ = self._matrices(types)
+        r_min = np.zeros(nt * nt)
+        width = np.zeros(nt * nt, np.uint32)
+        Us, Fs = [], []
+        for i, a in enumerate(types):
+            for j, b in enumerate(types):
+                p = self.params.get(a, b)
+                if p is None:
+                    raise RuntimeError(f"params for ({a}, {b}) are not set")
+                rm, U, F = self._table(p)
+                if rc[i, j] > 0 and not rm < rc[i, j]:
+                    raise ValueError(f"r_min must be below r_cut for ({a}, {b})")
+                r_min[i * nt + j] = rm
+                width[i * nt + j] = U.size
+                Us.append(U)
+                Fs.append(F)
+        U = np.ascontiguousarray(np.concatenate(Us))
+        F = np.ascontiguousarray(np.concatenate(Fs))
+        rcf = np.ascontiguousarray(rc.ravel())
+        fid = _lib.lib().hm_sys_add_pair_table(h, rcf.ctypes.data, r_min.ctypes.data,
+                                               width.ctypes.data, U.ctypes.data, F.ctypes.data)
+        _lib.check(fid, "pair.Table")
         self._force_id = fid
         self._simulation = sim
         self.nlist._simulation = sim

@@ -188,6 +188,23 @@ typedef struct hm_pair_args
  * :140-404) as instantiated by PotentialPairGPUKernel.cu.inc:19-21. */
 int hm_pair_lj(const hm_pair_args* args, const hm_lj_params* params, hm_stream_t stream);
 
+/* Tabulated pair parameters for one type pair, as EvaluatorPairTable::param_type
+ * (hoomd/md/EvaluatorPairTable.h:38-118): width points U[k], F[k] (F = -dU/dr) at
+ * r_min + k (r_cut - r_min) / width. U and F are DEVICE arrays of width floats. */
+typedef struct hm_table_params
+{
+    double r_min;
+    unsigned int width;
+    const float* U;
+    const float* F;
+} hm_table_params;
+
+/* Pair force with the Table evaluator: the same sums and outputs as hm_pair_lj, mode none
+ * (shift_mode must be 0; r_on is ignored). Replaces
+ * gpu_compute_pair_forces<EvaluatorPairTable> (hoomd/md/PotentialPairGPU.cuh:579-656) as
+ * instantiated by PotentialPairGPUKernel.cu.inc:19-21 for PotentialPairTable. */
+int hm_pair_table(const hm_pair_args* args, const hm_table_params* params, hm_stream_t stream);
+
 typedef struct hm_pair_cell_args
 {
     unsigned int N;
@@ -310,6 +327,12 @@ int hm_sys_set_nlist_policy(hm_system* s, unsigned int rebuild_check_delay, int 
  * r_cut <= 0 disables a pair. mode: 0 none, 1 shift, 2 xplor. Returns the force id (>= 0). */
 int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
                        const hm_lj_params* params, int mode);
+/* Tabulated pair force (md.pair.Table, mode none): r_cut per type pair; per type pair q
+ * (row-major, symmetric) r_min[q] and width[q] points of U and F, stored one pair after another
+ * in the host arrays U and F (pair q starts at the sum of the earlier widths). width 0 or
+ * r_cut <= 0 disables a pair. Returns the force id. */
+int hm_sys_add_pair_table(hm_system* s, const double* r_cut, const double* r_min,
+                          const unsigned int* width, const double* U, const double* F);
 /* FENE+WCA bonds (md.bond.FENEWCA): bonds by tag [2*nb], bond types [nb] (may be NULL),
  * params per bond type. Returns the force id. */
 int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,

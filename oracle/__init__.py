@@ -49,6 +49,7 @@ def lib():
             "orng_uneg11_u64": (_d, [C.c_uint64]),
             "oe_lj": (_i, [_d, _d, _d, _d, _i, _p, _p]),
             "oe_fene": (_i, [_d, _d, _d, _d, _d, _d, _p, _p]),
+            "oe_table": (_i, [_d, _d, _d, _u, _p, _p, _p, _p]),
             "oc_compute_dims": (None, [_p, _d, _p]),
             "oc_n_adj": (_u, [_p]),
             "oc_cell_adj": (None, [_p, _p]),
@@ -57,6 +58,8 @@ def lib():
             "on_build_brute": (_u, [_p, _sz, _p, _d, _i, _u, _p, _p]),
             "on_distance_check": (_i, [_p, _sz, _p, _p, _d, _d]),
             "op_pair_lj": (None, [_p, _sz, _p, _p, _p, _u, _i, _d, _d, _d, _d, _i, _i, _p, _p]),
+            "op_pair_table": (None, [_p, _sz, _p, _p, _p, _u, _i, _d, _d, _u, _p, _p, _i, _p,
+                                     _p]),
             "op_bond_fene": (_i, [_p, _sz, _p, _sz, _p, _d, _d, _d, _d, _d, _i, _p, _p]),
             "oi_nvt_step_one": (None, [_p, _sz, _p, _p, _p, _p, _d, _d]),
             "oi_nvt_step_two": (None, [_sz, _p, _p, _p, _p, _d, _d]),
@@ -193,6 +196,16 @@ def lj(rsq, rcutsq, sigma, epsilon, shift=False):
     return bool(ok), f.value, e.value
 
 
+def table(rsq, rcutsq, r_min, U, F):
+    """EvaluatorPairTable on one pair: (evaluated, force / r, energy)."""
+    U = _f64(U)
+    F = _f64(F)
+    f = C.c_double()
+    e = C.c_double()
+    ok = lib().oe_table(rsq, rcutsq, r_min, len(U), _ptr(U), _ptr(F), C.byref(f), C.byref(e))
+    return bool(ok), f.value, e.value
+
+
 def fene(rsq, k, r0, sigma, epsilon, delta):
     f = C.c_double()
     e = C.c_double()
@@ -311,6 +324,20 @@ def pair_lj(box, pos, nlist, n_neigh, third_law, sigma=1.0, epsilon=1.0, r_cut=2
     m = {"none": 0, "shift": 1, "xplor": 2}[mode]
     lib().op_pair_lj(box.ptr, N, _ptr(pos), _ptr(nl), _ptr(nn), nl.shape[1], int(third_law), sigma,
                      epsilon, r_cut, r_on, m, int(virial), _ptr(f), _ptr(v))
+    return f, v
+
+
+def pair_table(box, pos, nlist, n_neigh, third_law, r_cut, r_min, U, F, virial=True):
+    pos = _f64(pos)
+    N = len(pos)
+    nl = _u32(nlist)
+    nn = _u32(n_neigh)
+    U = _f64(U)
+    Fv = _f64(F)
+    f = np.zeros((N, 4))
+    v = np.zeros((6, N))
+    lib().op_pair_table(box.ptr, N, _ptr(pos), _ptr(nl), _ptr(nn), nl.shape[1], int(third_law),
+                        r_cut, r_min, len(U), _ptr(U), _ptr(Fv), int(virial), _ptr(f), _ptr(v))
     return f, v
 
 

@@ -270,6 +270,37 @@ int oe_lj(double rsq, double rcutsq, double sigma, double epsilon, int shift, do
     return 0;
 }
 
+int oe_table(double rsq, double rcutsq, double rmin, unsigned int width, const double* V,
+             const double* F, double* force_divr, double* energy)
+{
+    /* EvaluatorPairTable::evalForceAndEnergy (hoomd/md/EvaluatorPairTable.h:152-196): linear
+     * interpolation on width points r_min + k dr, dr = (r_cut - r_min) / width, the point past
+     * the last one taken as zero. The index is kept below width (the reference reads past the
+     * table when rounding puts r - r_min at width dr). */
+    double r = sqrt(rsq);
+    if (rsq >= rcutsq || r < rmin || width == 0)
+        return 0;
+    double rcut = sqrt(rcutsq);
+    double delta_r = (rcut - rmin) / (double)width;
+    double value_f = (r - rmin) / delta_r;
+    unsigned int value_i = (unsigned int)floor(value_f);
+    if (value_i >= width)
+        value_i = width - 1;
+    double V0 = V[value_i], F0 = F[value_i], V1 = 0.0, F1 = 0.0;
+    if (value_i + 1 < width)
+    {
+        V1 = V[value_i + 1];
+        F1 = F[value_i + 1];
+    }
+    double f = value_f - (double)value_i;
+    double Vr = V0 + f * (V1 - V0);
+    double Fr = F0 + f * (F1 - F0);
+    if (rsq > 0.0)
+        *force_divr = Fr / r;
+    *energy = Vr;
+    return 1;
+}
+
 int oe_fene(double rsq, double K, double r_0, double sigma, double epsilon, double delta,
             double* force_divr, double* bond_eng)
 {
@@ -591,14 +622,18 @@ int on_distance_check(const ob_box* b, size_t N, const double* pos3, const doubl
 
 /* =========================================================================== forces */
 
-void op_pair_lj(const ob_box* b, size_t N, const double* pos3, const unsigned int* nlist,
-                const unsigned int* n_neigh, unsigned int nmax, int third_law, double sigma,
-                double epsilon, double rcut, double ron, int shift_mode, int compute_virial,
-                double* force4, double* virial6)
+/* One pair evaluation: force / r and pair energy for rsq; 0 when the pair does not interact */
+typedef int (*pair_eval_fn)(double rsq, const void* ctx, double* force_divr, double* energy);
+
+/* PotentialPair<Evaluator>::computeForces (hoomd/md/PotentialPair.h:587-794) with the evaluator
+ * as a callback: per particle, force sum, half pair energy and half virial (PotentialPair.h:
+ * 739-772); with a half list (third_law) the partner gets the opposite force and the other
+ * halves. */
+static void pair_accumulate(const ob_box* b, size_t N, const double* pos3,
+                            const unsigned int* nlist, const unsigned int* n_neigh,
+                            unsigned int nmax, int third_law, int compute_virial,
+                            pair_eval_fn eval, const void* ctx, double* force4, double* virial6)
 {
-    /* PotentialPair<EvaluatorPairLJ>::computeForces (hoomd/md/PotentialPair.h:587-794) */
-    double rcutsq = rcut * rcut;
-    double ronsq = ron * ron;
     memset(force4, 0, sizeof(double) * 4 * N);
     if (virial6)
         memset(virial6, 0, sizeof(double) * 6 * N);
@@ -613,26 +648,9 @@ void op_pair_lj(const ob_box* b, size_t N, const double* pos3, const unsigned in
             double dx[3] = {pi[0] - pos3[3 * j], pi[1] - pos3[3 * j + 1], pi[2] - pos3[3 * j + 2]};
             ob_min_image(b, dx);
             double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-            int energy_shift = 0;
-            if (shift_mode == 1)
-                energy_shift = 1;
-            else if (shift_mode == 2 && ronsq > rcutsq)
-                energy_shift = 1;
             double force_divr = 0.0, pair_eng = 0.0;
-            int ev = oe_lj(rsq, rcutsq, sigma, epsilon, energy_shift, &force_divr, &pair_eng);
-            if (!ev)
+            if (!eval(rsq, ctx, &force_divr, &pair_eng))
                 continue;
-            if (shift_mode == 2 && rsq >= ronsq && rsq < rcutsq)
-            {
-                /* XPLOR smoothing (PotentialPair.h:706-731) */
-                double old_e = pair_eng, old_f = force_divr;
-                double den = 1.0 / ((rcutsq - ronsq) * (rcutsq - ronsq) * (rcutsq - ronsq));
-                double rmc = rsq - rcutsq;
-                double s = rmc * rmc * (rcutsq + 2.0 * rsq - 3.0 * ronsq) * den;
-                double ds = 12.0 * (rsq - ronsq) * rmc * den;
-                pair_eng = old_e * s;
-                force_divr = s * old_f - ds * old_e;
-            }
             double f2 = force_divr * 0.5;
             fi[0] += dx[0] * force_divr;
             fi[1] += dx[1] * force_divr;
@@ -678,6 +696,71 @@ void op_pair_lj(const ob_box* b, size_t N, const double* pos3, const unsigned in
             virial6[5 * N + i] += vzz;
         }
     }
+}
+
+typedef struct
+{
+    double sigma, epsilon, rcutsq, ronsq;
+    int shift_mode;
+} lj_ctx;
+
+static int lj_eval(double rsq, const void* c, double* force_divr, double* pair_eng)
+{
+    const lj_ctx* p = (const lj_ctx*)c;
+    int energy_shift = 0;
+    if (p->shift_mode == 1)
+        energy_shift = 1;
+    else if (p->shift_mode == 2 && p->ronsq > p->rcutsq)
+        energy_shift = 1;
+    if (!oe_lj(rsq, p->rcutsq, p->sigma, p->epsilon, energy_shift, force_divr, pair_eng))
+        return 0;
+    if (p->shift_mode == 2 && rsq >= p->ronsq && rsq < p->rcutsq)
+    {
+        /* XPLOR smoothing (PotentialPair.h:706-731) */
+        double old_e = *pair_eng, old_f = *force_divr;
+        double den = 1.0 / ((p->rcutsq - p->ronsq) * (p->rcutsq - p->ronsq) * (p->rcutsq - p->ronsq));
+        double rmc = rsq - p->rcutsq;
+        double s = rmc * rmc * (p->rcutsq + 2.0 * rsq - 3.0 * p->ronsq) * den;
+        double ds = 12.0 * (rsq - p->ronsq) * rmc * den;
+        *pair_eng = old_e * s;
+        *force_divr = s * old_f - ds * old_e;
+    }
+    return 1;
+}
+
+void op_pair_lj(const ob_box* b, size_t N, const double* pos3, const unsigned int* nlist,
+                const unsigned int* n_neigh, unsigned int nmax, int third_law, double sigma,
+                double epsilon, double rcut, double ron, int shift_mode, int compute_virial,
+                double* force4, double* virial6)
+{
+    /* PotentialPair<EvaluatorPairLJ>::computeForces (hoomd/md/PotentialPair.h:587-794) */
+    lj_ctx c = {sigma, epsilon, rcut * rcut, ron * ron, shift_mode};
+    pair_accumulate(b, N, pos3, nlist, n_neigh, nmax, third_law, compute_virial, lj_eval, &c,
+                    force4, virial6);
+}
+
+typedef struct
+{
+    double rcutsq, rmin;
+    unsigned int width;
+    const double *V, *F;
+} table_ctx;
+
+static int table_eval(double rsq, const void* c, double* force_divr, double* pair_eng)
+{
+    const table_ctx* p = (const table_ctx*)c;
+    return oe_table(rsq, p->rcutsq, p->rmin, p->width, p->V, p->F, force_divr, pair_eng);
+}
+
+void op_pair_table(const ob_box* b, size_t N, const double* pos3, const unsigned int* nlist,
+                   const unsigned int* n_neigh, unsigned int nmax, int third_law, double rcut,
+                   double rmin, unsigned int width, const double* V, const double* F,
+                   int compute_virial, double* force4, double* virial6)
+{
+    /* PotentialPair<EvaluatorPairTable>::computeForces (hoomd/md/PotentialPair.h:587-794) */
+    table_ctx c = {rcut * rcut, rmin, width, V, F};
+    pair_accumulate(b, N, pos3, nlist, n_neigh, nmax, third_law, compute_virial, table_eval, &c,
+                    force4, virial6);
 }
 
 int op_bond_fene(const ob_box* b, size_t N, const double* pos3, size_t nb, const unsigned int* bonds,

@@ -80,6 +80,9 @@ int oe_lj(double rsq, double rcutsq, double sigma, double epsilon, int shift, do
           double* energy);
 int oe_fene(double rsq, double k, double r0, double sigma, double epsilon, double delta,
             double* force_divr, double* energy);
+/* tabulated pair: width points V, F from r_min in steps of (r_cut - r_min) / width */
+int oe_table(double rsq, double rcutsq, double rmin, unsigned int width, const double* V,
+             const double* F, double* force_divr, double* energy);
 
 /* ----------------------------------------------------------------------------- cell list */
 void oc_compute_dims(const ob_box* b, double nominal_width, unsigned int dim[3]);
@@ -119,6 +122,13 @@ void op_pair_lj(const ob_box* b, size_t N, const double* pos3, const unsigned in
                 const unsigned int* n_neigh, unsigned int nmax, int third_law, double sigma,
                 double epsilon, double rcut, double ron, int shift_mode, int compute_virial,
                 double* force4, double* virial6);
+
+/* PotentialPair::computeForces with the Table evaluator for one particle type: V, F hold width
+ * points from r_min in steps of (r_cut - r_min) / width. Same outputs as op_pair_lj. */
+void op_pair_table(const ob_box* b, size_t N, const double* pos3, const unsigned int* nlist,
+                   const unsigned int* n_neigh, unsigned int nmax, int third_law, double rcut,
+                   double rmin, unsigned int width, const double* V, const double* F,
+                   int compute_virial, double* force4, double* virial6);
 
 /* PotentialBond::computeForces with the FENE+WCA evaluator. bonds[2*nb] are indices.
  * Returns 0, or -(b+1) if bond b is stretched beyond r0. */

@@ -8,6 +8,8 @@ Every fixture is data only: the inputs and expected outputs that the reference's
 hold. Sources:
   * hoomd/md/pytest/forces_and_energies.json  -- the "LJ" entry (Mathematica values at r = 0.75
     and 1.5 for three parameter sets; compared at rtol 5e-6 by test_potential.py:994-1051)
+  * hoomd/md/pytest/forces_and_energies.json  -- the "Table" entry (three tables at r = 0.75
+    and 1.5, r_cut 2.5)
   * hoomd/md/pytest/true_pair_list.json        -- 1,029 pairs of the 7^3 simple-cubic lattice,
     a = 1, r_cut 1.1, buffer 0 (test_nlist.py:221-231,353-372)
   * hoomd/md/pytest/test_bond.py:39-58          -- FENEWCA force/energy vectors at d = 0.969
@@ -35,6 +37,10 @@ def main():
         json.dump({"source": "hoomd/md/pytest/forces_and_energies.json[LJ]",
                    "distances": [0.75, 1.5], "r_cut": 2.5, "mode": "none",
                    "LJ": fe["LJ"]}, f, indent=1)
+    with open(os.path.join(HERE, "table_forces_and_energies.json"), "w") as f:
+        json.dump({"source": "hoomd/md/pytest/forces_and_energies.json[Table]",
+                   "distances": [0.75, 1.5], "r_cut": 2.5, "mode": "none",
+                   "Table": fe["Table"]}, f, indent=1)
 
     with open(os.path.join(pytest_dir, "true_pair_list.json")) as f:
         pairs = json.load(f)

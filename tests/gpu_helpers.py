@@ -221,3 +221,26 @@ class CellPipeline:
         H.check(H.lib().hm_synchronize())
         return (d_f.numpy().reshape(N, 4).astype(np.float64),
                 d_v.numpy().reshape(6, N).astype(np.float64))
+
+
+def pair_table(pos, L, nl, nn, r_cut, r_min, U, F, virial=True, tpp=4, block=256):
+    """hm_pair_table for one particle type; U and F uploaded as FP32 device tables."""
+    N = len(pos)
+    d_pos = dev(pos4(pos))
+    d_nl = dev(nl, np.uint32)
+    d_nn = dev(nn, np.uint32)
+    d_U = dev(np.asarray(U, np.float32), np.float32)
+    d_F = dev(np.asarray(F, np.float32), np.float32)
+    d_f = H.DeviceArray((N * 4,), np.float32, fill=0)
+    d_v = H.DeviceArray((N * 6,), np.float32, fill=0)
+    a = H.HmPairArgs()
+    a.N, a.pos, a.box = N, d_pos.ptr, H.box(L)
+    a.nlist, a.nmax, a.n_neigh, a.ntypes = d_nl.ptr, nl.shape[1], d_nn.ptr, 1
+    a.r_cut[0] = r_cut
+    a.shift_mode, a.compute_virial = 0, int(virial)
+    a.force, a.virial, a.tpp, a.block_size = d_f.ptr, d_v.ptr, tpp, block
+    P = (H.HmTableParams * 1)()
+    P[0].r_min, P[0].width, P[0].U, P[0].F = r_min, len(U), d_U.ptr, d_F.ptr
+    H.check(H.lib().hm_pair_table(C.byref(a), P, None), "pair table")
+    H.check(H.lib().hm_synchronize())
+    return d_f.numpy().reshape(N, 4).astype(np.float64), d_v.numpy().reshape(6, N).astype(np.float64)

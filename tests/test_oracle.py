@@ -96,6 +96,60 @@ def test_lj_shift_closed_form():
     assert not ok
 
 
+def test_table_golden(golden):
+    """forces_and_energies.json 'Table' (three tables, r = 0.75 and 1.5, r_cut 2.5) through the
+    pair accumulation of a two-particle system: the energies of the two particles sum to the
+    pair energy, particle 0 (below particle 1 on z) feels the reference's force along z."""
+    fx = golden("table_forces_and_energies.json")
+    t = fx["Table"]
+    box = O.Box(20.0)
+    for p, forces, energies in zip(t["params"], t["forces"], t["energies"]):
+        for i, d in enumerate(fx["distances"]):
+            pos = np.array([[0, 0, 0.1], [0, 0, d + 0.1]])
+            nl = np.array([[1], [0]], np.uint32)
+            nn = np.array([1, 1], np.uint32)
+            f, v = O.pair_table(box, pos, nl, nn, False, fx["r_cut"], p["r_min"], p["U"], p["F"])
+            assert math.isclose(f[:, 3].sum(), energies[i], rel_tol=5e-6, abs_tol=1e-6)
+            assert math.isclose(f[0, 2], forces[i], rel_tol=5e-6, abs_tol=1e-6)
+            assert abs(f[0, 0]) + abs(f[0, 1]) == 0 and f[1, 2] == -f[0, 2]
+            # the half list with the third law gives the same forces
+            f3, _ = O.pair_table(box, pos, np.array([[1], [0]], np.uint32),
+                                 np.array([1, 0], np.uint32), True, fx["r_cut"], p["r_min"],
+                                 p["U"], p["F"])
+            np.testing.assert_array_equal(f3, f)
+
+
+def test_table_interpolation_closed_forms():
+    """Linear interpolation on r_min + k dr, dr = (r_cut - r_min) / width; zero below r_min and
+    from r_cut on; the interval after the last point runs linearly to zero."""
+    rmin, rc, w = 0.8, 2.0, 6
+    dr = (rc - rmin) / w
+    U = np.array([5.0, 3.0, 2.0, 1.5, 1.0, 0.5])
+    F = np.array([9.0, 4.0, 2.0, 1.0, 0.5, 0.25])
+    for k in range(w):
+        r = rmin + k * dr
+        ok, fdr, e = O.table(r * r, rc * rc, rmin, U, F)
+        assert ok and math.isclose(e, U[k], rel_tol=1e-12) and math.isclose(fdr * r, F[k],
+                                                                              rel_tol=1e-12)
+        rm = r + 0.5 * dr
+        nxt = (U[k + 1], F[k + 1]) if k + 1 < w else (0.0, 0.0)
+        ok, fdr, e = O.table(rm * rm, rc * rc, rmin, U, F)
+        assert math.isclose(e, 0.5 * (U[k] + nxt[0]), rel_tol=1e-9)
+        assert math.isclose(fdr * rm, 0.5 * (F[k] + nxt[1]), rel_tol=1e-9)
+    assert not O.table((rmin - 1e-9) ** 2, rc * rc, rmin, U, F)[0]
+    assert not O.table(rc * rc, rc * rc, rmin, U, F)[0]
+    ok, fdr, e = O.table((rc - 1e-12) ** 2, rc * rc, rmin, U, F)
+    assert ok and abs(e) < 1e-9
+    # a fine table of LJ reproduces LJ to second order in dr
+    rs = np.linspace(0.9, 2.5, 4000, endpoint=False)
+    Ul = 4 * (rs**-12 - rs**-6)
+    Fl = 24 * (2 * rs**-13 - rs**-7)
+    for r in (0.95, 1.12, 1.5, 2.2):
+        ok, fdr, e = O.table(r * r, 6.25, 0.9, Ul, Fl)
+        ok2, fl, el = O.lj(r * r, 6.25, 1.0, 1.0)
+        assert abs(e - el) < 1e-4 * max(1.0, abs(el)) and abs(fdr - fl) < 1e-4 * max(1.0, abs(fl))
+
+
 def test_fene_golden(golden):
     fx = golden("fene_bond.json")
     d = fx["d"]
