@@ -37,12 +37,13 @@ class Operations:
 
     def add(self, op):
         from .md.compute import ThermodynamicQuantities
+        from .md.tune import NeighborListBuffer
         from .tune import ParticleSorter
         from .write import GSD
         if isinstance(op, GSD):
             self.writers.append(op)
             return
-        if isinstance(op, ParticleSorter):
+        if isinstance(op, (ParticleSorter, NeighborListBuffer)):
             self.tuners.append(op)
         elif isinstance(op, ThermodynamicQuantities):
             self.computes.append(op)
@@ -99,6 +100,7 @@ class Simulation:
         self._walltime = 0.0
         self._configured = False
         self._run_stats = None
+        self._run_id = 0
         self.operations = Operations(self)
 
     # -------------------------------------------------------------- properties
@@ -191,12 +193,25 @@ class Simulation:
         self._configured = False
         _lib.check(_lib.lib().hm_sys_set_timestep(h, self._timestep), "timestep")
 
+    def _op_key(self):
+        integ = self.operations.integrator
+        return (id(integ), integ._signature(),
+                tuple((type(t).__name__, t._signature()) for t in self.operations.tuners))
+
+    def _set_nlist_buffer(self, nl):
+        """A new buffer for the running system's neighbour list (NeighborList::setRBuff): the
+        engine rebuilds its cells and list at the next step."""
+        if self._system is None or not self._configured:
+            return
+        _lib.check(_lib.lib().hm_sys_set_nlist(self._system, nl.buffer,
+                                               int("bond" in nl.exclusions)), "nlist buffer")
+        self._attached_key = self._op_key()
+
     def _attach(self):
         integ = self.operations.integrator
         if integ is None:
             raise RuntimeError("Simulation.run needs an md.Integrator")
-        key = (id(integ), integ._signature(),
-               tuple((type(t).__name__, t._signature()) for t in self.operations.tuners))
+        key = self._op_key()
         for c in self.operations.computes:
             c._simulation = self
         if self._system is not None and self._configured and self._attached_key == key:
@@ -221,17 +236,20 @@ class Simulation:
     # -------------------------------------------------------------- run
     def run(self, steps, write_at_start=False):
         """Simulation.run (System::run, System.cc:80-181). The engine runs the steps between two
-        writer steps in one call; writers act after the step counter reaches their step, and on
-        the starting step too when write_at_start is set."""
+        operation steps in one call: Python tuners act before the step at their trigger's
+        steps, writers after the step counter reaches their step (and on the starting step too
+        when write_at_start is set)."""
         steps = int(steps)
         if steps < 0:
             raise ValueError("steps must be non-negative")
         self._attach()
         L = _lib.lib()
         writers = self.operations.writers
+        tuners = [t for t in self.operations.tuners if hasattr(t, "_act")]
         self._run_stats = None
+        self._run_id += 1
         t0 = time.perf_counter()
-        if not writers:
+        if not writers and not tuners:
             _lib.check(L.hm_sys_run(self._system, steps), "run")
         else:
             # the engine restarts its build statistics at each call: keep them over the run
@@ -242,7 +260,11 @@ class Simulation:
             if write_at_start:
                 self._write(t)
             while t < end:
+                for tu in tuners:  # tuners act before the step (System.cc:131-137)
+                    if tu.trigger(t):
+                        tu._act(self, t)
                 nxt = [w.trigger.next_after(t) for w in writers]
+                nxt += [tu.trigger.next_after(t) for tu in tuners]
                 stop = min([s for s in nxt if s is not None] + [end])
                 _lib.check(L.hm_sys_run(self._system, stop - t), "run")
                 _lib.check(L.hm_sys_stats(self._system, st.ctypes.data), "stats")

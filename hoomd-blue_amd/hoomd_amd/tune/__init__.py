@@ -1,6 +1,10 @@
-"""hoomd.tune subset: ParticleSorter (hoomd/tune/sorter.py)."""
+"""hoomd.tune subset: ParticleSorter (hoomd/tune/sorter.py), GridOptimizer and
+ManualTuneDefinition (hoomd/tune/solve.py, attr_tuner.py)."""
 
 import math
+
+from ..trigger import Periodic  # noqa: F401
+from .solve import GridOptimizer, ManualTuneDefinition, Optimizer, SolverStep  # noqa: F401
 
 
 class ParticleSorter:
@@ -39,10 +43,5 @@ class ParticleSorter:
     def _signature(self):
         return (self._period(), self._grid())
 
-
-class Periodic:
-    """hoomd.trigger.Periodic(period, phase=0)."""
-
-    def __init__(self, period, phase=0):
-        self.period = int(period)
-        self.phase = int(phase)
+    def __call__(self, timestep):
+        return False  # the engine sorts natively on its own period

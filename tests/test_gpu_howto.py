@@ -110,3 +110,32 @@ def test_gsd_writer_split_run(hoomd, tmp_path):
     np.testing.assert_allclose(last.particles.position, sc.particles.position.astype(np.float32),
                                atol=2e-6)
     np.testing.assert_array_equal(last.particles.image, sc.particles.image)
+
+
+def test_nlist_buffer_tuner_grid(hoomd):
+    """NeighborListBuffer.with_grid (md/tune/nlist_buffer.py): acts every 100 steps; the first
+    act of the run starts the clock, the second bins the domain, five measured bins, then the
+    best bin's centre, solved twice -> tuned. Forces stay exact through the buffer changes."""
+    sim, lj = howto_sim(hoomd, (2, 2, 2), seed=2)
+    tuner = hoomd.md.tune.NeighborListBuffer.with_grid(
+        trigger=100, nlist=lj.nlist, maximum_buffer=0.45, minimum_buffer=0.05, n_bins=5)
+    sim.operations.add(tuner)
+    sim.run(900)
+    assert tuner.tuned
+    lo, hi = tuner.final_bin
+    centres = [0.09, 0.17, 0.25, 0.33, 0.41]
+    assert any(abs((lo + hi) / 2 - c) < 1e-12 for c in centres)
+    assert abs(lj.nlist.buffer - (lo + hi) / 2) < 1e-12
+    assert tuner.max_tps > 0 and 0.05 <= tuner.best_buffer_size <= 0.45
+    assert sim.timestep == 900
+    snap = sim.state.get_snapshot()
+    L = np.array(snap.configuration.box[:3])
+    pos = snap.particles.position
+    box = O.Box(*L)
+    nl, nn = O.nlist_binned(box, pos, WCA, 0.0, full=True)
+    rf, _ = O.pair_lj(box, pos, nl, nn, third_law=False, r_cut=WCA)
+    assert rel_rms(lj.forces, rf[:, :3]) < 1e-5
+    # a later run keeps the tuned buffer without rebuilding the system
+    h = sim._system
+    sim.run(50)
+    assert sim._system == h and abs(lj.nlist.buffer - (lo + hi) / 2) < 1e-12

@@ -181,3 +181,35 @@ def test_periodic_trigger():
             assert tr.next_after(t) == min(f for f in range(t + 1, 200) if tr(f))
     assert On(10).next_after(3) == 10 and On(10).next_after(10) is None
     assert isinstance(as_trigger(100), Periodic) and as_trigger(100).period == 100
+
+
+def test_grid_optimizer_closed_form():
+    """GridOptimizer (solve.py:511-614) maximising -(x - 0.31)^2 on [0, 1]: 5 bins per round,
+    two rounds, the bin centres measured in order; the final bin holds the maximum."""
+    from hoomd_amd.tune import GridOptimizer, ManualTuneDefinition
+    state = {"x": 0.5}
+    seen = []
+    tun = ManualTuneDefinition(get_y=lambda: -(state["x"] - 0.31) ** 2, target=0.0,
+                               get_x=lambda: state["x"],
+                               set_x=lambda v: state.__setitem__("x", v), domain=(0.0, 1.0))
+    opt = GridOptimizer(n_bins=5, n_rounds=2)
+    done = False
+    for _ in range(20):
+        done = opt.solve([tun])
+        seen.append(state["x"])
+        if done:
+            break
+    assert done
+    np.testing.assert_allclose(seen[:5], [0.1, 0.3, 0.5, 0.7, 0.9])
+    lo, hi = opt._final_bins[tun]
+    assert lo <= 0.31 <= hi and abs((hi - lo) - 0.04) < 1e-12
+    assert state["x"] == (lo + hi) / 2
+    assert opt.solve([tun])  # stays solved
+    # a y of None is skipped and reported as not solved
+    opt2 = GridOptimizer()
+    none = ManualTuneDefinition(get_y=lambda: None, target=0.0, get_x=lambda: 0.0,
+                                set_x=lambda v: None, domain=(0.0, 1.0))
+    assert not opt2.solve([none]) and none not in opt2._bins
+    # x is clamped into the domain
+    tun.x = 5.0
+    assert state["x"] == 1.0
