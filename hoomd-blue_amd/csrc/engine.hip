@@ -311,7 +311,7 @@ struct hm_system
         {
             double expect = rho * 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
             nmax = (unsigned int)std::ceil(expect * 1.3 + 16.0);
-            nmax = (nmax + 7) & ~7u;
+            nmax = (nmax + ROW_BLOCK - 1) & ~(ROW_BLOCK - 1);
             alloc_rows();
         }
         if (!pcell)
@@ -373,13 +373,14 @@ struct hm_system
 
     void grow_nmax(unsigned int need)
     {
-        // Nmax rounds up to a multiple of 8 (the reference uses 4, NeighborList.cc:1601-1617,
-        // the pair kernel reads rows 8 entries at a time); extra headroom avoids
-        // repeated replays when the maximum drifts upward by one or two.
+        // Nmax rounds up to a multiple of ROW_BLOCK (the reference uses 4,
+        // NeighborList.cc:1601-1617; rows with offset codes are stored in transposed blocks of
+        // ROW_BLOCK entries); extra headroom avoids repeated replays when the maximum drifts
+        // upward by one or two.
         unsigned int m = (unsigned int)std::ceil(need * 1.15 + 8.0);
-        m = (m + 7) & ~7u;
+        m = (m + ROW_BLOCK - 1) & ~(ROW_BLOCK - 1);
         if (m <= nmax)
-            m = nmax + 8;
+            m = nmax + ROW_BLOCK;
         nmax = m;
         alloc_rows();
         clear_graphs();
@@ -1717,7 +1718,8 @@ long long hm_sys_get_pairs(hm_system* s, unsigned int* pairs, long long max_pair
         for (unsigned int i = 0; i < N; i++)
             for (unsigned int k = 0; k < nn[i] && k < s->nmax; k++)
             {
-                unsigned int j = nl[(size_t)i * s->nmax + k] & jmask;
+                // slot rows are stored in transposed blocks (row_store_pos)
+                unsigned int j = nl[(size_t)i * s->nmax + (slots ? row_store_pos(k) : k)] & jmask;
                 if (slots)
                     j = part[j];
                 unsigned int a = tg[i], b = tg[j];

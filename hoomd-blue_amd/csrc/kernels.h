@@ -87,6 +87,21 @@ constexpr unsigned int OFFSET_CODE_MAX_N = 1u << OFFSET_CODE_SHIFT;
 // entries allocated after the last neighbour row, so batched row reads need no clamp
 constexpr unsigned int NLIST_PAD = 256;
 
+// Rows with offset codes are stored in blocks of ROW_BLOCK = 32 entries, transposed so that the
+// four entries of one 16-byte load are neighbours r, r + 8, r + 16, r + 24 of the block: a lane
+// group of 8 reads 32 consecutive entries with one load each, and its q-th entries are 8
+// consecutive neighbours. Neighbour p (candidate order) is stored at position row_store_pos(p);
+// stored position P holds neighbour row_neighbour(P). Row strides are multiples of ROW_BLOCK.
+constexpr unsigned int ROW_BLOCK = 32;
+__host__ __device__ inline unsigned int row_store_pos(unsigned int p)
+{
+    return (p & ~31u) | ((p & 7u) << 2) | ((p >> 3) & 3u);
+}
+__host__ __device__ inline unsigned int row_neighbour(unsigned int P)
+{
+    return (P & ~31u) | ((P >> 2) & 7u) | ((P & 3u) << 3);
+}
+
 __host__ __device__ inline unsigned int offset_code(int ox, int oy, int oz)
 {
     return (((unsigned int)ox & 3u) | (((unsigned int)oy & 3u) << 2)

@@ -563,8 +563,13 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                             {
                                 // rows are streamed out: keep them from evicting the candidates
                                 if (!a.skip_index_rows)
-                                    __builtin_nontemporal_store(a.offset_codes ? S.sslot[k] : j,
-                                                                row + p);
+                                {
+                                    if (a.offset_codes)
+                                        __builtin_nontemporal_store(S.sslot[k],
+                                                                    row + row_store_pos(p));
+                                    else
+                                        __builtin_nontemporal_store(j, row + p);
+                                }
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
                             }
@@ -606,7 +611,12 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         if (p < a.nmax)
                         {
                             if (!a.skip_index_rows)
-                                row[p] = a.offset_codes ? S.sslot[k] : j;
+                            {
+                                if (a.offset_codes)
+                                    row[row_store_pos(p)] = S.sslot[k];
+                                else
+                                    row[p] = j;
+                            }
                             if (lrow)
                                 lrow[p] = (unsigned short)(base_k + k);
                         }
@@ -1077,15 +1087,27 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
     constexpr unsigned int S = TPP * B;
 
     PairAcc acc;
-    // rows are followed by at least TPP * B readable entries (the list is padded), so a batch
-    // reads its entries unclamped; entries past n are stale but valid slots and are masked
-    const unsigned int* row = a.nlist + (size_t)ii * a.nmax + sub;
-    for (unsigned int k0 = 0; k0 < n; k0 += S)
+    // Rows are stored in transposed blocks of 32 (row_store_pos): lane sub reads B entries as
+    // B / 4 16-byte loads at stored positions p0 + 4 (h TPP + sub); entry q of the lane holds
+    // neighbour row_neighbour(p0 + 4 ((q / 4) TPP + sub) + q % 4), so for a fixed q the lanes
+    // of a particle hold consecutive neighbours. Whole blocks are read (a block's neighbours
+    // are spread over all its positions); rows are followed by NLIST_PAD readable entries, and
+    // entries of neighbours >= n are stale but valid slots and are masked.
+    static_assert(B % 4 == 0 && (S % ROW_BLOCK == 0 || ROW_BLOCK % S == 0), "row blocks");
+    const uint4* row4 = (const uint4*)(a.nlist + (size_t)ii * a.nmax);
+    const unsigned int nblk = (n + ROW_BLOCK - 1) & ~(ROW_BLOCK - 1);
+    for (unsigned int k0 = 0; k0 < nblk; k0 += S)
     {
         unsigned int ec[B];
 #pragma unroll
-        for (int q = 0; q < B; q++)
-            ec[q] = row[k0 + q * TPP];
+        for (int h = 0; h < B / 4; h++)
+        {
+            const uint4 v = row4[(k0 >> 2) + h * TPP + sub];
+            ec[4 * h + 0] = v.x;
+            ec[4 * h + 1] = v.y;
+            ec[4 * h + 2] = v.z;
+            ec[4 * h + 3] = v.w;
+        }
         pin_loads<B>(ec);
         unsigned int js[B];
         float4 rj[B];
@@ -1109,7 +1131,8 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
             dz[q] = (ri.z - rj[q].z) + sh.z;
             rsq[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
             tp[q] = MT ? ti * a.ntypes + (__float_as_uint(rj[q].w) >> 24) : 0u;
-            const bool valid = k0 + q * TPP + sub < n;
+            const bool valid =
+                row_neighbour(k0 + 4u * ((unsigned int)(q / 4) * TPP + sub) + (q % 4)) < n;
             const float t = rsq[q] - a.rcutsq_f[tp[q]];
             in[q] = valid && t < 0.f;
             chk |= (valid && fabsf(t) <= a.guard[tp[q]]) ? (1u << q) : 0u;
