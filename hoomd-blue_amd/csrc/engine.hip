@@ -121,7 +121,8 @@ struct hm_system
     unsigned short* lrow = nullptr;
     // pair kernel kind 2: cell-slot relative FP32 positions [ncell * cmax], packed cell and
     // slot of every particle [N] (RelFrame)
-    float4* rel = nullptr;
+    float3* rel = nullptr;
+    uint32_t* relw = nullptr;
     uint32_t *pcell = nullptr, *slot_of = nullptr;
     double rcut_mat[HM_MAX_TYPES * HM_MAX_TYPES] = {};
     double rcut_max = 0.0;
@@ -212,6 +213,7 @@ struct hm_system
         dfree(cell_idx);
         dfree(lrow);
         dfree(rel);
+        dfree(relw);
         dfree(pcell);
         dfree(slot_of);
         dfree(n_excl_tag);
@@ -346,11 +348,14 @@ struct hm_system
         dfree(entries);
         dfree(cell_idx);
         dfree(rel);
+        dfree(relw);
         entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
         cell_idx = dalloc<unsigned int>((size_t)ncell * cmax);
-        rel = dalloc<float4>((size_t)ncell * cmax);
+        rel = dalloc<float3>((size_t)ncell * cmax);
+        relw = dalloc<uint32_t>((size_t)ncell * cmax);
         hchk(hipMemsetAsync(cell_idx, 0, sizeof(unsigned int) * ncell * cmax, stream), "clear");
-        hchk(hipMemsetAsync(rel, 0, sizeof(float4) * ncell * cmax, stream), "clear");
+        hchk(hipMemsetAsync(rel, 0, sizeof(float3) * ncell * cmax, stream), "clear");
+        hchk(hipMemsetAsync(relw, 0, sizeof(uint32_t) * ncell * cmax, stream), "clear");
         if (slot_of)  // slots of the old table may exceed the new one until the next build
             hchk(hipMemsetAsync(slot_of, 0, sizeof(uint32_t) * N, stream), "clear");
     }
@@ -508,7 +513,7 @@ struct hm_system
 
     RelFrame rel_frame() const
     {
-        return make_rel_frame(uses_rel() ? rel : nullptr, pcell, slot_of, box, dim);
+        return make_rel_frame(uses_rel() ? rel : nullptr, relw, pcell, slot_of, box, dim);
     }
 
     // ------------------------------------------------------------------ kernels of one step
@@ -522,6 +527,7 @@ struct hm_system
             k.N = N;
             k.pos = pos;
             k.rel = rel;
+            k.relw = relw;
             k.cell_idx = cell_idx;
             k.cell_size = cell_size;
             k.cell_adj = cell_adj;
@@ -560,6 +566,7 @@ struct hm_system
             k.N = N;
             k.pos = pos;
             k.rel = rel;
+            k.relw = relw;
             k.slot_of = slot_of;
             k.part_of = cell_idx;
             k.nslot = ncell * cmax;
@@ -706,10 +713,20 @@ struct hm_system
             a1.dist_check = dist_check;
             a1.rel = rel_frame();
         }
+        // the gather pair kernels read no cell list between builds: step one clears the
+        // occupancy counters every step, which saves a launch; the cell-tiled kernels keep them
+        const int kind = pair_kind();
+        const bool fused_clear = have_nl && (kind == 1 || kind == 2);
+        if (fused_clear)
+        {
+            a1.cell_size = cell_size;
+            a1.ncell = ncell;
+        }
         hchk(launch_step_one(a1, stream), "step one");
         if (have_nl)
         {
-            hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
+            if (!fused_clear)
+                hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
             hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
                                    &cond->cell_need, st, stream),
                  "cell build");

@@ -63,15 +63,17 @@ struct PairLaunch
 
 // Cell-slot relative positions. At every cell-list build the cell sort gives particle p the
 // slot c * cmax + k (cell c, k-th particle of the cell in index order) and writes
-//   rel[slot].xyz = position - origin of cell c (minimum image, FP32),
-//   rel[slot].w   = bits of pcell[p] = cx | cy << 8 | cz << 16 | type << 24,
-//   slot_of[p]    = slot.
+//   rel[slot]   = position - origin of cell c (minimum image, FP32; packed 12-byte records, so a
+//                 cell's particles share as few cache lines as possible),
+//   relw[slot]  = pcell[p] = cx | cy << 8 | cz << 16 | type << 24,
+//   slot_of[p]  = slot.
 // Step one refreshes rel[slot_of[p]] from the new position every step; the cell (and slot) of a
 // particle stay those of the last build. A neighbour cell's particles are therefore contiguous
 // in rel. Needs 3 <= dim <= 255 per axis and ncell * cmax <= OFFSET_CODE_MAX_N.
 struct RelFrame
 {
-    float4* rel;              // [ncell * cmax], slot order
+    float3* rel;              // [ncell * cmax], slot order
+    uint32_t* relw;           // [ncell * cmax], slot order
     const uint32_t* pcell;    // [N]
     const uint32_t* slot_of;  // [N]
     unsigned int dim[3];
@@ -120,11 +122,13 @@ __host__ __device__ inline uint32_t pack_cell(unsigned int cx, unsigned int cy, 
     return cx | (cy << 8) | (cz << 16) | (type << 24);
 }
 
-inline RelFrame make_rel_frame(float4* rel, const uint32_t* pcell, const uint32_t* slot_of,
-                               const DevBox& box, const unsigned int dim[3])
+inline RelFrame make_rel_frame(float3* rel, uint32_t* relw, const uint32_t* pcell,
+                               const uint32_t* slot_of, const DevBox& box,
+                               const unsigned int dim[3])
 {
     RelFrame f;
     f.rel = rel;
+    f.relw = relw;
     f.pcell = pcell;
     f.slot_of = slot_of;
     for (int d = 0; d < 3; d++)
@@ -252,7 +256,8 @@ struct PairRelArgs
 {
     unsigned int N;
     const double4* pos;      // FP64 positions for the guard-band re-decision of the cutoff
-    const float4* rel;       // slot order (RelFrame)
+    const float3* rel;       // slot order (RelFrame)
+    const uint32_t* relw;    // slot order (RelFrame): packed cell and type
     const uint32_t* slot_of; // [N]
     const uint32_t* part_of; // [ncell * cmax]: particle of each slot (the cell slot table)
     unsigned int nslot;      // ncell * cmax
@@ -286,7 +291,8 @@ struct PairTileArgs
 {
     unsigned int N;
     const double4* pos;       // FP64 positions for the guard-band re-decision of the cutoff
-    const float4* rel;        // slot order (RelFrame)
+    const float3* rel;        // slot order (RelFrame)
+    const uint32_t* relw;     // slot order (RelFrame): packed cell and type
     const uint32_t* cell_idx; // [ncell * cmax] particle of each slot
     const unsigned int* cell_size;
     const unsigned int* cell_adj;

@@ -264,8 +264,10 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
                                                         const StepState* gate, DevBox box,
                                                         RelFrame rf)
 {
+    // one wavefront per cell; a lane keeps its (at most NLHOME / 64) entries in registers, so
+    // the workgroup's LDS is only the index table
+    constexpr int EPL = NLHOME / 64;
     __shared__ unsigned int sidx[4][NLHOME];
-    __shared__ hm_cell_entry sent[4][NLHOME];
     if (gate_closed(gate))
         return;
     const unsigned int w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -273,22 +275,30 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
     for (unsigned int r = 0; r < rounds; r++)
     {
         const unsigned int c = (r * gridDim.x + blockIdx.x) * 4 + w;
-        const unsigned int n = c < ncell ? min(cell_size[c], cmax) : 0u;
-        for (unsigned int k = lane; k < n; k += 64)
+        const unsigned int n = c < ncell ? min(min(cell_size[c], cmax), (unsigned int)NLHOME) : 0u;
+        hm_cell_entry e[EPL];
+#pragma unroll
+        for (int u = 0; u < EPL; u++)
         {
-            hm_cell_entry e = entries[(size_t)c * cmax + k];
-            sent[w][k] = e;
-            sidx[w][k] = e.idx;
+            const unsigned int k = lane + 64u * u;
+            if (k < n)
+            {
+                e[u] = entries[(size_t)c * cmax + k];
+                sidx[w][k] = e[u].idx;
+            }
         }
         __syncthreads();
-        for (unsigned int k = lane; k < n; k += 64)
+#pragma unroll
+        for (int u = 0; u < EPL; u++)
         {
+            const unsigned int k = lane + 64u * u;
+            if (k >= n)
+                continue;
             const unsigned int my = sidx[w][k];
             unsigned int rank = 0;
             for (unsigned int m = 0; m < n; m++)
                 rank += sidx[w][m] < my ? 1u : 0u;
-            const hm_cell_entry e = sent[w][k];
-            entries[(size_t)c * cmax + rank] = e;
+            entries[(size_t)c * cmax + rank] = e[u];
             if (cell_idx)
                 cell_idx[(size_t)c * cmax + rank] = my;
             if (rf.rel)
@@ -296,8 +306,10 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
                 // cell-slot relative position, packed cell and slot of particle my
                 const unsigned int slot = c * cmax + rank;
                 const uint32_t pc = pack_cell(c % rf.dim[0], (c / rf.dim[0]) % rf.dim[1],
-                                              c / (rf.dim[0] * rf.dim[1]), e.type);
-                rf.rel[slot] = rel_of(e.x, e.y, e.z, pc, box, rf);
+                                              c / (rf.dim[0] * rf.dim[1]), e[u].type);
+                const float4 rr = rel_of(e[u].x, e[u].y, e[u].z, pc, box, rf);
+                rf.rel[slot] = make_float3(rr.x, rr.y, rr.z);
+                rf.relw[slot] = pc;
                 const_cast<uint32_t*>(rf.pcell)[my] = pc;
                 const_cast<uint32_t*>(rf.slot_of)[my] = slot;
             }
@@ -1080,9 +1092,9 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
     const unsigned int ii = active ? i : a.N - 1;
     const unsigned int smax = a.nslot - 1;
     const unsigned int si = min(a.slot_of[ii], smax);
-    const float4 ri = a.rel[si];
+    const float3 ri = a.rel[si];
     const unsigned int n = active ? min(a.n_neigh[ii], a.nmax) : 0u;
-    const unsigned int ti = MT ? (__float_as_uint(ri.w) >> 24) : 0u;
+    const unsigned int ti = MT ? (a.relw[si] >> 24) : 0u;
     const unsigned int jmax = a.N - 1;
     constexpr unsigned int S = TPP * B;
 
@@ -1096,6 +1108,13 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
     static_assert(B % 4 == 0 && (S % ROW_BLOCK == 0 || ROW_BLOCK % S == 0), "row blocks");
     const uint4* row4 = (const uint4*)(a.nlist + (size_t)ii * a.nmax);
     const unsigned int nblk = (n + ROW_BLOCK - 1) & ~(ROW_BLOCK - 1);
+    // with whole blocks per iteration (k0 a multiple of 32) entry q holds neighbour
+    // k0 + row_neighbour(offset of q), so its validity is k0 < n - row_neighbour(offset of q)
+    constexpr bool WHOLE = S % ROW_BLOCK == 0;
+    int lim[B];
+#pragma unroll
+    for (int q = 0; q < B; q++)
+        lim[q] = (int)n - (int)row_neighbour(4u * ((unsigned int)(q / 4) * TPP + sub) + (q % 4));
     for (unsigned int k0 = 0; k0 < nblk; k0 += S)
     {
         unsigned int ec[B];
@@ -1110,12 +1129,14 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
         }
         pin_loads<B>(ec);
         unsigned int js[B];
-        float4 rj[B];
+        float3 rj[B];
+        unsigned int tj[B];
 #pragma unroll
         for (int q = 0; q < B; q++)
         {
             js[q] = min(ec[q] & OFFSET_CODE_MASK, smax);
             rj[q] = a.rel[js[q]];
+            tj[q] = MT ? (a.relw[js[q]] >> 24) : 0u;
         }
         // phase 1: pair vectors, FP32 cutoff decisions and the guard-band flags of the batch
         float dx[B], dy[B], dz[B], rsq[B];
@@ -1130,9 +1151,10 @@ __global__ __launch_bounds__(1024) void pair_rel_kernel(PairRelArgs a, RelTable<
             dy[q] = (ri.y - rj[q].y) + sh.y;
             dz[q] = (ri.z - rj[q].z) + sh.z;
             rsq[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
-            tp[q] = MT ? ti * a.ntypes + (__float_as_uint(rj[q].w) >> 24) : 0u;
+            tp[q] = MT ? ti * a.ntypes + tj[q] : 0u;
             const bool valid =
-                row_neighbour(k0 + 4u * ((unsigned int)(q / 4) * TPP + sub) + (q % 4)) < n;
+                WHOLE ? (int)k0 < lim[q]
+                      : row_neighbour(k0 + 4u * ((unsigned int)(q / 4) * TPP + sub) + (q % 4)) < n;
             const float t = rsq[q] - a.rcutsq_f[tp[q]];
             in[q] = valid && t < 0.f;
             chk |= (valid && fabsf(t) <= a.guard[tp[q]]) ? (1u << q) : 0u;
@@ -1380,9 +1402,9 @@ __device__ __forceinline__ void tile_cand(const PairTileArgs& a, const TileShare
     }
     const unsigned int q = find_segment(S.aoff, 0, a.n_adj, l);
     const unsigned int slot = S.acell[q] * a.cmax + (l - S.aoff[q]);
-    const float4 rr = a.rel[slot];
+    const float3 rr = a.rel[slot];
     const float4 sh = S.ash[q];
-    r = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z, rr.w);
+    r = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z, __uint_as_float(a.relw[slot]));
     j = a.cell_idx[slot];
 }
 
@@ -1600,9 +1622,10 @@ __global__ __launch_bounds__(TBLOCK) void pair_tile_kernel(PairTileArgs a, RelTa
         {
             const unsigned int q = find_segment(S.aoff, 0, a.n_adj, t);
             const unsigned int slot = S.acell[q] * a.cmax + (t - S.aoff[q]);
-            const float4 rr = a.rel[slot];
+            const float3 rr = a.rel[slot];
             const float4 sh = S.ash[q];
-            S.sp[t] = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z, rr.w);
+            S.sp[t] = make_float4(rr.x + sh.x, rr.y + sh.y, rr.z + sh.z,
+                                  __uint_as_float(a.relw[slot]));
             S.sidx[t] = a.cell_idx[slot];
         }
         __syncthreads();
@@ -2141,7 +2164,10 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
         a.pos[n] = p;
         a.image[n] = img;
         if (a.rel.rel)
-            a.rel.rel[a.rel.slot_of[n]] = rel_of(px, py, pz, a.rel.pcell[n], a.box, a.rel);
+        {
+            const float4 r = rel_of(px, py, pz, a.rel.pcell[n], a.box, a.rel);
+            a.rel.rel[a.rel.slot_of[n]] = make_float3(r.x, r.y, r.z);
+        }
         if (a.last)
         {
             const d3 l = a.last[n];
