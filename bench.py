@@ -72,22 +72,33 @@ PAIR_KERNELS = {
 }
 
 
-def pair_roofline(sim, lj, N, kind):
-    """Average pair-kernel launch duration (HIP events on the engine stream) and the
-    algorithmic bytes of one launch: each neighbour-list entry once plus, per particle, its
-    n_neigh, its own position and its force (PAIR_KERNELS); every neighbour position gathered
-    is some particle's own position, so it is counted once."""
+def pair_roofline(sim, lj, N, kind, period=500, samples=10, launches=20):
+    """Average pair-kernel launch duration (HIP events on the engine stream around back-to-back
+    launches) and the algorithmic bytes of one launch: each neighbour-list entry once plus, per
+    particle, its n_neigh, its own position and its force (PAIR_KERNELS); every neighbour
+    position gathered is some particle's own position, so it is counted once.
+
+    The launch time depends on how far particles have drifted from their sorted order, so it is
+    sampled at `samples` points spread over one sort period of steps (as the timed steps are),
+    `launches` launches each; the entry count is averaged the same way."""
     import ctypes as C
 
     from hoomd_amd import _lib
     ms = C.c_double()
     ent = C.c_uint64()
-    _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, 200, C.byref(ms),
-                                           C.byref(ent)), "time pair")
+    tot_ms, tot_ent = 0.0, 0
+    for _ in range(samples):
+        sim.run(period // samples)
+        _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, launches, C.byref(ms),
+                                               C.byref(ent)), "time pair")
+        tot_ms += ms.value
+        tot_ent += ent.value
+    ms_avg = tot_ms / samples
+    ent_avg = tot_ent / samples
     _, per_entry, per_particle = PAIR_KERNELS[kind]
-    B = float(per_entry) * ent.value + N * per_particle
-    achieved = B / (ms.value * 1e-3) / 1e9
-    return ms.value, ent.value, B, achieved
+    B = float(per_entry) * ent_avg + N * per_particle
+    achieved = B / (ms_avg * 1e-3) / 1e9
+    return ms_avg, ent_avg, B, achieved
 
 
 def load_traffic(N):
