@@ -230,6 +230,13 @@ __device__ __forceinline__ unsigned int lane_id()
 }
 
 // number of set bits of mask below this lane
+// Wave64 ballot of a lane predicate, taken straight from the compare mask (HIP's __ballot goes
+// through an int and costs two extra VALU instructions per call)
+__device__ __forceinline__ uint64_t wave_ballot(bool p)
+{
+    return __builtin_amdgcn_ballot_w64(p);
+}
+
 __device__ __forceinline__ unsigned int lanes_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),

@@ -150,13 +150,13 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
         // active lanes (Hilbert-ordered particles put a wave into a few cells)
         unsigned int slot = 0;
         bool todo = ok;
-        uint64_t pend = __ballot(todo);
+        uint64_t pend = wave_ballot(todo);
         while (pend)
         {
             const int leader = __ffsll((unsigned long long)pend) - 1;
             const unsigned int lb = __shfl(bin, leader);
             const bool mine = todo && bin == lb;
-            const uint64_t m = __ballot(mine);
+            const uint64_t m = wave_ballot(mine);
             unsigned int base = 0;
             if ((int)lane_id() == leader)
                 base = atomicAdd(&cell_size[lb], (unsigned int)__popcll(m));
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
                 slot = base + lanes_below(m);
                 todo = false;
             }
-            pend = __ballot(todo);
+            pend = wave_ballot(todo);
         }
         if (ok)
         {
@@ -414,10 +414,10 @@ __device__ __forceinline__ void cell_centre(const DevBox& b, const unsigned int*
 // (needs >= 4 cells per axis); otherwise every candidate is tested in FP64.
 template<bool FAST, bool MT, bool EXCL> struct NlistShared
 {
-    float4 sc[FAST ? NLCAP : 1];
+    float4 sc[FAST ? NLCAP + 64 : 1];  // + far-away padding up to a whole chunk of 64
     double sx[FAST ? 1 : NLCAP], sy[FAST ? 1 : NLCAP], sz[FAST ? 1 : NLCAP];
     unsigned int sj[FAST ? 1 : NLCAP];
-    unsigned char stp[MT ? NLCAP : 1];
+    unsigned char stp[MT ? NLCAP + 64 : 1];
     float4 sh[NLHOME];
     unsigned char shtp[MT ? NLHOME : 1];
     unsigned int sslot[NLCAP];  // slot mode: candidate slot | offset code
@@ -499,6 +499,18 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             if (MT)
                 S.stp[k] = (unsigned char)e.type;
         }
+        if (FAST)
+        {
+            // pad the pass to a whole chunk with candidates far outside every cutoff: the sweep
+            // then needs no bounds test (their r^2 is far above r_list^2 and the guard band)
+            const unsigned int padded = (total + 63u) & ~63u;
+            for (unsigned int k = total + tid; k < padded; k += NLB)
+            {
+                S.sc[k] = make_float4(3.0e18f, 3.0e18f, 3.0e18f, __uint_as_float(0u));
+                if (MT)
+                    S.stp[k] = 0;
+            }
+        }
         __syncthreads();
         for (unsigned int s = wave; s < nhome; s += NLB / 64)
         {
@@ -531,19 +543,20 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                     unsigned int tp[U];
                     bool hit[U];
                     unsigned int chk = 0;
+                    // positions past the pass's candidates hold the far-away padding
 #pragma unroll
                     for (int u = 0; u < U; u++)
-                        c4[u] = S.sc[min(b + 64u * u + lane, total - 1)];
+                        c4[u] = S.sc[b + 64u * u + lane];
 #pragma unroll
                     for (int u = 0; u < U; u++)
                     {
-                        const bool valid = b + 64u * u + lane < total;
                         const float dx = h4.x - c4[u].x, dy = h4.y - c4[u].y, dz = h4.z - c4[u].z;
-                        const float dr2 = dx * dx + dy * dy + dz * dz;
-                        tp[u] = MT ? ti * a.ntypes + S.stp[min(b + 64u * u + lane, total - 1)] : 0u;
+                        // fused: at most the rounding error the guard band allows for
+                        const float dr2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        tp[u] = MT ? ti * a.ntypes + S.stp[b + 64u * u + lane] : 0u;
                         const float t = dr2 - a.rlistsq_f[tp[u]];
-                        hit[u] = valid && t <= 0.f;
-                        chk |= (valid && fabsf(t) <= a.guard[tp[u]]) ? (1u << u) : 0u;
+                        hit[u] = t <= 0.f;
+                        chk |= fabsf(t) <= a.guard[tp[u]] ? (1u << u) : 0u;
                     }
                     if (chk)
                     {
@@ -566,7 +579,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         bool h = hit[u] && (j != i) && a.enabled[tp[u]];
                         if (EXCL)
                             h = h && j != ex0 && j != ex1 && j != ex2 && j != ex3;
-                        const uint64_t m = __ballot(h);
+                        const uint64_t m = wave_ballot(h);
                         if (h)
                         {
                             const unsigned int k = b + 64u * u + lane;
