@@ -210,6 +210,90 @@ def howto_cpu(snap, seed, seconds):
     return steps / dt, dt, steps
 
 
+def kg_melt(nchain=20000, length=50, rho=0.85):
+    """Kremer-Grest melt without overlaps: beads on a simple-cubic lattice at density rho, chains
+    of `length` consecutive sites along z rows (rows of n sites hold n / length chains), so
+    every bond starts at the lattice spacing (1.056 at rho 0.85) and no two beads overlap."""
+    N = nchain * length
+    n = int(round(N ** (1 / 3)))
+    if n ** 3 != N or n % length:
+        raise ValueError("nchain * length must be a cube whose edge is a multiple of length")
+    a = rho ** (-1 / 3)
+    L = n * a
+    g = np.arange(n)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")  # x varies slowest: index = (x n + y) n + z
+    # chains run along z (consecutive indices), length beads each
+    pos = (np.stack([x, y, z], -1).reshape(-1, 3) + 0.5) * a - L / 2
+    idx = np.arange(N).reshape(-1, length)
+    bonds = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1).astype(np.uint32)
+    return pos, bonds, L
+
+
+def kg(steps, warmup, seed, cpu_steps):
+    """SURVEY config C2 production step: 20 000 chains x 50 beads at rho 0.85, FENE (k 30,
+    r0 1.5) + WCA bonds, WCA pairs (r_cut 2^(1/6), mode shift, bond exclusions, buffer 0.4),
+    Langevin kT 1, gamma 1, dt 0.01. Prints one JSON line."""
+    import hoomd_amd as hoomd
+    from hoomd_amd import _lib
+    pos, bonds, L = kg_melt()
+    N = len(pos)
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = N
+    snap.particles.types = ["A"]
+    snap.particles.position[:] = pos
+    snap.bonds.N = len(bonds)
+    snap.bonds.types = ["A-A"]
+    snap.bonds.group[:] = bonds
+    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
+    sim.create_state_from_snapshot(snap)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
+    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=("bond",))
+    wca = hoomd.md.pair.LJ(nlist=nl, default_r_cut=2 ** (1 / 6), mode="shift")
+    wca.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0)
+    fene = hoomd.md.bond.FENEWCA()
+    fene.params["A-A"] = dict(k=30.0, r0=1.5, epsilon=1.0, sigma=1.0, delta=0.0)
+    lang = hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.0, default_gamma=1.0)
+    sim.operations.integrator = hoomd.md.Integrator(dt=0.01, forces=[wca, fene], methods=[lang])
+    log(f"[bench] KG melt N={N}, {len(bonds)} bonds: warmup {warmup} steps")
+    sim.run(warmup)
+    _lib.check(_lib.lib().hm_synchronize())
+    t0 = time.perf_counter()
+    sim.run(steps)
+    _lib.check(_lib.lib().hm_synchronize())
+    elapsed = time.perf_counter() - t0
+    tps = steps / elapsed
+    cpu = None
+    if cpu_steps:
+        import oracle as O
+        snap = sim.state.get_snapshot()
+        s = O.System(snap.particles.position, snap.particles.velocity, L, r_cut=2 ** (1 / 6),
+                     mode="shift", buffer=0.4, dt=0.01, seed=seed)
+        s.set_bonds(bonds, 30.0, 1.5, 1.0, 1.0, 0.0, exclude=True)
+        s.set_langevin(1.0, 1.0)
+        s.set_sorter(200, 256)
+        s.run(1)
+        t1 = time.perf_counter()
+        s.run(cpu_steps)
+        cdt = time.perf_counter() - t1
+        cpu = {"value": cpu_steps / cdt, "unit": "timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"N={N} KG melt state after the timed steps, {cpu_steps} steps, "
+                         f"{cdt:.1f} s on one host core"}
+    print(json.dumps({
+        "metric": "timesteps/s (Kremer-Grest melt, Langevin)", "value": tps,
+        "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+        "particle_steps_per_s": tps * N,
+        "dtype": "f32 pair/bond arithmetic, f64 positions/integration",
+        "data": "synthetic (lattice chains, thermalized)",
+        "config": {"workload": f"KG melt {N // 50} x 50 beads, rho 0.85, FENE k 30 r0 1.5 + WCA, "
+                               "WCA pairs shift, bond exclusions, buffer 0.4, Langevin kT 1 "
+                               "gamma 1, dt 0.01", "N": N, "bonds": int(len(bonds)),
+                   "nlist_builds_in_timed_steps": nl.num_builds},
+        "cpu_baseline": cpu}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,7 +317,14 @@ def main():
     ap.add_argument("--howto", type=int, default=0, metavar="R",
                     help="run the reference's how-to device benchmark instead: spheres.gsd "
                          "replicated R x R x R (1: N = 2048, 4: N = 131072)")
+    ap.add_argument("--kg", action="store_true",
+                    help="run the Kremer-Grest melt (10^6 beads, Langevin) instead")
     a = ap.parse_args()
+    if a.kg:
+        if a.gpus != 1:
+            print("bench.py runs on exactly one GPU", file=sys.stderr)
+            return 2
+        return kg(a.steps, a.warmup, a.seed, 0 if a.no_cpu else a.cpu_steps)
     if a.howto:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)

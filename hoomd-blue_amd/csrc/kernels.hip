@@ -233,7 +233,6 @@ hipError_t launch_cell_clear(unsigned int ncell, unsigned int* cell_size, const 
 // the CPU's FP64 formula every candidate whose FP32 distance lies within a guard band of the
 // threshold, the band being four times the worst-case FP32 error; rows therefore equal the CPU
 // rows entry for entry. Grids with fewer than 4 cells along an axis use the FP64 kernel.
-constexpr int NLB = 256;
 #ifndef HM_NLU
 #define HM_NLU 4
 #endif
@@ -243,6 +242,17 @@ constexpr int NLU = HM_NLU;  // candidate chunks of 64 per sweep iteration (FP32
 #endif
 constexpr int NLCAP = HM_NLCAP;
 constexpr int NLHOME = 256;
+// Workgroup shapes of the list build. Large cells (tens of particles): four wavefronts share a
+// home cell's candidates. Small cells (a few particles, short cutoffs such as WCA): one
+// wavefront per workgroup and a smaller stage, so that more cells are in flight per CU.
+struct NlBig
+{
+    static constexpr int B = 256, CAP = NLCAP, HOME = NLHOME;
+};
+struct NlSmall
+{
+    static constexpr int B = 64, CAP = 384, HOME = 64;
+};
 
 // Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
 // minimum image, rounded to FP32; w keeps the packed word.
@@ -353,7 +363,8 @@ __device__ __forceinline__ unsigned int find_segment(const unsigned int* aoff, u
     return lo;
 }
 
-// thread 0: offsets of the adjacent cells of one staging pass (at most NLCAP candidates)
+// thread 0: offsets of the adjacent cells of one staging pass (at most CAP candidates)
+template<int CAP>
 __device__ __forceinline__ void pass_bounds(unsigned int n_adj, const unsigned int* asize,
                                             unsigned int* aoff, unsigned int first,
                                             unsigned int* pass_last, unsigned int* pass_total)
@@ -361,7 +372,7 @@ __device__ __forceinline__ void pass_bounds(unsigned int n_adj, const unsigned i
     if (threadIdx.x == 0)
     {
         unsigned int tot = 0, l = first;
-        while (l < n_adj && tot + asize[l] <= NLCAP)
+        while (l < n_adj && tot + asize[l] <= CAP)
         {
             aoff[l] = tot;
             tot += asize[l];
@@ -412,23 +423,23 @@ __device__ __forceinline__ void cell_centre(const DevBox& b, const unsigned int*
 
 // One home cell of the neighbour-list build. FAST: FP32 test with FP64 guard-band re-decision
 // (needs >= 4 cells per axis); otherwise every candidate is tested in FP64.
-template<bool FAST, bool MT, bool EXCL> struct NlistShared
+template<bool FAST, bool MT, bool EXCL, class C> struct NlistShared
 {
-    float4 sc[FAST ? NLCAP + 64 : 1];  // + far-away padding up to a whole chunk of 64
-    double sx[FAST ? 1 : NLCAP], sy[FAST ? 1 : NLCAP], sz[FAST ? 1 : NLCAP];
-    unsigned int sj[FAST ? 1 : NLCAP];
-    unsigned char stp[MT ? NLCAP + 64 : 1];
-    float4 sh[NLHOME];
-    unsigned char shtp[MT ? NLHOME : 1];
-    unsigned int sslot[NLCAP];  // slot mode: candidate slot | offset code
+    float4 sc[FAST ? C::CAP + 64 : 1];  // + far-away padding up to a whole chunk of 64
+    double sx[FAST ? 1 : C::CAP], sy[FAST ? 1 : C::CAP], sz[FAST ? 1 : C::CAP];
+    unsigned int sj[FAST ? 1 : C::CAP];
+    unsigned char stp[MT ? C::CAP + 64 : 1];
+    float4 sh[C::HOME];
+    unsigned char shtp[MT ? C::HOME : 1];
+    unsigned int sslot[C::CAP];  // slot mode: candidate slot | offset code
     unsigned int acell[27], asize[27], aoff[27], acode[27];
-    unsigned int hcnt[NLHOME];
+    unsigned int hcnt[C::HOME];
     unsigned int pass_last, pass_total;
 };
 
-template<bool FAST, bool MT, bool EXCL>
+template<bool FAST, bool MT, bool EXCL, class C>
 __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
-                           NlistShared<FAST, MT, EXCL>& S)
+                           NlistShared<FAST, MT, EXCL, C>& S)
 {
 #pragma clang fp contract(off)
     const unsigned int nhome = min(a.cell_size[cell], a.cmax);
@@ -456,7 +467,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         }
         S.acode[tid] = code;
     }
-    for (unsigned int s = tid; s < nhome; s += NLB)
+    for (unsigned int s = tid; s < nhome; s += C::B)
     {
         const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
         if (FAST)
@@ -475,10 +486,10 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
     unsigned int first = 0, base_k = 0;
     while (first < a.n_adj)
     {
-        pass_bounds(a.n_adj, S.asize, S.aoff, first, &S.pass_last, &S.pass_total);
+        pass_bounds<C::CAP>(a.n_adj, S.asize, S.aoff, first, &S.pass_last, &S.pass_total);
         __syncthreads();
         const unsigned int last = S.pass_last, total = S.pass_total;
-        for (unsigned int k = tid; k < total; k += NLB)
+        for (unsigned int k = tid; k < total; k += C::B)
         {
             const unsigned int q = find_segment(S.aoff, first, last, k);
             const hm_cell_entry e = a.entries[(size_t)S.acell[q] * a.cmax + (k - S.aoff[q])];
@@ -504,7 +515,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             // pad the pass to a whole chunk with candidates far outside every cutoff: the sweep
             // then needs no bounds test (their r^2 is far above r_list^2 and the guard band)
             const unsigned int padded = (total + 63u) & ~63u;
-            for (unsigned int k = total + tid; k < padded; k += NLB)
+            for (unsigned int k = total + tid; k < padded; k += C::B)
             {
                 S.sc[k] = make_float4(3.0e18f, 3.0e18f, 3.0e18f, __uint_as_float(0u));
                 if (MT)
@@ -512,7 +523,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             }
         }
         __syncthreads();
-        for (unsigned int s = wave; s < nhome; s += NLB / 64)
+        for (unsigned int s = wave; s < nhome; s += C::B / 64)
         {
             const float4 h4 = S.sh[s];
             const unsigned int i = __float_as_uint(h4.w);
@@ -655,7 +666,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         base_k += total;
         first = last;
     }
-    for (unsigned int s = tid; s < nhome; s += NLB)
+    for (unsigned int s = tid; s < nhome; s += C::B)
     {
         const unsigned int i = __float_as_uint(S.sh[s].w);
         const unsigned int c = S.hcnt[s];
@@ -666,10 +677,10 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
     __syncthreads();
 }
 
-template<bool FAST, bool MT, bool EXCL>
-__global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
+template<bool FAST, bool MT, bool EXCL, class C>
+__global__ __launch_bounds__(C::B) void nlist_kernel(NlistKernelArgs a)
 {
-    __shared__ NlistShared<FAST, MT, EXCL> S;
+    __shared__ NlistShared<FAST, MT, EXCL, C> S;
     if (gate_closed(a.gate))
         return;
     if (a.counter && blockIdx.x == 0 && threadIdx.x == 0)
@@ -689,21 +700,22 @@ __global__ __launch_bounds__(NLB) void nlist_kernel(NlistKernelArgs a)
     }
     const XcdRange r = xcd_range(a.ncell);
     for (unsigned int cell = r.begin; cell < r.end; cell += r.stride)
-        nlist_cell<FAST, MT, EXCL>(a, cell, S);
+        nlist_cell<FAST, MT, EXCL, C>(a, cell, S);
 }
 
-template<bool FAST>
+template<bool FAST, class C>
 static void nlist_dispatch(const NlistKernelArgs& a, bool mt, bool ex, unsigned int grid,
                            hipStream_t s)
 {
     if (mt && ex)
-        hipLaunchKernelGGL((nlist_kernel<FAST, true, true>), dim3(grid), dim3(NLB), 0, s, a);
+        hipLaunchKernelGGL((nlist_kernel<FAST, true, true, C>), dim3(grid), dim3(C::B), 0, s, a);
     else if (mt)
-        hipLaunchKernelGGL((nlist_kernel<FAST, true, false>), dim3(grid), dim3(NLB), 0, s, a);
+        hipLaunchKernelGGL((nlist_kernel<FAST, true, false, C>), dim3(grid), dim3(C::B), 0, s, a);
     else if (ex)
-        hipLaunchKernelGGL((nlist_kernel<FAST, false, true>), dim3(grid), dim3(NLB), 0, s, a);
+        hipLaunchKernelGGL((nlist_kernel<FAST, false, true, C>), dim3(grid), dim3(C::B), 0, s, a);
     else
-        hipLaunchKernelGGL((nlist_kernel<FAST, false, false>), dim3(grid), dim3(NLB), 0, s, a);
+        hipLaunchKernelGGL((nlist_kernel<FAST, false, false, C>), dim3(grid), dim3(C::B), 0, s,
+                           a);
 }
 
 hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
@@ -712,6 +724,10 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
         return hipSuccess;
     NlistKernelArgs a = a_in;
     const unsigned int grid = round_up(min(a.ncell, GATED_BLOCKS), 8);
+    // small cells: a few particles per cell on average, and every home cell fits the
+    // one-wavefront workgroup (cmax <= 64)
+    const bool small = a.cmax <= (unsigned int)NlSmall::HOME && a.N <= 8ull * a.ncell;
+    const unsigned int grid_small = round_up(min(a.ncell, 2 * GATED_BLOCKS), 8);
     const bool mt = a.ntypes > 1;
     const bool ex = a.n_excl != nullptr && a.max_excl > 0;
     const bool fast = a.pos != nullptr && a.dim[0] >= 4 && a.dim[1] >= 4 && a.dim[2] >= 4;
@@ -733,10 +749,15 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
             a.rlistsq_f[q] = (float)rl2;
             a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rl * e_dx + 6.0 * u * rl2) + 1e-12);
         }
-        nlist_dispatch<true>(a, mt, ex, grid, s);
+        if (small)
+            nlist_dispatch<true, NlSmall>(a, mt, ex, grid_small, s);
+        else
+            nlist_dispatch<true, NlBig>(a, mt, ex, grid, s);
     }
+    else if (small)
+        nlist_dispatch<false, NlSmall>(a, mt, ex, grid_small, s);
     else
-        nlist_dispatch<false>(a, mt, ex, grid, s);
+        nlist_dispatch<false, NlBig>(a, mt, ex, grid, s);
     return hipGetLastError();
 }
 
