@@ -251,7 +251,7 @@ struct NlBig
 };
 struct NlSmall
 {
-    static constexpr int B = 64, CAP = 384, HOME = 64;
+    static constexpr int B = 64, CAP = 256, HOME = 64;
 };
 
 // Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
@@ -727,7 +727,7 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
     // small cells: a few particles per cell on average, and every home cell fits the
     // one-wavefront workgroup (cmax <= 64)
     const bool small = a.cmax <= (unsigned int)NlSmall::HOME && a.N <= 8ull * a.ncell;
-    const unsigned int grid_small = round_up(min(a.ncell, 2 * GATED_BLOCKS), 8);
+    const unsigned int grid_small = round_up(min(a.ncell, 4 * GATED_BLOCKS), 8);
     const bool mt = a.ntypes > 1;
     const bool ex = a.n_excl != nullptr && a.max_excl > 0;
     const bool fast = a.pos != nullptr && a.dim[0] >= 4 && a.dim[1] >= 4 && a.dim[2] >= 4;
