@@ -249,6 +249,10 @@ struct NlBig
 {
     static constexpr int B = 256, CAP = NLCAP, HOME = NLHOME;
 };
+struct NlMid
+{
+    static constexpr int B = 256, CAP = NLCAP, HOME = 64;
+};
 struct NlSmall
 {
     static constexpr int B = 64, CAP = 256, HOME = 64;
@@ -751,6 +755,8 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
         }
         if (small)
             nlist_dispatch<true, NlSmall>(a, mt, ex, grid_small, s);
+        else if (a.cmax <= (unsigned int)NlMid::HOME)
+            nlist_dispatch<true, NlMid>(a, mt, ex, grid, s);
         else
             nlist_dispatch<true, NlBig>(a, mt, ex, grid, s);
     }
