@@ -11,6 +11,8 @@ class FENEWCA(Force):
     """hoomd.md.bond.FENEWCA(): params[bond_type] = dict(k, r0, epsilon, sigma, delta)
     (EvaluatorBondFENE.h:30-156, bond.py:108-163)."""
 
+    _log_namespace = ("md", "bond", "FENEWCA")
+
     def __init__(self):
         super().__init__()
         self.params = TypeParameter()

@@ -6,6 +6,12 @@ from .. import _lib
 
 
 class ThermodynamicQuantities:
+    _log_namespace = ("md", "compute", "ThermodynamicQuantities")
+    _loggables = {"kinetic_temperature": ("scalar", True), "pressure": ("scalar", True),
+                  "pressure_tensor": ("sequence", True), "kinetic_energy": ("scalar", True),
+                  "potential_energy": ("scalar", True), "num_particles": ("scalar", True),
+                  "translational_degrees_of_freedom": ("scalar", True),
+                  "volume": ("scalar", True)}
     def __init__(self, filter):
         self.filter = filter
         self._simulation = None

@@ -10,6 +10,10 @@ from .. import _lib
 
 
 class Force:
+    # md/force.py Force loggables: the total energy by default, per-particle arrays on request
+    _loggables = {"energy": ("scalar", True), "energies": ("particle", False),
+                  "forces": ("particle", False), "virials": ("particle", False)}
+
     def __init__(self):
         self._simulation = None
         self._force_id = None

@@ -68,6 +68,9 @@ class NeighborList:
 class Cell(NeighborList):
     """hoomd.md.nlist.Cell(buffer, exclusions=('bond',), rebuild_check_delay=1, check_dist=True)."""
 
+    _log_namespace = ("md", "nlist", "Cell")
+    _loggables = {"shortest_rebuild": ("scalar", True), "num_builds": ("scalar", False)}
+
     def __init__(self, buffer, exclusions=("bond",), rebuild_check_delay=1, check_dist=True,
                  deterministic=False, mesh=None):
         super().__init__(buffer, exclusions, rebuild_check_delay, check_dist)

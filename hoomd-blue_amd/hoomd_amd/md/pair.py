@@ -41,6 +41,8 @@ class LJ(Pair):
     """hoomd.md.pair.LJ(nlist, default_r_cut=None, default_r_on=0., mode='none'):
     params[(a, b)] = dict(epsilon=..., sigma=...) (EvaluatorPairLJ.h:99-155)."""
 
+    _log_namespace = ("md", "pair", "LJ")
+
     def __init__(self, nlist, default_r_cut=None, default_r_on=0.0, mode="none",
                  tail_correction=False):
         super().__init__(nlist, default_r_cut, default_r_on, mode)
@@ -81,6 +83,7 @@ class Table(Pair):
     (EvaluatorPairTable.h:152-196). Mode 'none' only."""
 
     _modes = {"none": 0}
+    _log_namespace = ("md", "pair", "Table")
 
     def __init__(self, nlist, default_r_cut=None):
         super().__init__(nlist, default_r_cut, 0.0, "none")

@@ -8,6 +8,10 @@ from ..tune.solve import GridOptimizer, ManualTuneDefinition, Optimizer
 
 
 class NeighborListBuffer:
+    _log_namespace = ("md", "tune", "NeighborListBuffer")
+    _loggables = {"max_tps": ("scalar", True), "best_buffer_size": ("scalar", True),
+                  "last_tps": ("scalar", True)}
+
     """Acts on the steps its trigger selects, before the step runs (as tuners do in
     System::run, System.cc:131-137): the TPS since the previous act in the same run is the
     measurement for the current buffer (nlist_buffer.py:20-57, the first act of a run only

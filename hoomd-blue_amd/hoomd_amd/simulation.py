@@ -39,8 +39,8 @@ class Operations:
         from .md.compute import ThermodynamicQuantities
         from .md.tune import NeighborListBuffer
         from .tune import ParticleSorter
-        from .write import GSD
-        if isinstance(op, GSD):
+        from .write import GSD, Table
+        if isinstance(op, (GSD, Table)):
             self.writers.append(op)
             return
         if isinstance(op, (ParticleSorter, NeighborListBuffer)):
@@ -87,6 +87,10 @@ class Operations:
 
 class Simulation:
     """hoomd.Simulation(device, seed)."""
+
+    _log_namespace = ("Simulation",)
+    _loggables = {"timestep": ("scalar", True), "seed": ("scalar", True), "tps": ("scalar", True),
+                  "walltime": ("scalar", True)}
 
     def __init__(self, device, seed=None):
         self.device = device

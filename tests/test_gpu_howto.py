@@ -139,3 +139,30 @@ def test_nlist_buffer_tuner_grid(hoomd):
     h = sim._system
     sim.run(50)
     assert sim._system == h and abs(lj.nlist.buffer - (lo + hi) / 2) < 1e-12
+
+
+def test_logger_table_during_run(hoomd):
+    """A Logger of the simulation, thermodynamic quantities and the pair force, written by
+    write.Table every 10 steps of a run: a header and one row per trigger step, the last row
+    holding the values the objects report after the run."""
+    import io
+    sim, lj = howto_sim(hoomd, (1, 1, 1), seed=4)
+    thermo = hoomd.md.compute.ThermodynamicQuantities(hoomd.filter.All())
+    sim.operations.add(thermo)
+    log = hoomd.logging.Logger(categories=["scalar", "string"])
+    log.add(sim, quantities=["timestep"])
+    log.add(thermo, quantities=["kinetic_temperature", "potential_energy"])
+    log.add(lj, quantities=["energy"])
+    out = io.StringIO()
+    sim.operations.add(hoomd.write.Table(trigger=10, logger=log, output=out))
+    sim.run(30)
+    lines = out.getvalue().splitlines()
+    assert lines[0].split() == ["Simulation.timestep",
+                                "md.compute.ThermodynamicQuantities.kinetic_temperature",
+                                "md.compute.ThermodynamicQuantities.potential_energy",
+                                "md.pair.LJ.energy"]
+    rows = [r.split() for r in lines[1:]]
+    assert [int(r[0]) for r in rows] == [10, 20, 30]
+    last = rows[-1]
+    assert abs(float(last[1]) - thermo.kinetic_temperature) < 1e-4 * thermo.kinetic_temperature
+    assert abs(float(last[3]) - lj.energy) < 1e-3 * abs(lj.energy) + 1e-6
