@@ -373,17 +373,44 @@ __device__ __forceinline__ void pass_bounds(unsigned int n_adj, const unsigned i
                                             unsigned int* aoff, unsigned int first,
                                             unsigned int* pass_last, unsigned int* pass_total)
 {
-    if (threadIdx.x == 0)
+    // wave 0: prefix sum of the remaining cells' sizes across lanes; when they all fit the
+    // stage the pass is every remaining cell, otherwise thread 0 takes cells while they fit
+    if (threadIdx.x < 64)
     {
-        unsigned int tot = 0, l = first;
-        while (l < n_adj && tot + asize[l] <= CAP)
+        const unsigned int t = threadIdx.x;
+        const unsigned int q = first + t;
+        const unsigned int v = q < n_adj ? asize[q] : 0u;
+        unsigned int incl = v;
+#pragma unroll
+        for (unsigned int off = 1; off < 64; off <<= 1)
         {
-            aoff[l] = tot;
-            tot += asize[l];
-            l++;
+            const unsigned int u = __shfl_up(incl, off, 64);
+            if (t >= off)
+                incl += u;
         }
-        *pass_last = l;
-        *pass_total = tot;
+        const unsigned int all = __shfl(incl, 63, 64);
+        if (all <= (unsigned int)CAP)
+        {
+            if (q < n_adj)
+                aoff[q] = incl - v;
+            if (t == 0)
+            {
+                *pass_last = n_adj;
+                *pass_total = all;
+            }
+        }
+        else if (t == 0)
+        {
+            unsigned int tot = 0, l = first;
+            while (l < n_adj && tot + asize[l] <= CAP)
+            {
+                aoff[l] = tot;
+                tot += asize[l];
+                l++;
+            }
+            *pass_last = l;
+            *pass_total = tot;
+        }
     }
 }
 
@@ -453,7 +480,39 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
     double cx = 0, cy = 0, cz = 0;
     if (FAST)
         cell_centre(a.box, a.dim, cell, cx, cy, cz);
-    if (tid < a.n_adj)
+    if (a.n_adj == 27 && a.dim[0] >= 3 && a.dim[1] >= 3 && a.dim[2] >= 3)
+    {
+        // the 27 adjacent cells computed from the home cell's coordinates and put in
+        // increasing index order (the order of cell_adj), which saves a dependent load
+        if (tid < 64)
+        {
+            const int dx = (int)a.dim[0], dy = (int)a.dim[1], dz = (int)a.dim[2];
+            const int hx = (int)(cell % a.dim[0]), hy = (int)((cell / a.dim[0]) % a.dim[1]),
+                      hz = (int)(cell / (a.dim[0] * a.dim[1]));
+            unsigned int c = 0xffffffffu, code = 0;
+            if (tid < 27)
+            {
+                const int ox = (int)(tid % 3) - 1, oy = (int)((tid / 3) % 3) - 1,
+                          oz = (int)(tid / 9) - 1;
+                const int nx = (hx + ox + dx) % dx, ny = (hy + oy + dy) % dy,
+                          nz = (hz + oz + dz) % dz;
+                c = (unsigned int)(nx + dx * (ny + dy * nz));
+                if (a.offset_codes)
+                    code = offset_code(-ox, -oy, -oz);
+            }
+            unsigned int rank = 0;
+#pragma unroll
+            for (int p = 0; p < 27; p++)
+                rank += __builtin_amdgcn_readlane(c, p) < c ? 1u : 0u;
+            if (tid < 27)
+            {
+                S.acell[rank] = c;
+                S.asize[rank] = min(a.cell_size[c], a.cmax);
+                S.acode[rank] = code;
+            }
+        }
+    }
+    else if (tid < a.n_adj)
     {
         unsigned int c = a.cell_adj[(size_t)cell * a.n_adj + tid];
         S.acell[tid] = c;
