@@ -118,6 +118,9 @@ struct hm_system
     unsigned int *cell_size = nullptr, *cell_adj = nullptr, *nlist = nullptr, *n_neigh = nullptr;
     hm_cell_entry* entries = nullptr;
     unsigned int* cell_idx = nullptr;
+    // binning pass output: unordered particle index of each slot [ncell * cmax], unordered slot
+    // of each particle [N]
+    unsigned int *ckey = nullptr, *chome = nullptr;
     unsigned short* lrow = nullptr;
     // pair kernel kind 2: cell-slot relative FP32 positions [ncell * cmax], packed cell and
     // slot of every particle [N] (RelFrame)
@@ -211,6 +214,8 @@ struct hm_system
         dfree(n_neigh);
         dfree(entries);
         dfree(cell_idx);
+        dfree(ckey);
+        dfree(chome);
         dfree(lrow);
         dfree(rel);
         dfree(relw);
@@ -320,6 +325,7 @@ struct hm_system
         {
             pcell = dalloc<uint32_t>(N);
             slot_of = dalloc<uint32_t>(N);
+            chome = dalloc<unsigned int>(N);
             hchk(hipMemsetAsync(pcell, 0, sizeof(uint32_t) * N, stream), "clear");
             hchk(hipMemsetAsync(slot_of, 0, sizeof(uint32_t) * N, stream), "clear");
         }
@@ -347,10 +353,12 @@ struct hm_system
     {
         dfree(entries);
         dfree(cell_idx);
+        dfree(ckey);
         dfree(rel);
         dfree(relw);
         entries = dalloc<hm_cell_entry>((size_t)ncell * cmax);
         cell_idx = dalloc<unsigned int>((size_t)ncell * cmax);
+        ckey = dalloc<unsigned int>((size_t)ncell * cmax);
         rel = dalloc<float3>((size_t)ncell * cmax);
         relw = dalloc<uint32_t>((size_t)ncell * cmax);
         hchk(hipMemsetAsync(cell_idx, 0, sizeof(unsigned int) * ncell * cmax, stream), "clear");
@@ -727,11 +735,11 @@ struct hm_system
         {
             if (!fused_clear)
                 hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
-            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, st, stream),
+            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, nullptr, last,
+                                   &cond->cell_need, st, stream, ckey, chome),
                  "cell build");
-            const RelFrame rf = rel_frame();
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, st, stream, &box, &rf),
+            hchk(launch_cell_rank(N, cmax, pos, cell_size, ckey, chome, entries, cell_idx, st,
+                                  stream, box, rel_frame()),
                  "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, true);
@@ -768,8 +776,8 @@ struct hm_system
         {
             hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
             hchk(hipMemsetAsync(cell_size, 0, sizeof(unsigned int) * ncell, stream), "clear cells");
-            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, entries, last,
-                                   &cond->cell_need, nullptr, stream),
+            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, nullptr, last,
+                                   &cond->cell_need, nullptr, stream, ckey, chome),
                  "cell build");
             hchk(hipMemcpyAsync(h_cond, cond, sizeof(Conditions), hipMemcpyDeviceToHost, stream),
                  "read conditions");
@@ -783,9 +791,8 @@ struct hm_system
                 grow_cmax(h_cond->cell_need);
                 continue;
             }
-            const RelFrame rf = rel_frame();
-            hchk(launch_cell_sort(ncell, cmax, cell_size, entries, cell_idx, nullptr, stream, &box,
-                                  &rf),
+            hchk(launch_cell_rank(N, cmax, pos, cell_size, ckey, chome, entries, cell_idx, nullptr,
+                                  stream, box, rel_frame()),
                  "cell order");
             NlistKernelArgs k;
             fill_nlist_args(k, false);

@@ -143,10 +143,15 @@ inline RelFrame make_rel_frame(float3* rel, uint32_t* relw, const uint32_t* pcel
 // ---------------------------------------------------------------- cell list / nlist
 hipError_t launch_cell_adj(const unsigned int dim[3], unsigned int* adj, hipStream_t s);
 
+// Bin every particle. With keys == NULL the particle's entry goes to its (unordered) slot of
+// entries; otherwise keys[slot] receives the particle index and home[i] the slot (NO_SLOT when
+// the particle was not binned), and launch_cell_rank writes the ordered entries.
+constexpr unsigned int NO_SLOT = 0xffffffffu;
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s);
+                             unsigned int* cond3, const StepState* gate, hipStream_t s,
+                             unsigned int* keys = nullptr, unsigned int* home = nullptr);
 
 struct NlistKernelArgs
 {
@@ -221,6 +226,13 @@ hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigne
                             hm_cell_entry* entries, unsigned int* cell_idx, const StepState* gate,
                             hipStream_t s, const DevBox* box = nullptr,
                             const RelFrame* rel = nullptr);
+// order each cell's slots by particle index from the key table of launch_cell_build: one thread
+// per particle, writes the ordered entries, index table and relative positions
+hipError_t launch_cell_rank(unsigned int N, unsigned int cmax, const double4* pos,
+                            const unsigned int* cell_size, const unsigned int* keys,
+                            const unsigned int* home, hm_cell_entry* entries,
+                            unsigned int* cell_idx, const StepState* gate, hipStream_t s,
+                            const DevBox& box, const RelFrame& rel);
 hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
 
 hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,

@@ -104,7 +104,8 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
                                                          unsigned int dz, unsigned int cmax,
                                                          unsigned int* cell_size,
                                                          hm_cell_entry* entries, d3* last,
-                                                         unsigned int* cond, const StepState* gate)
+                                                         unsigned int* cond, const StepState* gate,
+                                                         unsigned int* keys, unsigned int* home)
 {
 #pragma clang fp contract(off)
     if (gate_closed(gate))
@@ -146,42 +147,45 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
             else
                 bin = (unsigned int)((kb * (int)dy + jb) * (int)dx + ib);
         }
-        // slot allocation aggregated over the wave: one atomic per distinct cell among the
-        // active lanes (Hilbert-ordered particles put a wave into a few cells)
-        unsigned int slot = 0;
-        bool todo = ok;
-        uint64_t pend = wave_ballot(todo);
-        while (pend)
+        // slot allocation aggregated over runs of consecutive lanes in one cell (particles in
+        // Hilbert order fill a cell from neighbouring indices): the first lane of a run adds the
+        // run's length, and every run's atomic is issued by the one instruction. Active lanes are
+        // the low lanes of the wave, so a lane's predecessor is active.
+        const unsigned int lane = lane_id();
+        const unsigned int prev = __shfl_up(bin, 1, 64);
+        const bool head = ok && (lane == 0 || prev != bin || !__shfl_up((int)ok, 1, 64));
+        const uint64_t heads = wave_ballot(head);
+        // lanes that do not continue a run: heads, lanes not binned, inactive lanes
+        const uint64_t stop = ~wave_ballot(ok && !head);
+        const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+        const unsigned int first = 63u - (unsigned int)__clzll((long long)(heads & upto));
+        unsigned int base = 0;
+        if (head)
         {
-            const int leader = __ffsll((unsigned long long)pend) - 1;
-            const unsigned int lb = __shfl(bin, leader);
-            const bool mine = todo && bin == lb;
-            const uint64_t m = wave_ballot(mine);
-            unsigned int base = 0;
-            if ((int)lane_id() == leader)
-                base = atomicAdd(&cell_size[lb], (unsigned int)__popcll(m));
-            base = __shfl(base, leader);
-            if (mine)
-            {
-                slot = base + lanes_below(m);
-                todo = false;
-            }
-            pend = wave_ballot(todo);
+            const uint64_t after = lane == 63 ? 0ull : (stop >> (lane + 1));
+            const unsigned int len = after ? (unsigned int)__ffsll((long long)after) : 64u - lane;
+            base = atomicAdd(&cell_size[bin], len);
         }
-        if (ok)
+        base = __shfl(base, (int)(ok ? first : lane), 64);
+        const unsigned int slot = ok ? base + (lane - first) : 0u;
+        const bool placed = ok && slot < cmax;
+        if (ok && !placed)
+            atomicMax(&cond[0], slot + 1);
+        if (keys)
         {
-            if (slot < cmax)
-            {
-                hm_cell_entry e;
-                e.x = p.x;
-                e.y = p.y;
-                e.z = p.z;
-                e.idx = n;
-                e.type = (uint32_t)p.w;
-                entries[(size_t)bin * cmax + slot] = e;
-            }
-            else
-                atomicMax(&cond[0], slot + 1);
+            home[n] = placed ? bin * cmax + slot : NO_SLOT;
+            if (placed)
+                keys[(size_t)bin * cmax + slot] = n;
+        }
+        else if (placed)
+        {
+            hm_cell_entry e;
+            e.x = p.x;
+            e.y = p.y;
+            e.z = p.z;
+            e.idx = n;
+            e.type = (uint32_t)p.w;
+            entries[(size_t)bin * cmax + slot] = e;
         }
     }
 }
@@ -189,13 +193,17 @@ __global__ __launch_bounds__(256) void cell_build_kernel(unsigned int N, const d
 hipError_t launch_cell_build(unsigned int N, const double4* pos, const DevBox& box,
                              const unsigned int dim[3], unsigned int cmax,
                              unsigned int* cell_size, hm_cell_entry* entries, d3* last_pos,
-                             unsigned int* cond3, const StepState* gate, hipStream_t s)
+                             unsigned int* cond3, const StepState* gate, hipStream_t s,
+                             unsigned int* keys, unsigned int* home)
 {
     if (N == 0)
         return hipSuccess;
+    if ((keys == nullptr) != (home == nullptr) || (keys == nullptr && entries == nullptr))
+        return hipErrorInvalidValue;
     unsigned int blocks = min((N + 255) / 256, GATED_BLOCKS);
     hipLaunchKernelGGL(cell_build_kernel, dim3(blocks), dim3(256), 0, s, N, pos, box, dim[0],
-                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate);
+                       dim[1], dim[2], cmax, cell_size, entries, last_pos, cond3, gate, keys,
+                       home);
     return hipGetLastError();
 }
 
@@ -244,18 +252,28 @@ constexpr int NLCAP = HM_NLCAP;
 constexpr int NLHOME = 256;
 // Workgroup shapes of the list build. Large cells (tens of particles): four wavefronts share a
 // home cell's candidates. Small cells (a few particles, short cutoffs such as WCA): one
-// wavefront per workgroup and a smaller stage, so that more cells are in flight per CU.
+// wavefront per workgroup, a smaller stage and two chunks per sweep iteration, so that more
+// cells are in flight per CU.
 struct NlBig
 {
-    static constexpr int B = 256, CAP = NLCAP, HOME = NLHOME;
+    static constexpr int B = 256, CAP = NLCAP, HOME = NLHOME, U = NLU, WAVES = 1;
 };
 struct NlMid
 {
-    static constexpr int B = 256, CAP = NLCAP, HOME = 64;
+    static constexpr int B = 256, CAP = NLCAP, HOME = 64, U = NLU, WAVES = 1;
 };
 struct NlSmall
 {
-    static constexpr int B = 64, CAP = 256, HOME = 64;
+    static constexpr int B = 64, CAP = 256, HOME = 64, U = 2, WAVES = 1;
+};
+// at most about 4 particles per cell: a stage of 128 candidates, and a workgroup small enough
+// in LDS and registers for 8 wavefronts per SIMD
+#ifndef HM_TINY_WAVES
+#define HM_TINY_WAVES 8
+#endif
+struct NlTiny
+{
+    static constexpr int B = 64, CAP = 128, HOME = 64, U = 2, WAVES = HM_TINY_WAVES;
 };
 
 // Position relative to the origin of cell c (packed cx | cy << 8 | cz << 16 | type << 24),
@@ -278,18 +296,55 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
                                                         const StepState* gate, DevBox box,
                                                         RelFrame rf)
 {
-    // one wavefront per cell; a lane keeps its (at most NLHOME / 64) entries in registers, so
-    // the workgroup's LDS is only the index table
+    // One wavefront per cell, each independent of the others (no workgroup barrier). A cell of
+    // at most 64 particles is ranked in registers: lane k holds entry k and counts the smaller
+    // indices with lane reads. Larger cells keep their entries in registers (at most
+    // NLHOME / 64 per lane) and rank through the wave's own LDS index table.
     constexpr int EPL = NLHOME / 64;
     __shared__ unsigned int sidx[4][NLHOME];
     if (gate_closed(gate))
         return;
     const unsigned int w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const unsigned int rounds = (ncell + gridDim.x * 4 - 1) / (gridDim.x * 4);
+    auto place = [&](unsigned int c, const hm_cell_entry& e, unsigned int rank) {
+        const unsigned int my = e.idx;
+        entries[(size_t)c * cmax + rank] = e;
+        if (cell_idx)
+            cell_idx[(size_t)c * cmax + rank] = my;
+        if (rf.rel)
+        {
+            // cell-slot relative position, packed cell and slot of particle my
+            const unsigned int slot = c * cmax + rank;
+            const uint32_t pc = pack_cell(c % rf.dim[0], (c / rf.dim[0]) % rf.dim[1],
+                                          c / (rf.dim[0] * rf.dim[1]), e.type);
+            const float4 rr = rel_of(e.x, e.y, e.z, pc, box, rf);
+            rf.rel[slot] = make_float3(rr.x, rr.y, rr.z);
+            rf.relw[slot] = pc;
+            const_cast<uint32_t*>(rf.pcell)[my] = pc;
+            const_cast<uint32_t*>(rf.slot_of)[my] = slot;
+        }
+    };
     for (unsigned int r = 0; r < rounds; r++)
     {
         const unsigned int c = (r * gridDim.x + blockIdx.x) * 4 + w;
         const unsigned int n = c < ncell ? min(min(cell_size[c], cmax), (unsigned int)NLHOME) : 0u;
+        if (n <= 64)
+        {
+            hm_cell_entry e{};
+            unsigned int key = 0xffffffffu;
+            if (lane < n)
+            {
+                e = entries[(size_t)c * cmax + lane];
+                key = e.idx;
+            }
+            // every lane's entry is in registers before any lane stores (the ranks need all keys)
+            unsigned int rank = 0;
+            for (unsigned int m = 0; m < n; m++)
+                rank += __builtin_amdgcn_readlane(key, m) < key ? 1u : 0u;
+            if (lane < n)
+                place(c, e, rank);
+            continue;
+        }
         hm_cell_entry e[EPL];
 #pragma unroll
         for (int u = 0; u < EPL; u++)
@@ -301,34 +356,28 @@ __global__ __launch_bounds__(256) void cell_sort_kernel(unsigned int ncell, unsi
                 sidx[w][k] = e[u].idx;
             }
         }
-        __syncthreads();
+        // the table is this wave's own: its LDS accesses are ordered, a wavefront fence suffices
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        unsigned int ranks[EPL];
 #pragma unroll
         for (int u = 0; u < EPL; u++)
         {
             const unsigned int k = lane + 64u * u;
-            if (k >= n)
-                continue;
-            const unsigned int my = sidx[w][k];
-            unsigned int rank = 0;
-            for (unsigned int m = 0; m < n; m++)
-                rank += sidx[w][m] < my ? 1u : 0u;
-            entries[(size_t)c * cmax + rank] = e[u];
-            if (cell_idx)
-                cell_idx[(size_t)c * cmax + rank] = my;
-            if (rf.rel)
+            ranks[u] = 0;
+            if (k < n)
             {
-                // cell-slot relative position, packed cell and slot of particle my
-                const unsigned int slot = c * cmax + rank;
-                const uint32_t pc = pack_cell(c % rf.dim[0], (c / rf.dim[0]) % rf.dim[1],
-                                              c / (rf.dim[0] * rf.dim[1]), e[u].type);
-                const float4 rr = rel_of(e[u].x, e[u].y, e[u].z, pc, box, rf);
-                rf.rel[slot] = make_float3(rr.x, rr.y, rr.z);
-                rf.relw[slot] = pc;
-                const_cast<uint32_t*>(rf.pcell)[my] = pc;
-                const_cast<uint32_t*>(rf.slot_of)[my] = slot;
+                const unsigned int my = sidx[w][k];
+                for (unsigned int m = 0; m < n; m++)
+                    ranks[u] += sidx[w][m] < my ? 1u : 0u;
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < EPL; u++)
+            if (lane + 64u * u < n)
+                place(c, e[u], ranks[u]);
     }
 }
 
@@ -348,6 +397,74 @@ hipError_t launch_cell_sort(unsigned int ncell, unsigned int cmax, const unsigne
     }
     hipLaunchKernelGGL(cell_sort_kernel, dim3(blocks), dim3(256), 0, s, ncell, cmax, cell_size,
                        entries, cell_idx, gate, b, rf);
+    return hipGetLastError();
+}
+
+// Per-particle cell ordering (the engine's form of cell_sort_kernel): particle i's place in its
+// cell is the number of smaller particle indices among the cell's keys. Every thread reads only
+// the key table and writes only ordered slots, so the cells need no in-place exchange, and the
+// work is spread over particles rather than over cells (most cells of a short cutoff hold a few
+// particles).
+__global__ __launch_bounds__(256) void cell_rank_kernel(unsigned int N, unsigned int cmax,
+                                                        const double4* __restrict__ pos,
+                                                        const unsigned int* __restrict__ cell_size,
+                                                        const unsigned int* __restrict__ keys,
+                                                        const unsigned int* __restrict__ home,
+                                                        hm_cell_entry* entries,
+                                                        unsigned int* cell_idx,
+                                                        const StepState* gate, DevBox box,
+                                                        RelFrame rf)
+{
+    if (gate_closed(gate))
+        return;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += gridDim.x * blockDim.x)
+    {
+        const unsigned int h = home[i];
+        if (h == NO_SLOT)
+            continue;
+        const unsigned int c = h / cmax;
+        const unsigned int n = min(cell_size[c], cmax);
+        const unsigned int* kc = keys + (size_t)c * cmax;
+        unsigned int rank = 0;
+        for (unsigned int m = 0; m < n; m++)
+            rank += kc[m] < i ? 1u : 0u;
+        const double4 p = pos[i];
+        const unsigned int slot = c * cmax + rank;
+        hm_cell_entry e;
+        e.x = p.x;
+        e.y = p.y;
+        e.z = p.z;
+        e.idx = i;
+        e.type = (uint32_t)p.w;
+        entries[slot] = e;
+        cell_idx[slot] = i;
+        if (rf.rel)
+        {
+            const uint32_t pc = pack_cell(c % rf.dim[0], (c / rf.dim[0]) % rf.dim[1],
+                                          c / (rf.dim[0] * rf.dim[1]), e.type);
+            const float4 rr = rel_of(p.x, p.y, p.z, pc, box, rf);
+            rf.rel[slot] = make_float3(rr.x, rr.y, rr.z);
+            rf.relw[slot] = pc;
+            const_cast<uint32_t*>(rf.pcell)[i] = pc;
+            const_cast<uint32_t*>(rf.slot_of)[i] = slot;
+        }
+    }
+}
+
+hipError_t launch_cell_rank(unsigned int N, unsigned int cmax, const double4* pos,
+                            const unsigned int* cell_size, const unsigned int* keys,
+                            const unsigned int* home, hm_cell_entry* entries,
+                            unsigned int* cell_idx, const StepState* gate, hipStream_t s,
+                            const DevBox& box, const RelFrame& rel)
+{
+    if (N == 0)
+        return hipSuccess;
+    if (!pos || !cell_size || !keys || !home || !entries || !cell_idx || cmax == 0)
+        return hipErrorInvalidValue;
+    const unsigned int blocks = min((N + 255) / 256, GATED_BLOCKS);
+    hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks), dim3(256), 0, s, N, cmax, pos, cell_size,
+                       keys, home, entries, cell_idx, gate, box, rel);
     return hipGetLastError();
 }
 
@@ -456,10 +573,11 @@ __device__ __forceinline__ void cell_centre(const DevBox& b, const unsigned int*
 // (needs >= 4 cells per axis); otherwise every candidate is tested in FP64.
 template<bool FAST, bool MT, bool EXCL, class C> struct NlistShared
 {
-    float4 sc[FAST ? C::CAP + 64 : 1];  // + far-away padding up to a whole chunk of 64
+    static_assert(C::CAP % 64 == 0, "a pass padded to whole chunks of 64 must fit the stage");
+    float4 sc[FAST ? C::CAP : 1];
     double sx[FAST ? 1 : C::CAP], sy[FAST ? 1 : C::CAP], sz[FAST ? 1 : C::CAP];
     unsigned int sj[FAST ? 1 : C::CAP];
-    unsigned char stp[MT ? C::CAP + 64 : 1];
+    unsigned char stp[MT ? C::CAP : 1];
     float4 sh[C::HOME];
     unsigned char shtp[MT ? C::HOME : 1];
     unsigned int sslot[C::CAP];  // slot mode: candidate slot | offset code
@@ -608,7 +726,7 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             unsigned short* lrow = a.lrow ? a.lrow + (size_t)i * a.nmax : nullptr;
             if (FAST)
             {
-                // NLU chunks of 64 candidates per iteration: the LDS reads and distance tests of
+                // C::U chunks of 64 candidates per iteration: the LDS reads and distance tests of
                 // the chunks are independent, the rare guard-band re-decisions are taken out of
                 // the main path, then the hits are compacted chunk by chunk in candidate order
                 auto chunks = [&](auto uc, unsigned int b) {
@@ -676,10 +794,10 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                         cnt += (unsigned int)__popcll(m);
                     }
                 };
-                // whole groups of NLU chunks, then the remaining chunks one at a time
+                // whole groups of C::U chunks, then the remaining chunks one at a time
                 unsigned int b = 0;
-                for (; b + 64u * NLU <= total; b += 64u * NLU)
-                    chunks(std::integral_constant<int, NLU>(), b);
+                for (; b + 64u * C::U <= total; b += 64u * C::U)
+                    chunks(std::integral_constant<int, C::U>(), b);
                 for (; b < total; b += 64u)
                     chunks(std::integral_constant<int, 1>(), b);
             }
@@ -741,7 +859,8 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
 }
 
 template<bool FAST, bool MT, bool EXCL, class C>
-__global__ __launch_bounds__(C::B) void nlist_kernel(NlistKernelArgs a)
+__global__ __launch_bounds__(C::B) __attribute__((amdgpu_waves_per_eu(C::WAVES)))
+void nlist_kernel(NlistKernelArgs a)
 {
     __shared__ NlistShared<FAST, MT, EXCL, C> S;
     if (gate_closed(a.gate))
@@ -790,6 +909,7 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
     // small cells: a few particles per cell on average, and every home cell fits the
     // one-wavefront workgroup (cmax <= 64)
     const bool small = a.cmax <= (unsigned int)NlSmall::HOME && a.N <= 8ull * a.ncell;
+    const bool tiny = small && a.N <= 4ull * a.ncell;
     const unsigned int grid_small = round_up(min(a.ncell, 4 * GATED_BLOCKS), 8);
     const bool mt = a.ntypes > 1;
     const bool ex = a.n_excl != nullptr && a.max_excl > 0;
@@ -812,7 +932,9 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
             a.rlistsq_f[q] = (float)rl2;
             a.guard[q] = (float)(4.0 * (2.0 * sqrt(3.0) * rl * e_dx + 6.0 * u * rl2) + 1e-12);
         }
-        if (small)
+        if (tiny)
+            nlist_dispatch<true, NlTiny>(a, mt, ex, grid_small, s);
+        else if (small)
             nlist_dispatch<true, NlSmall>(a, mt, ex, grid_small, s);
         else if (a.cmax <= (unsigned int)NlMid::HOME)
             nlist_dispatch<true, NlMid>(a, mt, ex, grid, s);
