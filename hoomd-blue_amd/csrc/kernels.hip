@@ -253,14 +253,19 @@ constexpr int NLHOME = 256;
 // Workgroup shapes of the list build. Large cells (tens of particles): four wavefronts share a
 // home cell's candidates. Small cells (a few particles, short cutoffs such as WCA): one
 // wavefront per workgroup, a smaller stage and two chunks per sweep iteration, so that more
-// cells are in flight per CU.
+// cells are in flight per CU. WAVES > 1 asks the compiler for that many wavefronts per SIMD
+// (it limits the registers; the build waits mostly on memory, so the extra wavefronts in
+// flight outweigh the few spills).
 struct NlBig
 {
     static constexpr int B = 256, CAP = NLCAP, HOME = NLHOME, U = NLU, WAVES = 1;
 };
+#ifndef HM_MID_WAVES
+#define HM_MID_WAVES 8
+#endif
 struct NlMid
 {
-    static constexpr int B = 256, CAP = NLCAP, HOME = 64, U = NLU, WAVES = 1;
+    static constexpr int B = 256, CAP = NLCAP, HOME = 64, U = NLU, WAVES = HM_MID_WAVES;
 };
 struct NlSmall
 {
@@ -547,6 +552,49 @@ __device__ __forceinline__ void load_exclusions(const NlistKernelArgs& a, unsign
         ex3 = er[3];
 }
 
+// The k-th smallest (k = 0, 1, 2) of the wrapped coordinates h - 1, h, h + 1 on an axis of
+// d >= 3 cells; returns its offset from h and sets n to the coordinate.
+__device__ __forceinline__ int sorted_neighbour(int h, int d, int k, int& n)
+{
+    int off;
+    if (h == 0)  // h - 1 wraps to d - 1, the largest
+        off = k == 0 ? 0 : (k == 1 ? 1 : -1);
+    else if (h == d - 1)  // h + 1 wraps to 0, the smallest
+        off = k == 0 ? 1 : (k == 1 ? -1 : 0);
+    else
+        off = k - 1;
+    n = h + off;
+    if (n < 0)
+        n += d;
+    else if (n >= d)
+        n -= d;
+    return off;
+}
+
+// load_exclusions without the count-to-row dependency: the row's first four entries (within the
+// row's max_excl) are read together with the count, then masked by it
+__device__ __forceinline__ void load_exclusions_flat(const NlistKernelArgs& a, unsigned int i,
+                                                     unsigned int& ex0, unsigned int& ex1,
+                                                     unsigned int& ex2, unsigned int& ex3)
+{
+    const unsigned int* er = a.excl + (size_t)i * a.max_excl;
+    const unsigned int cnt = a.n_excl[i];
+    unsigned int v0 = 0xffffffffu, v1 = 0xffffffffu, v2 = 0xffffffffu, v3 = 0xffffffffu;
+    if (a.max_excl > 0)
+        v0 = er[0];
+    if (a.max_excl > 1)
+        v1 = er[1];
+    if (a.max_excl > 2)
+        v2 = er[2];
+    if (a.max_excl > 3)
+        v3 = er[3];
+    const unsigned int ne = min(cnt, a.max_excl);
+    ex0 = ne > 0 ? v0 : 0xffffffffu;
+    ex1 = ne > 1 ? v1 : 0xffffffffu;
+    ex2 = ne > 2 ? v2 : 0xffffffffu;
+    ex3 = ne > 3 ? v3 : 0xffffffffu;
+}
+
 // Minimum-image offset of a position from a cell centre, rounded to FP32 (dims >= 4 keeps every
 // staged particle within 1.5 cell widths of the centre, well inside half a box).
 __device__ __forceinline__ float3 rel_to_centre(const DevBox& b, double x, double y, double z,
@@ -591,43 +639,37 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                            NlistShared<FAST, MT, EXCL, C>& S)
 {
 #pragma clang fp contract(off)
-    const unsigned int nhome = min(a.cell_size[cell], a.cmax);
-    if (nhome == 0)
-        return;
     const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // one-wavefront workgroups (cmax <= 64): lane s loads slot s of the home cell together with
+    // the cell's size and the adjacent cells' sizes, so the three loads overlap (slots at or past
+    // the size are read and ignored)
+    constexpr bool ONE_WAVE = C::B == 64;
+    const unsigned int nraw = a.cell_size[cell];
+    hm_cell_entry h1{};
+    if (ONE_WAVE && tid < a.cmax)
+        h1 = a.entries[(size_t)cell * a.cmax + tid];
     double cx = 0, cy = 0, cz = 0;
     if (FAST)
         cell_centre(a.box, a.dim, cell, cx, cy, cz);
     if (a.n_adj == 27 && a.dim[0] >= 3 && a.dim[1] >= 3 && a.dim[2] >= 3)
     {
-        // the 27 adjacent cells computed from the home cell's coordinates and put in
-        // increasing index order (the order of cell_adj), which saves a dependent load
-        if (tid < 64)
+        // the 27 adjacent cells computed from the home cell's coordinates, in increasing index
+        // order (the order of cell_adj), which saves a dependent load. The index order is the
+        // lexicographic order of (z, y, x), so lane t takes the (t % 3)-th smallest x, the
+        // ((t / 3) % 3)-th smallest y and the (t / 9)-th smallest z of the wrapped neighbours.
+        if (tid < 27)
         {
             const int dx = (int)a.dim[0], dy = (int)a.dim[1], dz = (int)a.dim[2];
             const int hx = (int)(cell % a.dim[0]), hy = (int)((cell / a.dim[0]) % a.dim[1]),
                       hz = (int)(cell / (a.dim[0] * a.dim[1]));
-            unsigned int c = 0xffffffffu, code = 0;
-            if (tid < 27)
-            {
-                const int ox = (int)(tid % 3) - 1, oy = (int)((tid / 3) % 3) - 1,
-                          oz = (int)(tid / 9) - 1;
-                const int nx = (hx + ox + dx) % dx, ny = (hy + oy + dy) % dy,
-                          nz = (hz + oz + dz) % dz;
-                c = (unsigned int)(nx + dx * (ny + dy * nz));
-                if (a.offset_codes)
-                    code = offset_code(-ox, -oy, -oz);
-            }
-            unsigned int rank = 0;
-#pragma unroll
-            for (int p = 0; p < 27; p++)
-                rank += __builtin_amdgcn_readlane(c, p) < c ? 1u : 0u;
-            if (tid < 27)
-            {
-                S.acell[rank] = c;
-                S.asize[rank] = min(a.cell_size[c], a.cmax);
-                S.acode[rank] = code;
-            }
+            int nx, ny, nz;
+            const int ox = sorted_neighbour(hx, dx, (int)(tid % 3), nx);
+            const int oy = sorted_neighbour(hy, dy, (int)((tid / 3) % 3), ny);
+            const int oz = sorted_neighbour(hz, dz, (int)(tid / 9), nz);
+            const unsigned int c = (unsigned int)(nx + dx * (ny + dy * nz));
+            S.acell[tid] = c;
+            S.asize[tid] = min(a.cell_size[c], a.cmax);
+            S.acode[tid] = a.offset_codes ? offset_code(-ox, -oy, -oz) : 0u;
         }
     }
     else if (tid < a.n_adj)
@@ -648,9 +690,18 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         }
         S.acode[tid] = code;
     }
+    const unsigned int nhome = min(nraw, a.cmax);
+    if (nhome == 0)
+        return;
+    // one-wavefront workgroups: lane s loads home particle s's exclusions here, where their
+    // latency overlaps the staging, and the sweep reads them from lane s
+    constexpr bool LANE_EXCL = EXCL && ONE_WAVE;
+    unsigned int hx0 = 0xffffffffu, hx1 = 0xffffffffu, hx2 = 0xffffffffu, hx3 = 0xffffffffu;
     for (unsigned int s = tid; s < nhome; s += C::B)
     {
-        const hm_cell_entry h = a.entries[(size_t)cell * a.cmax + s];
+        const hm_cell_entry h = ONE_WAVE ? h1 : a.entries[(size_t)cell * a.cmax + s];
+        if (LANE_EXCL)
+            load_exclusions_flat(a, h.idx, hx0, hx1, hx2, hx3);
         if (FAST)
         {
             const float3 r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
@@ -711,7 +762,14 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             const unsigned int ti = MT ? S.shtp[s] : 0u;
             unsigned int ex0 = 0xffffffffu, ex1 = 0xffffffffu, ex2 = 0xffffffffu,
                          ex3 = 0xffffffffu;
-            if (EXCL)
+            if (LANE_EXCL)
+            {
+                ex0 = __builtin_amdgcn_readlane(hx0, s);
+                ex1 = __builtin_amdgcn_readlane(hx1, s);
+                ex2 = __builtin_amdgcn_readlane(hx2, s);
+                ex3 = __builtin_amdgcn_readlane(hx3, s);
+            }
+            else if (EXCL)
                 load_exclusions(a, i, ex0, ex1, ex2, ex3);
             double hx = 0, hy = 0, hz = 0;
             if (!FAST)
