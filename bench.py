@@ -230,7 +230,7 @@ def kg_melt(nchain=20000, length=50, rho=0.85):
 
 
 def kg(steps, warmup, seed, cpu_steps):
-    """SURVEY config C2 production step: 20 000 chains x 50 beads at rho 0.85, FENE (k 30,
+    """Kremer-Grest melt production step: 20 000 chains x 50 beads at rho 0.85, FENE (k 30,
     r0 1.5) + WCA bonds, WCA pairs (r_cut 2^(1/6), mode shift, bond exclusions, buffer 0.4),
     Langevin kT 1, gamma 1, dt 0.01. Prints one JSON line."""
     import hoomd_amd as hoomd
