@@ -84,6 +84,10 @@ struct ForceSlot
     unsigned int nbt = 1;
     hm_fene_params fp[HM_MAX_TYPES] = {};
     unsigned int* flag = nullptr;
+    // partners (and types, with several bond types) in the current index order, [maxb * N]
+    unsigned int* bpart = nullptr;
+    unsigned int* bptype = nullptr;
+    unsigned int maxb = 0;
 };
 bool is_pair(const ForceSlot& f)
 {
@@ -204,6 +208,8 @@ struct hm_system
             dfree(f.row);
             dfree(f.other);
             dfree(f.btype);
+            dfree(f.bpart);
+            dfree(f.bptype);
             dfree(f.flag);
             dfree(f.tab);
         }
@@ -397,6 +403,15 @@ struct hm_system
         nmax = m;
         alloc_rows();
         clear_graphs();
+    }
+
+    void update_bond_index()
+    {
+        for (auto& f : forces)
+            if (f.kind == 1 && f.bpart)
+                hchk(launch_bond_index(N, tag, rtag, f.row, f.other, f.btype, f.maxb, f.bpart,
+                                       f.bptype, stream),
+                     "bond index");
     }
 
     void update_exclusion_index()
@@ -673,6 +688,9 @@ struct hm_system
                 k.virial = f.v;
                 k.flag = f.flag;
                 k.st = st;
+                k.bpart = f.bpart;
+                k.bptype = f.bptype;
+                k.max_bonds = f.maxb;
                 hchk(launch_bond_fene(k, f.fp, stream), "bond force");
             }
         }
@@ -911,6 +929,7 @@ struct hm_system
         s0.n_builds = rcut_max > 0.0 ? 1 : 0;
         hchk(hipMemcpyAsync(st, &s0, sizeof(StepState), hipMemcpyHostToDevice, stream), "state");
         update_exclusion_index();
+        update_bond_index();
         if (rcut_max > 0.0)
             build_nlist_now();
         if (autotune && !pair_tuned)
@@ -967,6 +986,7 @@ struct hm_system
         permute(last, 24);
         hchk(launch_rtag(N, tag, rtag, stream), "rtag");
         update_exclusion_index();
+        update_bond_index();
         hchk(launch_set_rebuild(st, stream), "force rebuild");
         n_sorts++;
         last_sort_step = timestep;
@@ -1016,6 +1036,7 @@ struct hm_system
         hchk(hipMemcpyAsync(st, &s, sizeof(StepState), hipMemcpyHostToDevice, stream), "restore");
         hchk(launch_rtag(N, tag, rtag, stream), "rtag");
         update_exclusion_index();
+        update_bond_index();
         hchk(hipStreamSynchronize(stream), "restore sync");
     }
 
@@ -1377,6 +1398,13 @@ int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond
             mx = std::max(mx, deg[i]);
         if (mx > 4)
             throw HmError{HM_ERR_ARGS, "at most 4 bonds per particle are supported"};
+        if (mx > 0)
+        {
+            f.maxb = mx;
+            f.bpart = dalloc<unsigned int>((size_t)N * mx);
+            if (nbond_types > 1)
+                f.bptype = dalloc<unsigned int>((size_t)N * mx);
+        }
         if (mx > 0)
         {
             std::vector<unsigned int> ne(N, 0), ex((size_t)N * mx, 0xffffffffu);

@@ -349,8 +349,20 @@ struct BondKernelArgs
     float* virial;
     unsigned int* flag;
     StepState* st;                  // error bit on stretched bonds (may be NULL)
+    // optional partner table in the current index order (launch_bond_index): bond k of particle
+    // i is bpart[k * N + i] (NO_SLOT past its bonds), of type bptype[k * N + i]; when set, tag,
+    // rtag, row, other_tag and btype are not read
+    const unsigned int* bpart;
+    const unsigned int* bptype;
+    unsigned int max_bonds;         // bonds per particle in bpart (at most 4)
 };
 hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* params, hipStream_t s);
+// the partner table of BondKernelArgs from the CSR-by-tag bonds; rebuilt whenever the particle
+// order changes (sort, restore)
+hipError_t launch_bond_index(unsigned int N, const unsigned int* tag, const unsigned int* rtag,
+                             const unsigned int* row, const unsigned int* other_tag,
+                             const unsigned int* btype, unsigned int max_bonds,
+                             unsigned int* bpart, unsigned int* bptype, hipStream_t s);
 
 // ---------------------------------------------------------------- integration
 struct StepOneArgs

@@ -2330,15 +2330,51 @@ __global__ __launch_bounds__(256) void bond_fene_kernel(BondKernelArgs a,
     unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
-    const unsigned int tg = a.tag[i];
-    const unsigned int b0 = a.row[tg], b1 = a.row[tg + 1];
+    // partners in bond order: from the index-ordered table (all of its loads issued together),
+    // else through tag -> CSR row -> partner tag -> index
+    constexpr unsigned int MAXB = 4;
+    unsigned int part[MAXB], ptype[MAXB];
+    unsigned int nb = 0;
+    if (a.bpart)
+    {
+        nb = a.max_bonds;
+#pragma unroll
+        for (unsigned int k = 0; k < MAXB; k++)
+        {
+            part[k] = k < nb ? a.bpart[(size_t)k * a.N + i] : NO_SLOT;
+            ptype[k] = (k < nb && a.bptype) ? a.bptype[(size_t)k * a.N + i] : 0u;
+        }
+    }
+    const unsigned int tg = a.bpart ? 0u : a.tag[i];
+    const unsigned int b0 = a.bpart ? 0u : a.row[tg];
+    const unsigned int b1 = a.bpart ? nb : a.row[tg + 1];
     const double4 pi = a.pos[i];
     float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
     float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool bad = false;
     for (unsigned int b = b0; b < b1; b++)
     {
-        const unsigned int j = a.rtag[a.other_tag[b]];
+        unsigned int j, bt;
+        if (a.bpart)
+        {
+            // constant indices into part / ptype keep them in registers
+            j = part[0];
+            bt = ptype[0];
+#pragma unroll
+            for (unsigned int k = 1; k < MAXB; k++)
+                if (b == k)
+                {
+                    j = part[k];
+                    bt = ptype[k];
+                }
+            if (j == NO_SLOT)
+                break;
+        }
+        else
+        {
+            j = a.rtag[a.other_tag[b]];
+            bt = a.btype ? a.btype[b] : 0u;
+        }
         const double4 pj = a.pos[j];
         double dx, dy, dz, rsq;
         {
@@ -2349,7 +2385,6 @@ __global__ __launch_bounds__(256) void bond_fene_kernel(BondKernelArgs a,
             min_image(a.box, dx, dy, dz);
             rsq = dx * dx + dy * dy + dz * dz;
         }
-        const unsigned int bt = a.btype ? a.btype[b] : 0u;
         EvaluatorBondFENE<float> ev((float)rsq, tab.p[bt]);
         float fdr = 0.f, e = 0.f;
         if (!ev.evalForceAndEnergy(fdr, e))
@@ -2396,7 +2431,46 @@ hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* param
     for (unsigned int q = 0; q < nt; q++)
         tab.p[q] = E::param_type::make(params[q].k, params[q].r0, params[q].epsilon,
                                        params[q].sigma, params[q].delta);
+    if (a.bpart && a.max_bonds > 4)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(bond_fene_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a, tab);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void bond_index_kernel(unsigned int N,
+                                                         const unsigned int* __restrict__ tag,
+                                                         const unsigned int* __restrict__ rtag,
+                                                         const unsigned int* __restrict__ row,
+                                                         const unsigned int* __restrict__ other,
+                                                         const unsigned int* __restrict__ btype,
+                                                         unsigned int max_bonds,
+                                                         unsigned int* bpart, unsigned int* bptype)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const unsigned int tg = tag[i];
+    const unsigned int b0 = row[tg], deg = row[tg + 1] - b0;
+    for (unsigned int k = 0; k < max_bonds; k++)
+    {
+        const bool has = k < deg;
+        bpart[(size_t)k * N + i] = has ? rtag[other[b0 + k]] : NO_SLOT;
+        if (bptype)
+            bptype[(size_t)k * N + i] = has && btype ? btype[b0 + k] : 0u;
+    }
+}
+
+hipError_t launch_bond_index(unsigned int N, const unsigned int* tag, const unsigned int* rtag,
+                             const unsigned int* row, const unsigned int* other_tag,
+                             const unsigned int* btype, unsigned int max_bonds,
+                             unsigned int* bpart, unsigned int* bptype, hipStream_t s)
+{
+    if (N == 0 || max_bonds == 0)
+        return hipSuccess;
+    if (max_bonds > 4 || !tag || !rtag || !row || !other_tag || !bpart)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bond_index_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, tag, rtag,
+                       row, other_tag, btype, max_bonds, bpart, bptype);
     return hipGetLastError();
 }
 

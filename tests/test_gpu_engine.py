@@ -175,7 +175,10 @@ def test_thermo_matches_oracle(hoomd):
     assert abs((got[0] + got[3] + got[5]) / 3 - thermo.pressure) < 1e-9 * abs(P) + 1e-12
 
 
-def test_langevin_bead_spring_matches_oracle(hoomd):
+@pytest.mark.parametrize("use_sorter", [False, True])
+def test_langevin_bead_spring_matches_oracle(hoomd, use_sorter):
+    """With the sorter the particle order changes every 5 steps, so the bond kernel's
+    index-ordered partner table is rebuilt from the tags four times in the run."""
     rng = np.random.default_rng(5)
     nch, M = 120, 10  # chains (9.7 long) shorter than the box (11.2)
     N = nch * M
@@ -197,11 +200,14 @@ def test_langevin_bead_spring_matches_oracle(hoomd):
     vel = maxwell_boltzmann(N, 1.0, seed=5)
     fene = dict(k=30.0, r0=1.5, epsilon=1.0, sigma=1.0, delta=0.0)
     sim, lj = make_sim(hoomd, pos, vel, L, r_cut=2 ** (1 / 6), mode="shift", dt=0.002,
-                       langevin=(1.0, 1.0), seed=11, bonds=np.array(bonds), fene=fene)
+                       langevin=(1.0, 1.0), seed=11, bonds=np.array(bonds), fene=fene,
+                       sorter=(5, 16) if use_sorter else None)
     sim.run(20)
     ref = O.System(pos, vel, L, r_cut=2 ** (1 / 6), mode="shift", dt=0.002, seed=11)
     ref.set_langevin(1.0, 1.0)
     ref.set_bonds(np.array(bonds), **fene)
+    if use_sorter:
+        ref.set_sorter(5, 16)
     ref.run(20)
     rp, rv, ri = ref.state()
     snap = sim.state.get_snapshot()
