@@ -753,12 +753,20 @@ struct hm_system
         {
             if (!fused_clear)
                 hchk(launch_cell_clear(ncell, cell_size, st, stream), "cell clear");
-            hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, nullptr, last,
-                                   &cond->cell_need, st, stream, ckey, chome),
-                 "cell build");
-            hchk(launch_cell_rank(N, cmax, pos, cell_size, ckey, chome, entries, cell_idx, st,
-                                  stream, box, rel_frame()),
-                 "cell order");
+            if (N <= CELL_FUSED_MAX_N)
+                hchk(launch_cell_bin_rank(N, dim, cmax, pos, cell_size, last, &cond->cell_need,
+                                          ckey, chome, entries, cell_idx, st, stream, box,
+                                          rel_frame()),
+                     "cell build and order");
+            else
+            {
+                hchk(launch_cell_build(N, pos, box, dim, cmax, cell_size, nullptr, last,
+                                       &cond->cell_need, st, stream, ckey, chome),
+                     "cell build");
+                hchk(launch_cell_rank(N, cmax, pos, cell_size, ckey, chome, entries, cell_idx,
+                                      st, stream, box, rel_frame()),
+                     "cell order");
+            }
             NlistKernelArgs k;
             fill_nlist_args(k, true);
             hchk(launch_nlist_build(k, stream), "nlist build");

@@ -233,6 +233,15 @@ hipError_t launch_cell_rank(unsigned int N, unsigned int cmax, const double4* po
                             const unsigned int* home, hm_cell_entry* entries,
                             unsigned int* cell_idx, const StepState* gate, hipStream_t s,
                             const DevBox& box, const RelFrame& rel);
+// launch_cell_build (key table form) and launch_cell_rank in one single-workgroup launch, for
+// systems of at most CELL_FUSED_MAX_N particles
+constexpr unsigned int CELL_FUSED_MAX_N = 4096;
+hipError_t launch_cell_bin_rank(unsigned int N, const unsigned int dim[3], unsigned int cmax,
+                                const double4* pos, unsigned int* cell_size, d3* last_pos,
+                                unsigned int* cond3, unsigned int* keys, unsigned int* home,
+                                hm_cell_entry* entries, unsigned int* cell_idx,
+                                const StepState* gate, hipStream_t s, const DevBox& box,
+                                const RelFrame& rel);
 hipError_t launch_nlist_build(const NlistKernelArgs& a, hipStream_t s);
 
 hipError_t launch_nlist_check(unsigned int N, const double4* pos, const d3* last,
