@@ -461,6 +461,15 @@ struct hm_system
         k.counter = st;
         k.check_delay = check_delay;
         k.dist_check = dist_check;
+        if (uses_rel())
+        {
+            // the cell ordering of this build wrote the relative positions the list stages
+            k.srel = rel;
+            k.sidx = cell_idx;
+            k.swrd = relw;
+            for (int d = 0; d < 3; d++)
+                k.cwf[d] = (float)(box.L[d] / (double)dim[d]);
+        }
     }
 
     bool rel_ok() const

@@ -188,6 +188,14 @@ struct NlistKernelArgs
     StepState* counter;       // n_builds and the rebuild statistics updated by block 0
     unsigned int check_delay;
     int dist_check;
+    // FP32 staging (the engine with the relative-position pair kernel): the build's cell-slot
+    // relative positions, slot -> particle index and packed cell + type. With 27 adjacent cells
+    // and the FP32 test, candidates are staged from these (16 bytes, three FP32 adds) instead of
+    // from entries (32 bytes, an FP64 minimum image); the guard band widens to match.
+    const float3* srel;
+    const unsigned int* sidx;
+    const uint32_t* swrd;
+    float cwf[3];             // cell widths rounded to FP32
 };
 
 // Cell-tiled pair kernel: neighbours read from the home cell's LDS-staged candidates through

@@ -695,6 +695,7 @@ template<bool FAST, bool MT, bool EXCL, class C> struct NlistShared
     unsigned char shtp[MT ? C::HOME : 1];
     unsigned int sslot[C::CAP];  // slot mode: candidate slot | offset code
     unsigned int acell[27], asize[27], aoff[27], acode[27];
+    float4 ashift[FAST ? 27 : 1];  // FP32 staging: offset of each adjacent cell's origin from the home centre
     unsigned int hcnt[C::HOME];
     unsigned int pass_last, pass_total;
 };
@@ -709,10 +710,25 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
     // the cell's size and the adjacent cells' sizes, so the three loads overlap (slots at or past
     // the size are read and ignored)
     constexpr bool ONE_WAVE = C::B == 64;
+    // FP32 staging from the relative positions (uniform over the workgroup)
+    const bool rs = FAST && a.srel != nullptr && a.n_adj == 27;
     const unsigned int nraw = a.cell_size[cell];
     hm_cell_entry h1{};
+    float3 hr1 = make_float3(0.f, 0.f, 0.f);
+    unsigned int hi1 = 0, hw1 = 0;
     if (ONE_WAVE && tid < a.cmax)
-        h1 = a.entries[(size_t)cell * a.cmax + tid];
+    {
+        const size_t sl = (size_t)cell * a.cmax + tid;
+        if (rs)
+        {
+            hr1 = a.srel[sl];
+            hi1 = a.sidx[sl];
+            if (MT)
+                hw1 = a.swrd[sl];
+        }
+        else
+            h1 = a.entries[sl];
+    }
     double cx = 0, cy = 0, cz = 0;
     if (FAST)
         cell_centre(a.box, a.dim, cell, cx, cy, cz);
@@ -735,6 +751,10 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
             S.acell[tid] = c;
             S.asize[tid] = min(a.cell_size[c], a.cmax);
             S.acode[tid] = a.offset_codes ? offset_code(-ox, -oy, -oz) : 0u;
+            if (FAST)
+                S.ashift[tid] = make_float4(((float)ox - 0.5f) * a.cwf[0],
+                                            ((float)oy - 0.5f) * a.cwf[1],
+                                            ((float)oz - 0.5f) * a.cwf[2], 0.f);
         }
     }
     else if (tid < a.n_adj)
@@ -764,18 +784,39 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
     unsigned int hx0 = 0xffffffffu, hx1 = 0xffffffffu, hx2 = 0xffffffffu, hx3 = 0xffffffffu;
     for (unsigned int s = tid; s < nhome; s += C::B)
     {
-        const hm_cell_entry h = ONE_WAVE ? h1 : a.entries[(size_t)cell * a.cmax + s];
-        if (LANE_EXCL)
-            load_exclusions_flat(a, h.idx, hx0, hx1, hx2, hx3);
-        if (FAST)
+        unsigned int hidx, htype;
+        float3 r = make_float3(0.f, 0.f, 0.f);
+        if (rs)
         {
-            const float3 r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
-            S.sh[s] = make_float4(r.x, r.y, r.z, __uint_as_float(h.idx));
+            // the home cell's own shift (offset 0), the same sum as its candidates get
+            float3 rr = hr1;
+            unsigned int w = hw1;
+            hidx = hi1;
+            if (!ONE_WAVE)
+            {
+                const size_t sl = (size_t)cell * a.cmax + s;
+                rr = a.srel[sl];
+                hidx = a.sidx[sl];
+                w = MT ? a.swrd[sl] : 0u;
+            }
+            const float sx0 = (0.f - 0.5f) * a.cwf[0], sy0 = (0.f - 0.5f) * a.cwf[1],
+                        sz0 = (0.f - 0.5f) * a.cwf[2];
+            r = make_float3(rr.x + sx0, rr.y + sy0, rr.z + sz0);
+            htype = w >> 24;
         }
         else
-            S.sh[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(h.idx));
+        {
+            const hm_cell_entry h = ONE_WAVE ? h1 : a.entries[(size_t)cell * a.cmax + s];
+            hidx = h.idx;
+            htype = h.type;
+            if (FAST)
+                r = rel_to_centre(a.box, h.x, h.y, h.z, cx, cy, cz);
+        }
+        if (LANE_EXCL)
+            load_exclusions_flat(a, hidx, hx0, hx1, hx2, hx3);
+        S.sh[s] = make_float4(r.x, r.y, r.z, __uint_as_float(hidx));
         if (MT)
-            S.shtp[s] = (unsigned char)h.type;
+            S.shtp[s] = (unsigned char)htype;
         S.hcnt[s] = 0;
     }
     __syncthreads();
@@ -789,9 +830,20 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
         for (unsigned int k = tid; k < total; k += C::B)
         {
             const unsigned int q = find_segment(S.aoff, first, last, k);
-            const hm_cell_entry e = a.entries[(size_t)S.acell[q] * a.cmax + (k - S.aoff[q])];
+            const unsigned int sl = S.acell[q] * a.cmax + (k - S.aoff[q]);
             if (a.offset_codes)
-                S.sslot[k] = (S.acell[q] * a.cmax + (k - S.aoff[q])) | S.acode[q];
+                S.sslot[k] = sl | S.acode[q];
+            if (rs)
+            {
+                const float3 r = a.srel[sl];
+                const float4 sh4 = S.ashift[q];
+                S.sc[k] = make_float4(r.x + sh4.x, r.y + sh4.y, r.z + sh4.z,
+                                      __uint_as_float(a.sidx[sl]));
+                if (MT)
+                    S.stp[k] = (unsigned char)(a.swrd[sl] >> 24);
+                continue;
+            }
+            const hm_cell_entry e = a.entries[sl];
             if (FAST)
             {
                 const float3 r = rel_to_centre(a.box, e.x, e.y, e.z, cx, cy, cz);
@@ -1046,7 +1098,10 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
         for (int d = 0; d < 3; d++)
             wmax = fmax(wmax, a.box.L[d] / (double)a.dim[d]);
         const double dmax = 1.5 * wmax;
-        const double e_dx = u * 2.0 * dmax + 2.0 * u * dmax;
+        // entries: each coordinate rounded once (u * dmax), the difference once more (2 u dmax).
+        // FP32 staging: the relative position (u w), the FP32 width times (o - 1/2) (3 u w) and
+        // the sum (1.5 u w) per coordinate, twice, and the difference (3 u w): 14 u w, taken as 16
+        const double e_dx = a.srel ? 16.0 * u * wmax : u * 2.0 * dmax + 2.0 * u * dmax;
         const unsigned int nt = a.ntypes ? a.ntypes : 1;
         for (unsigned int q = 0; q < nt * nt; q++)
         {
