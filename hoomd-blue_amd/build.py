@@ -1,6 +1,6 @@
 """Build libhoomd_amd.so (gfx950) in-tree with hipcc.
 
-    python hoomd-blue_amd/build.py [--jobs N]
+    python hoomd-blue_amd/build.py [--jobs N] [--incremental]
 
 Each .hip source under csrc/ is compiled to an object in parallel, then linked into
 hoomd-blue_amd/hoomd_amd/libhoomd_amd.so, next to the Python package that loads it.
@@ -49,9 +49,22 @@ def _filter(out):
     return "\n".join(lines)
 
 
-def compile_one(src):
+def _up_to_date(src, obj):
+    """The object is newer than its source and every header of csrc/ and include/."""
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)]
+    for d in (CSRC, os.path.join(ROOT, "include")):
+        deps += [os.path.join(d, f) for f in os.listdir(d) if f.endswith(".h")]
+    return all(os.path.getmtime(p) < t for p in deps)
+
+
+def compile_one(src, incremental=False):
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if incremental and _up_to_date(src, obj):
+        return obj
     cmd = [hipcc()] + CFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
@@ -59,9 +72,11 @@ def compile_one(src):
     return obj
 
 
-def build(jobs=4, verbose=False):
+def build(jobs=4, verbose=False, incremental=False):
+    """Compile every source (or, with incremental, only those older than their inputs) and
+    link the library."""
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+        objs = list(ex.map(lambda s: compile_one(s, incremental), SOURCES))
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
@@ -75,8 +90,10 @@ def build(jobs=4, verbose=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources older than their object files")
     a = ap.parse_args()
-    build(a.jobs, verbose=True)
+    build(a.jobs, verbose=True, incremental=a.incremental)
 
 
 if __name__ == "__main__":

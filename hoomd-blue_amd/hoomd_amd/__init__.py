@@ -8,7 +8,7 @@ meanings and per-particle outputs, backed by hand-written gfx950 kernels in libh
     sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=1)
 """
 
-from . import device, filter, logging, md, trigger, tune, write  # noqa: F401
+from . import device, filter, logging, md, trigger, tune, variant, write  # noqa: F401
 from .simulation import Simulation  # noqa: F401
 from .snapshot import Snapshot  # noqa: F401
 from .state import State  # noqa: F401

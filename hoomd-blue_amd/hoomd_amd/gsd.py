@@ -6,7 +6,9 @@ directly with numpy:
 
 * a 256-byte header: magic, index location and capacity, namelist location and capacity,
   schema version, file-layer version (2.0), application and schema names;
-* an index of 32-byte entries (frame, N, location, M, name id, type, flags), sorted by frame;
+* an index of 32-byte entries (frame, N, location, M, name id, type, flags), sorted by
+  (frame, name id): the gsd library finds a chunk by binary search over that order
+  (hoomd/extern/gsd.c:780-810);
 * a namelist of null-terminated chunk names;
 * the chunk data, each an N x M array of the entry's type.
 
@@ -24,7 +26,8 @@ SCHEMA_VERSION = (1 << 16) | 4
 HEADER = struct.Struct("<QQQQQII64s64s80s")
 ENTRY = struct.Struct("<QQqIHBB")
 
-# gsd_type codes and their numpy types
+# gsd_type codes and their numpy types (hoomd/extern/gsd.h:24-51 defines 1-10; code 11, the
+# character type of later gsd releases, is read as bytes but never written)
 _TYPES = {1: np.uint8, 2: np.uint16, 3: np.uint32, 4: np.uint64, 5: np.int8, 6: np.int16,
           7: np.int32, 8: np.int64, 9: np.float32, 10: np.float64, 11: np.uint8}
 _CODES = {np.dtype(v): k for k, v in _TYPES.items() if k != 11}
@@ -216,6 +219,10 @@ class GSDWriter:
         self._f.write(blob + b"\0" * (self._names_alloc * 64 - len(blob)))
 
     def _write_index(self):
+        # gsd_find_chunk binary-searches the index by (frame, name id), so the entries are kept
+        # in that order (a name first written in a later frame, or an append to a file whose
+        # names were numbered by another writer, would otherwise leave it unsorted)
+        self._entries.sort(key=lambda e: (e[0], e[4]))
         if len(self._entries) > self._index_alloc:
             while len(self._entries) > self._index_alloc:
                 self._index_alloc *= 2
@@ -232,7 +239,8 @@ class GSDWriter:
         if name not in self._names:
             self._names.append(name)
             self._write_names()
-        typ = 11 if name.endswith("/types") else _CODES[a.dtype]
+        # type names are written as uint8 rows, as GSDDumpWriter does (GSDDumpWriter.cc:436-442)
+        typ = 1 if name.endswith("/types") else _CODES[a.dtype]
         N = a.shape[0] if a.ndim else 1
         M = int(np.prod(a.shape[1:])) if a.ndim > 1 else 1
         self._f.seek(0, 2)

@@ -4,7 +4,8 @@ buffer for the largest measured timesteps per second."""
 import time
 
 from ..trigger import as_trigger
-from ..tune.solve import GridOptimizer, ManualTuneDefinition, Optimizer
+from ..tune.solve import GradientDescent, GridOptimizer, ManualTuneDefinition, Optimizer
+from ..variant import Ramp
 
 
 class NeighborListBuffer:
@@ -44,9 +45,14 @@ class NeighborListBuffer:
                    maximum_buffer=maximum_buffer, minimum_buffer=minimum_buffer)
 
     @classmethod
-    def with_gradient_descent(cls, *args, **kwargs):
-        raise NotImplementedError("NeighborListBuffer.with_gradient_descent is not provided; "
-                                  "use NeighborListBuffer.with_grid")
+    def with_gradient_descent(cls, trigger, nlist, maximum_buffer, minimum_buffer=0.0,
+                              alpha=Ramp(1e-5, 1e-6, 0, 30), kappa=(0.33, 0.165), tol=1e-5,
+                              max_delta=None):
+        """The tuner with a GradientDescent solver maximising TPS (nlist_buffer.py:242-305).
+        The domain is (minimum_buffer, maximum_buffer); alpha may vary with the number of
+        solver calls."""
+        return cls(trigger, nlist, GradientDescent(alpha, kappa, tol, True, max_delta),
+                   maximum_buffer=maximum_buffer, minimum_buffer=minimum_buffer)
 
     def _signature(self):
         return ("NeighborListBuffer", id(self))
@@ -97,7 +103,7 @@ class NeighborListBuffer:
     def final_bin(self):
         if not isinstance(self.solver, GridOptimizer):
             return None
-        return self.solver._final_bins.get(self._tunable, None)
+        return self.solver.final_bin(self._tunable)
 
     def reset(self):
         self._tuned = 0

@@ -1,10 +1,11 @@
-"""hoomd.tune subset: ParticleSorter (hoomd/tune/sorter.py), GridOptimizer and
+"""hoomd.tune subset: ParticleSorter (hoomd/tune/sorter.py), GridOptimizer, GradientDescent and
 ManualTuneDefinition (hoomd/tune/solve.py, attr_tuner.py)."""
 
 import math
 
 from ..trigger import Periodic  # noqa: F401
-from .solve import GridOptimizer, ManualTuneDefinition, Optimizer, SolverStep  # noqa: F401
+from .solve import (GradientDescent, GridOptimizer, ManualTuneDefinition, Optimizer,  # noqa: F401
+                    SolverStep)
 
 
 class ParticleSorter:

@@ -201,7 +201,7 @@ def test_grid_optimizer_closed_form():
             break
     assert done
     np.testing.assert_allclose(seen[:5], [0.1, 0.3, 0.5, 0.7, 0.9])
-    lo, hi = opt._final_bins[tun]
+    lo, hi = opt.final_bin(tun)
     assert lo <= 0.31 <= hi and abs((hi - lo) - 0.04) < 1e-12
     assert state["x"] == (lo + hi) / 2
     assert opt.solve([tun])  # stays solved
@@ -209,7 +209,118 @@ def test_grid_optimizer_closed_form():
     opt2 = GridOptimizer()
     none = ManualTuneDefinition(get_y=lambda: None, target=0.0, get_x=lambda: 0.0,
                                 set_x=lambda v: None, domain=(0.0, 1.0))
-    assert not opt2.solve([none]) and none not in opt2._bins
+    assert not opt2.solve([none]) and not opt2.is_searching(none)
     # x is clamped into the domain
     tun.x = 5.0
     assert state["x"] == 1.0
+
+
+def _index_entries(fn):
+    with open(fn, "rb") as f:
+        d = f.read()
+    (_, index_loc, index_n, names_loc, names_n, *_r) = gsd.HEADER.unpack_from(d, 0)
+    names = [n.decode() for n in d[names_loc:names_loc + names_n * 64].split(b"\0")]
+    out = []
+    for k in range(index_n):
+        frame, N, loc, M, nid, typ, _ = gsd.ENTRY.unpack_from(d, index_loc + k * gsd.ENTRY.size)
+        if loc == 0:
+            break
+        out.append((frame, nid, names[nid], typ))
+    return out
+
+
+def test_written_index_is_readable_by_the_gsd_library(tmp_path):
+    """The index must be sorted by (frame, name id), the order gsd_find_chunk binary-searches
+    (hoomd/extern/gsd.c:780-810), and every type code must be one the reference's gsd defines
+    (1-10, hoomd/extern/gsd.h:24-51); type names are uint8 rows (GSDDumpWriter.cc:436-442).
+    Cases: a chunk first written in frame 1 (velocity becomes non-default only later, with the
+    momentum category dynamic) and an append to the reference-written spheres.gsd."""
+    import shutil
+    s = gsd.read_snapshot(SPHERES)
+    fn = str(tmp_path / "b.gsd")
+    w = gsd.GSDWriter(fn, "wb", dynamic=("property", "momentum"))
+    w.write_frame(s, 0)
+    s.particles.velocity[:] = 0.5
+    w.write_frame(s, 10)
+    w.close()
+    ap = str(tmp_path / "spheres_append.gsd")
+    shutil.copy(SPHERES, ap)
+    gsd.write_frames(ap, [(s, 20), (s, 30)], mode="ab", dynamic=("property", "momentum"))
+    for f in (fn, ap):
+        ent = _index_entries(f)
+        keys = [(fr, nid) for fr, nid, _, _ in ent]
+        assert keys == sorted(keys), f
+        assert all(1 <= typ <= 10 for _, _, _, typ in ent), f
+        assert all(typ == 1 for _, _, name, typ in ent if name.endswith("/types")), f
+        assert len(set(keys)) == len(keys)
+    g = gsd.GSDFile(fn)
+    assert "particles/velocity" not in g.chunk_names(0)
+    assert "particles/velocity" in g.chunk_names(1)
+    assert gsd.read_snapshot(fn, 1).particles.velocity[0, 0] == np.float32(0.5)
+    g = gsd.GSDFile(ap)
+    assert len(g) == 3
+    t = gsd.read_snapshot(ap, 2)
+    np.testing.assert_array_equal(t.particles.velocity, s.particles.velocity)
+    assert t.particles.types == ["A"]
+
+
+def test_gradient_descent_finds_the_maximum():
+    """GradientDescent (solve.py:320-480) maximising -(x - 0.31)^2 + 2 from x = 0.1 with the
+    momentum weights kappa: a forward-difference gradient and the damped step converge to the
+    maximum within tol; the first call perturbs x by 10 %, and a constant y nudges x after
+    more than three unchanged measurements."""
+    from hoomd_amd.tune import GradientDescent, ManualTuneDefinition
+    from hoomd_amd.variant import Ramp
+    state = {"x": 0.1}
+    tun = ManualTuneDefinition(get_y=lambda: -(state["x"] - 0.31) ** 2 + 2, target=0.0,
+                               get_x=lambda: state["x"],
+                               set_x=lambda v: state.__setitem__("x", v), domain=(0.0, 1.0))
+    opt = GradientDescent(alpha=0.4, kappa=np.array([0.2, 0.1]), tol=1e-7, maximize=True)
+    assert not opt.solve([tun])
+    assert state["x"] == pytest.approx(0.11)
+    for _ in range(500):
+        if opt.solve([tun]):
+            break
+    assert opt.solve([tun])
+    assert abs(state["x"] - 0.31) < 1e-3
+    # minimising and a step cap
+    state["x"] = 0.9
+    opt = GradientDescent(alpha=0.5, tol=1e-8, maximize=False, max_delta=0.05)
+    tun2 = ManualTuneDefinition(get_y=lambda: (state["x"] - 0.31) ** 2, target=0.0,
+                                get_x=lambda: state["x"],
+                                set_x=lambda v: state.__setitem__("x", v), domain=(0.0, 1.0))
+    xs = []
+    for _ in range(500):
+        done = opt.solve([tun2])
+        xs.append(state["x"])
+        if done:
+            break
+    assert done and abs(state["x"] - 0.31) < 1e-3
+    assert max(abs(a - b) for a, b in zip(xs[1:], xs[2:])) <= 0.05 + 1e-12
+    # a y that does not change: x stays for three calls, then moves on by half its last change
+    state["x"] = 0.5
+    flat = ManualTuneDefinition(get_y=lambda: 1.0, target=0.0, get_x=lambda: state["x"],
+                                set_x=lambda v: state.__setitem__("x", v), domain=(0.0, 1.0))
+    opt = GradientDescent()
+    opt.solve([flat])
+    assert state["x"] == pytest.approx(0.55)
+    for _ in range(3):
+        opt.solve([flat])
+        assert state["x"] == pytest.approx(0.55)
+    opt.solve([flat])
+    assert state["x"] == pytest.approx(0.575)
+    # alpha as a variant of the number of solve calls
+    opt = GradientDescent(alpha=Ramp(1e-5, 1e-6, 0, 30))
+    assert opt.alpha == 1e-5
+    opt._calls = 15
+    assert opt.alpha == pytest.approx(5.5e-6)
+
+
+def test_variants():
+    from hoomd_amd.variant import Constant, Ramp, as_variant
+    r = Ramp(0.0, 100.0, 1000, 10000)
+    assert r(0) == 0.0 and r(1000) == 0.0 and r(6000) == 50.0 and r(11000) == 100.0
+    assert r(10 ** 9) == 100.0 and r.min == 0.0 and r.max == 100.0
+    assert as_variant(2)(5) == 2.0 and isinstance(as_variant(Constant(1)), Constant)
+    with pytest.raises(TypeError):
+        as_variant("x")
