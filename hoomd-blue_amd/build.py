@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "hoomd_amd", "libhoomd_amd.so")
 ARCH = os.environ.get("HM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "sort.hip", "capi.hip", "engine.hip"]
+SOURCES = ["kernels.hip", "sort.hip", "capi.hip", "engine.hip", "mlp.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",

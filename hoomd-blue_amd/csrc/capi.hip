@@ -281,6 +281,43 @@ int hm_pair_table(const hm_pair_args* a, const hm_table_params* params, hm_strea
     return check(launch_pair_table(k, params, cfg, (hipStream_t)stream), "hm_pair_table");
 }
 
+namespace
+{
+bool bond_args_ok(const hm_bond_args* a)
+{
+    return a && a->pos && a->tag && a->rtag && a->bond_row && a->bond_other && a->force
+           && (!a->compute_virial || a->virial) && a->nbond_types <= HM_MAX_TYPES;
+}
+
+BondKernelArgs bond_kernel_args(const hm_bond_args* a)
+{
+    BondKernelArgs k;
+    memset(&k, 0, sizeof(k));
+    k.N = a->N;
+    k.pos = (const double4*)a->pos;
+    k.box = make_devbox(a->box);
+    k.tag = a->tag;
+    k.rtag = a->rtag;
+    k.row = a->bond_row;
+    k.other_tag = a->bond_other;
+    k.btype = a->bond_type;
+    k.nbond_types = a->nbond_types ? a->nbond_types : 1;
+    k.compute_virial = a->compute_virial;
+    k.force = (float4*)a->force;
+    k.virial = a->virial;
+    k.flag = a->flag;
+    return k;
+}
+} // namespace
+
+int hm_bond_harmonic(const hm_bond_args* a, const hm_harmonic_params* params, hm_stream_t stream)
+{
+    if (!bond_args_ok(a) || !params)
+        return fail(HM_ERR_ARGS, "hm_bond_harmonic: invalid arguments");
+    return check(launch_bond_harmonic(bond_kernel_args(a), params, (hipStream_t)stream),
+                 "hm_bond_harmonic");
+}
+
 int hm_bond_fene(const hm_bond_args* a, const hm_fene_params* params, hm_stream_t stream)
 {
     if (!a || !params || !a->pos || !a->tag || !a->rtag || !a->bond_row || !a->bond_other

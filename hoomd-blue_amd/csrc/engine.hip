@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "capi_util.h"
+#include "mlp.h"
 #include "kernels.h"
 
 using namespace hm;
@@ -66,7 +67,7 @@ template<class T> T* halloc(size_t n)
 
 struct ForceSlot
 {
-    int kind = 0; // 0 LJ pair, 1 FENE bond, 2 Table pair
+    int kind = 0; // 0 LJ pair, 1 FENE bond, 2 Table pair, 3 learned CG site energy, 4 harmonic bond
     float4* f = nullptr;
     float* v = nullptr;
     // pair forces
@@ -88,10 +89,26 @@ struct ForceSlot
     unsigned int* bpart = nullptr;
     unsigned int* bptype = nullptr;
     unsigned int maxb = 0;
+    hm_harmonic_params hp[HM_MAX_TYPES] = {};
+    // learned CG site energy: FP32 parameters, bf16 operand copies, per-site densities,
+    // dE/dG and site energies; training buffers are allocated by the first training call
+    hm_mlp_cg_model mm{};
+    float* mparams = nullptr;
+    uint16_t* mbf16 = nullptr;
+    float4 *mG = nullptr, *mA = nullptr;
+    float* mE = nullptr;
+    float4 *mT = nullptr, *mErr = nullptr, *mC = nullptr;
+    float *mLoss = nullptr, *mGpart = nullptr, *mM1 = nullptr, *mM2 = nullptr, *mGrad = nullptr;
+    double* mLossOut = nullptr;
+    unsigned int adam_t = 0;
 };
 bool is_pair(const ForceSlot& f)
 {
-    return f.kind == 0 || f.kind == 2;
+    return f.kind == 0 || f.kind == 2 || f.kind == 3;
+}
+bool is_bond(const ForceSlot& f)
+{
+    return f.kind == 1 || f.kind == 4;
 }
 } // namespace
 
@@ -189,6 +206,10 @@ struct hm_system
 
     uint64_t n_sorts = 0, steps_run = 0, rollbacks = 0;
 
+    // coarse-grained site map (hm_sys_set_site_map): CSR of particle tags per site
+    unsigned int nsites = 0;
+    unsigned int *site_row = nullptr, *site_tags = nullptr;
+
     ~hm_system() { release(); }
 
     void release()
@@ -212,7 +233,23 @@ struct hm_system
             dfree(f.bptype);
             dfree(f.flag);
             dfree(f.tab);
+            dfree(f.mparams);
+            dfree(f.mbf16);
+            dfree(f.mG);
+            dfree(f.mA);
+            dfree(f.mE);
+            dfree(f.mT);
+            dfree(f.mErr);
+            dfree(f.mC);
+            dfree(f.mLoss);
+            dfree(f.mGpart);
+            dfree(f.mM1);
+            dfree(f.mM2);
+            dfree(f.mGrad);
+            dfree(f.mLossOut);
         }
+        dfree(site_row);
+        dfree(site_tags);
         forces.clear();
         dfree(cell_size);
         dfree(cell_adj);
@@ -408,7 +445,7 @@ struct hm_system
     void update_bond_index()
     {
         for (auto& f : forces)
-            if (f.kind == 1 && f.bpart)
+            if (is_bond(f) && f.bpart)
                 hchk(launch_bond_index(N, tag, rtag, f.row, f.other, f.btype, f.maxb, f.bpart,
                                        f.bptype, stream),
                      "bond index");
@@ -483,6 +520,10 @@ struct hm_system
     // only, so a Table force runs the relative gather (2); the relative forms need rel_ok()
     int pair_kind() const
     {
+        // the learned site energy reads plain particle-index rows
+        for (const auto& f : forces)
+            if (f.kind == 3)
+                return 1;
         int k = pair_kernel_kind;
         if (k == 0 || k == 3)
             for (const auto& f : forces)
@@ -673,11 +714,48 @@ struct hm_system
         hchk(launch_pair_cell_lj(c, f.p, pair_cell_cfg, stream), "pair force");
     }
 
+    MlpArgs mlp_args(ForceSlot& f, int with_virial)
+    {
+        MlpArgs a;
+        memset(&a, 0, sizeof(a));
+        a.N = N;
+        a.pos = pos;
+        a.box = box;
+        a.nlist = nlist;
+        a.nmax = nmax;
+        a.n_neigh = n_neigh;
+        a.d = make_mlp_descr(f.mm);
+        a.master = f.mparams;
+        a.wb = f.mbf16;
+        a.G = f.mG;
+        a.A = f.mA;
+        a.Esite = f.mE;
+        a.force = f.f;
+        a.virial = with_virial ? f.v : nullptr;
+        a.target = f.mT;
+        a.err = f.mErr;
+        a.C = f.mC;
+        a.loss_partial = f.mLoss;
+        a.gpart = f.mGpart;
+        a.loss_scale = (float)(2.0 / (3.0 * (double)N));
+        return a;
+    }
+
+    void launch_mlp(ForceSlot& f, int with_virial)
+    {
+        const MlpArgs a = mlp_args(f, with_virial);
+        hchk(launch_mlp_descriptor(a, stream), "mlp descriptor");
+        hchk(launch_mlp_site(a, stream), "mlp site energies");
+        hchk(launch_mlp_force(a, false, stream), "mlp forces");
+    }
+
     void launch_forces(int with_virial)
     {
         for (auto& f : forces)
         {
-            if (is_pair(f))
+            if (f.kind == 3)
+                launch_mlp(f, with_virial);
+            else if (is_pair(f))
                 launch_pair(f, with_virial);
             else
             {
@@ -700,7 +778,10 @@ struct hm_system
                 k.bpart = f.bpart;
                 k.bptype = f.bptype;
                 k.max_bonds = f.maxb;
-                hchk(launch_bond_fene(k, f.fp, stream), "bond force");
+                if (f.kind == 4)
+                    hchk(launch_bond_harmonic(k, f.hp, stream), "bond force");
+                else
+                    hchk(launch_bond_fene(k, f.fp, stream), "bond force");
             }
         }
     }
@@ -857,7 +938,7 @@ struct hm_system
     {
         ForceSlot* f = nullptr;
         for (auto& x : forces)
-            if (is_pair(x))
+            if (x.kind == 0 || x.kind == 2)
             {
                 f = &x;
                 break;
@@ -1353,13 +1434,14 @@ int hm_sys_add_pair_table(hm_system* s, const double* r_cut, const double* r_min
     });
 }
 
-int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
-                         const unsigned int* bond_typeid, unsigned int nbond_types,
-                         const hm_fene_params* params)
+namespace
 {
-    if (!s || (nb && !bond_tags) || !params || nbond_types == 0 || nbond_types > HM_MAX_TYPES
-        || s->forces.size() >= 4)
-        return fail(HM_ERR_ARGS, "hm_sys_add_bond_fene: invalid arguments");
+// the CSR-by-tag bond table, the index-order partner table and the bond exclusions of a bond
+// force (FENE: kind 1, harmonic: kind 4)
+int add_bond_force(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                   const unsigned int* bond_typeid, unsigned int nbond_types, int kind,
+                   const hm_fene_params* fene, const hm_harmonic_params* harm)
+{
     return guarded([&]() {
         unsigned int N = s->N;
         std::vector<unsigned int> deg(N + 1, 0);
@@ -1388,10 +1470,15 @@ int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond
             btype[fill[c]++] = t;
         }
         ForceSlot f;
-        f.kind = 1;
+        f.kind = kind;
         f.nbt = nbond_types;
         for (unsigned int q = 0; q < nbond_types; q++)
-            f.fp[q] = params[q];
+        {
+            if (fene)
+                f.fp[q] = fene[q];
+            if (harm)
+                f.hp[q] = harm[q];
+        }
         f.row = dalloc<unsigned int>(N + 1);
         f.other = dalloc<unsigned int>(other.size());
         f.btype = dalloc<unsigned int>(btype.size());
@@ -1456,6 +1543,54 @@ int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond
         s->clear_graphs();
         return (int)s->forces.size() - 1;
     });
+}
+} // namespace
+
+int hm_sys_update_pair_table(hm_system* s, int force_id, const double* U, const double* F)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size() || s->forces[force_id].kind != 2
+        || !U || !F)
+        return fail(HM_ERR_ARGS, "hm_sys_update_pair_table: invalid arguments");
+    return guarded([&]() {
+        ForceSlot& f = s->forces[force_id];
+        size_t total = 0;
+        for (unsigned int q = 0; q < s->ntypes * s->ntypes; q++)
+            total += f.tp[q].width;
+        std::vector<float> h(2 * total + 1, 0.f);
+        for (size_t k = 0; k < total; k++)
+        {
+            h[k] = (float)U[k];
+            h[total + k] = (float)F[k];
+        }
+        // the captured graphs read the table through the same pointers: new values take
+        // effect at the next step, no re-capture needed
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpyAsync(f.tab, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice,
+                            s->stream),
+             "upload");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        return HM_OK;
+    });
+}
+
+int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                         const unsigned int* bond_typeid, unsigned int nbond_types,
+                         const hm_fene_params* params)
+{
+    if (!s || (nb && !bond_tags) || !params || nbond_types == 0 || nbond_types > HM_MAX_TYPES
+        || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_bond_fene: invalid arguments");
+    return add_bond_force(s, nb, bond_tags, bond_typeid, nbond_types, 1, params, nullptr);
+}
+
+int hm_sys_add_bond_harmonic(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                             const unsigned int* bond_typeid, unsigned int nbond_types,
+                             const hm_harmonic_params* params)
+{
+    if (!s || (nb && !bond_tags) || !params || nbond_types == 0 || nbond_types > HM_MAX_TYPES
+        || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_bond_harmonic: invalid arguments");
+    return add_bond_force(s, nb, bond_tags, bond_typeid, nbond_types, 4, nullptr, params);
 }
 
 int hm_sys_set_method(hm_system* s, int method, double dt, double kT, const double* param)
@@ -1888,6 +2023,295 @@ int hm_sys_thermalize(hm_system* s, double kT)
 hm_stream_t hm_sys_stream(hm_system* s)
 {
     return s ? (hm_stream_t)s->stream : nullptr;
+}
+
+// ------------------------------------------------------------------ learned CG site energy
+namespace
+{
+ForceSlot* mlp_slot(hm_system* s, int force_id)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size()
+        || s->forces[force_id].kind != 3)
+        return nullptr;
+    return &s->forces[force_id];
+}
+
+void mlp_alloc_training(hm_system* s, ForceSlot& f)
+{
+    if (f.mT)
+        return;
+    const unsigned int N = s->N;
+    f.mT = dalloc<float4>(N);
+    f.mErr = dalloc<float4>(N);
+    f.mC = dalloc<float4>((size_t)4 * N);
+    f.mLoss = dalloc<float>(mlp_pair_blocks(N));
+    f.mGpart = dalloc<float>((size_t)MLP_GRAD_BLOCKS * MLP_NPARAM);
+    f.mM1 = dalloc<float>(MLP_NPARAM);
+    f.mM2 = dalloc<float>(MLP_NPARAM);
+    f.mGrad = dalloc<float>(MLP_NPARAM);
+    hchk(hipMemset(f.mM1, 0, sizeof(float) * MLP_NPARAM), "clear");
+    hchk(hipMemset(f.mM2, 0, sizeof(float) * MLP_NPARAM), "clear");
+}
+
+// the neighbour list of the current positions, built now (host-side regrow on overflow)
+void mlp_build_now(hm_system* s)
+{
+    if (!s->prepared)
+        s->prep();  // builds the list
+    else if (s->rcut_max > 0.0)
+        s->build_nlist_now();
+}
+
+// one training frame: positions and targets in, list built, forces, loss and gradient rows
+void mlp_frame_pass(hm_system* s, ForceSlot& f, const float* fpos, const float* fforce,
+                    double* loss_out)
+{
+    if (!s->prepared)
+        s->prep();
+    hchk(launch_mlp_load_frame(s->N, s->rtag, fpos, fforce, s->box, s->pos, f.mT, s->stream),
+         "mlp load frame");
+    if (s->rcut_max > 0.0)
+        s->build_nlist_now();
+    MlpArgs a = s->mlp_args(f, 0);
+    hchk(launch_mlp_descriptor(a, s->stream), "mlp descriptor");
+    hchk(launch_mlp_site(a, s->stream), "mlp site energies");
+    hchk(launch_mlp_force(a, true, s->stream), "mlp forces");
+    hchk(launch_mlp_backward(a, s->stream), "mlp backward");
+    hchk(launch_mlp_loss(f.mLoss, mlp_pair_blocks(s->N), loss_out, s->stream), "mlp loss");
+}
+} // namespace
+
+int hm_sys_add_mlp_cg(hm_system* s, const hm_mlp_cg_model* model, const float* params)
+{
+    if (!s || !model || !params || !(model->r_cut > 0.0) || !(model->width > 0.0)
+        || s->forces.size() >= 4)
+        return fail(HM_ERR_ARGS, "hm_sys_add_mlp_cg: invalid arguments");
+    return guarded([&]() {
+        ForceSlot f;
+        f.kind = 3;
+        f.mm = *model;
+        for (unsigned int q = 0; q < s->ntypes * s->ntypes; q++)
+            f.rcut[q] = model->r_cut;
+        const unsigned int N = s->N;
+        f.f = dalloc<float4>(N);
+        f.v = dalloc<float>((size_t)6 * N);
+        f.mparams = dalloc<float>(MLP_NPARAM);
+        f.mbf16 = dalloc<uint16_t>(MLP_NBF16);
+        f.mG = dalloc<float4>((size_t)4 * N);
+        f.mA = dalloc<float4>((size_t)4 * N);
+        f.mE = dalloc<float>(N);
+        hchk(hipMemset(f.f, 0, sizeof(float4) * N), "clear");
+        hchk(hipMemset(f.v, 0, sizeof(float) * 6 * N), "clear");
+        hchk(hipMemcpy(f.mparams, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice),
+             "upload");
+        hchk(launch_mlp_pack(f.mparams, f.mbf16, s->stream), "mlp pack");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        s->forces.push_back(f);
+        s->prepared = false;
+        s->nmax = 0;
+        s->clear_graphs();
+        return (int)s->forces.size() - 1;
+    });
+}
+
+int hm_sys_mlp_set_params(hm_system* s, int force_id, const float* params)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !params)
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_set_params: invalid arguments");
+    return guarded([&]() {
+        hchk(hipMemcpyAsync(f->mparams, params, sizeof(float) * MLP_NPARAM,
+                            hipMemcpyHostToDevice, s->stream),
+             "upload");
+        hchk(launch_mlp_pack(f->mparams, f->mbf16, s->stream), "mlp pack");
+        if (f->mM1)
+        {
+            hchk(hipMemsetAsync(f->mM1, 0, sizeof(float) * MLP_NPARAM, s->stream), "clear");
+            hchk(hipMemsetAsync(f->mM2, 0, sizeof(float) * MLP_NPARAM, s->stream), "clear");
+        }
+        f->adam_t = 0;
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        s->prepared = false;
+        return HM_OK;
+    });
+}
+
+int hm_sys_mlp_get_params(hm_system* s, int force_id, float* params)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !params)
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_get_params: invalid arguments");
+    return guarded([&]() {
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpy(params, f->mparams, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost),
+             "download");
+        return HM_OK;
+    });
+}
+
+int hm_sys_mlp_set_model(hm_system* s, int force_id, const hm_mlp_cg_model* model)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !model || model->r_cut != f->mm.r_cut || !(model->width > 0.0))
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_set_model: invalid arguments (r_cut is fixed)");
+    f->mm = *model;
+    s->prepared = false;
+    s->clear_graphs();
+    return HM_OK;
+}
+
+int hm_sys_mlp_descriptors(hm_system* s, int force_id, const float* frame_pos, float* G)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !G)
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_descriptors: invalid arguments");
+    return guarded([&]() {
+        if (frame_pos)
+        {
+            if (!s->prepared)
+                s->prep();
+            hchk(launch_mlp_load_frame(s->N, s->rtag, frame_pos, nullptr, s->box, s->pos,
+                                       nullptr, s->stream),
+                 "mlp load frame");
+            s->prepared = false;  // the next run starts from these positions: prepare again
+            if (s->rcut_max > 0.0)
+                s->build_nlist_now();
+        }
+        else
+            mlp_build_now(s);
+        MlpArgs a = s->mlp_args(*f, 0);
+        hchk(launch_mlp_descriptor(a, s->stream), "mlp descriptor");
+        std::vector<float> g((size_t)16 * s->N);
+        std::vector<unsigned int> tg(s->N);
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        hchk(hipMemcpy(g.data(), f->mG, sizeof(float) * g.size(), hipMemcpyDeviceToHost),
+             "download");
+        hchk(hipMemcpy(tg.data(), s->tag, sizeof(unsigned int) * s->N, hipMemcpyDeviceToHost),
+             "download");
+        for (unsigned int i = 0; i < s->N; i++)
+            memcpy(G + (size_t)16 * tg[i], &g[(size_t)16 * i], sizeof(float) * 16);
+        return HM_OK;
+    });
+}
+
+int hm_sys_mlp_loss_grad(hm_system* s, int force_id, const float* frame_pos,
+                         const float* frame_force, double* loss, float* grad)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !frame_pos || !frame_force || !loss)
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_loss_grad: invalid arguments");
+    return guarded([&]() {
+        mlp_alloc_training(s, *f);
+        if (!f->mLossOut)
+            f->mLossOut = dalloc<double>(1);
+        mlp_frame_pass(s, *f, frame_pos, frame_force, f->mLossOut);
+        hchk(launch_mlp_grad_sum(f->mGpart, f->mGrad, s->stream), "mlp gradient");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        double sq = 0.0;
+        hchk(hipMemcpy(&sq, f->mLossOut, sizeof(double), hipMemcpyDeviceToHost), "download");
+        *loss = sq / (3.0 * (double)s->N);
+        if (grad)
+            hchk(hipMemcpy(grad, f->mGrad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost),
+                 "download");
+        s->prepared = false;
+        return HM_OK;
+    });
+}
+
+int hm_sys_mlp_train(hm_system* s, int force_id, const hm_mlp_train_args* t)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || !t || !t->frame_pos || !t->frame_force || t->nframes == 0
+        || !(t->lr > 0.0) || !(t->beta1 >= 0.0 && t->beta1 < 1.0)
+        || !(t->beta2 >= 0.0 && t->beta2 < 1.0) || !(t->eps > 0.0))
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_train: invalid arguments");
+    if (t->order)
+        for (unsigned int k = 0; k < t->nsteps; k++)
+            if (t->order[k] >= t->nframes)
+                return fail(HM_ERR_ARGS, "hm_sys_mlp_train: frame order out of range");
+    return guarded([&]() {
+        mlp_alloc_training(s, *f);
+        dfree(f->mLossOut);
+        f->mLossOut = dalloc<double>(t->nsteps ? t->nsteps : 1);
+        const size_t stride = (size_t)3 * s->N;
+        for (unsigned int k = 0; k < t->nsteps; k++)
+        {
+            const unsigned int fr = t->order ? t->order[k] : k % t->nframes;
+            mlp_frame_pass(s, *f, t->frame_pos + stride * fr, t->frame_force + stride * fr,
+                           f->mLossOut + k);
+            hchk(launch_mlp_adam(f->mparams, f->mM1, f->mM2, f->mGpart, f->mbf16, (float)t->lr,
+                                 (float)t->beta1, (float)t->beta2, (float)t->eps, ++f->adam_t,
+                                 s->stream),
+                 "mlp adam");
+        }
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        if (t->loss && t->nsteps)
+        {
+            std::vector<double> h(t->nsteps);
+            hchk(hipMemcpy(h.data(), f->mLossOut, sizeof(double) * t->nsteps,
+                           hipMemcpyDeviceToHost),
+                 "download");
+            for (unsigned int k = 0; k < t->nsteps; k++)
+                t->loss[k] = h[k] / (3.0 * (double)s->N);
+        }
+        s->prepared = false;
+        return HM_OK;
+    });
+}
+
+int hm_sys_set_site_map(hm_system* s, unsigned int nsites, const unsigned int* row,
+                        const unsigned int* tags)
+{
+    if (!s || (nsites && (!row || !tags)))
+        return fail(HM_ERR_ARGS, "hm_sys_set_site_map: invalid arguments");
+    for (unsigned int I = 0; I < nsites; I++)
+        if (row[I + 1] <= row[I])
+            return fail(HM_ERR_ARGS, "hm_sys_set_site_map: every site needs a particle");
+    for (unsigned int b = 0; b < (nsites ? row[nsites] : 0u); b++)
+        if (tags[b] >= s->N)
+            return fail(HM_ERR_ARGS, "hm_sys_set_site_map: tag out of range");
+    return guarded([&]() {
+        dfree(s->site_row);
+        dfree(s->site_tags);
+        s->nsites = nsites;
+        if (nsites)
+        {
+            s->site_row = dalloc<unsigned int>(nsites + 1);
+            s->site_tags = dalloc<unsigned int>(row[nsites]);
+            hchk(hipMemcpy(s->site_row, row, sizeof(unsigned int) * (nsites + 1),
+                           hipMemcpyHostToDevice),
+                 "upload");
+            hchk(hipMemcpy(s->site_tags, tags, sizeof(unsigned int) * row[nsites],
+                           hipMemcpyHostToDevice),
+                 "upload");
+        }
+        return HM_OK;
+    });
+}
+
+int hm_sys_map_frame(hm_system* s, float* d_pos, float* d_force)
+{
+    if (!s || !d_pos || !s->nsites)
+        return fail(HM_ERR_ARGS, "hm_sys_map_frame: invalid arguments (no site map)");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        SiteMapArgs a;
+        memset(&a, 0, sizeof(a));
+        a.nsites = s->nsites;
+        a.row = s->site_row;
+        a.tags = s->site_tags;
+        a.rtag = s->rtag;
+        a.pos = s->pos;
+        a.vel = s->vel;
+        a.nforces = s->force_ptrs(a.f);
+        a.box = s->box;
+        a.out_pos = d_pos;
+        a.out_force = d_force;
+        hchk(launch_map_sites(a, s->stream), "map sites");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        return HM_OK;
+    });
 }
 
 int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg_ms,

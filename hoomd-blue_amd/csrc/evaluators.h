@@ -310,4 +310,44 @@ template<typename Real> struct EvaluatorBondFENE
     Real rsq, K, r_0, lj1, lj2, delta;
 };
 
+// Harmonic bond evaluator (hoomd/md/EvaluatorBondHarmonic.h:37-140): U = k (r - r0)^2 / 2,
+// force / r = k (r0 / r - 1), set to 0 when not finite (r = 0).
+template<typename Real> struct EvaluatorBondHarmonic
+{
+    struct param_type
+    {
+        Real k, r_0;
+
+        static param_type make(double k, double r0)
+        {
+            param_type p;
+            p.k = Real(k);
+            p.r_0 = Real(r0);
+            return p;
+        }
+    };
+
+    __host__ __device__ __forceinline__ EvaluatorBondHarmonic(Real _rsq, const param_type& p)
+        : rsq(_rsq), K(p.k), r_0(p.r_0)
+    {
+    }
+
+    __host__ __device__ __forceinline__ bool evalForceAndEnergy(Real& force_divr, Real& bond_eng)
+    {
+        Real r = sqrt(rsq);
+        force_divr = K * (r_0 / r - Real(1));
+        if (!isfinite(force_divr))
+            force_divr = Real(0);
+        bond_eng = Real(0.5) * K * (r_0 - r) * (r_0 - r);
+        return true;
+    }
+
+    static const char* getName()
+    {
+        return "harmonic";
+    }
+
+    Real rsq, K, r_0;
+};
+
 } // namespace hm

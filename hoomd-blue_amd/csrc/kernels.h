@@ -374,6 +374,8 @@ struct BondKernelArgs
     unsigned int max_bonds;         // bonds per particle in bpart (at most 4)
 };
 hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* params, hipStream_t s);
+hipError_t launch_bond_harmonic(const BondKernelArgs& a, const hm_harmonic_params* params,
+                                hipStream_t s);
 // the partner table of BondKernelArgs from the CSR-by-tag bonds; rebuilt whenever the particle
 // order changes (sort, restore)
 hipError_t launch_bond_index(unsigned int N, const unsigned int* tag, const unsigned int* rtag,

@@ -2435,7 +2435,7 @@ hipError_t launch_pair_cell_lj(const PairCellArgs& a, const hm_lj_params* params
     return hipGetLastError();
 }
 
-// ======================================================================== FENE bonds
+// ======================================================================== bonds
 // PotentialBond::computeForces (hoomd/md/PotentialBond.h:167-315) in the per-particle form of
 // gpu_compute_bond_forces_kernel (hoomd/md/PotentialBondGPU.cuh:98-239). Bonds are held by tag
 // (CSR), so a Hilbert sort only changes the tag->index map.
@@ -2444,8 +2444,8 @@ template<class Eval> struct BondTable
     typename Eval::param_type p[HM_MAX_TYPES];
 };
 
-__global__ __launch_bounds__(256) void bond_fene_kernel(BondKernelArgs a,
-                                                        BondTable<EvaluatorBondFENE<float>> tab)
+template<class Eval>
+__global__ __launch_bounds__(256) void bond_kernel(BondKernelArgs a, BondTable<Eval> tab)
 {
     unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N)
@@ -2505,7 +2505,7 @@ __global__ __launch_bounds__(256) void bond_fene_kernel(BondKernelArgs a,
             min_image(a.box, dx, dy, dz);
             rsq = dx * dx + dy * dy + dz * dz;
         }
-        EvaluatorBondFENE<float> ev((float)rsq, tab.p[bt]);
+        Eval ev((float)rsq, tab.p[bt]);
         float fdr = 0.f, e = 0.f;
         if (!ev.evalForceAndEnergy(fdr, e))
         {
@@ -2553,7 +2553,23 @@ hipError_t launch_bond_fene(const BondKernelArgs& a, const hm_fene_params* param
                                        params[q].sigma, params[q].delta);
     if (a.bpart && a.max_bonds > 4)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bond_fene_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a, tab);
+    hipLaunchKernelGGL(bond_kernel<E>, dim3((a.N + 255) / 256), dim3(256), 0, s, a, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_bond_harmonic(const BondKernelArgs& a, const hm_harmonic_params* params,
+                                hipStream_t s)
+{
+    if (a.N == 0)
+        return hipSuccess;
+    using E = EvaluatorBondHarmonic<float>;
+    BondTable<E> tab;
+    unsigned int nt = a.nbond_types ? a.nbond_types : 1;
+    for (unsigned int q = 0; q < nt; q++)
+        tab.p[q] = E::param_type::make(params[q].k, params[q].r0);
+    if (a.bpart && a.max_bonds > 4)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bond_kernel<E>, dim3((a.N + 255) / 256), dim3(256), 0, s, a, tab);
     return hipGetLastError();
 }
 

@@ -70,6 +70,34 @@ class HmFeneParams(C.Structure):
     _fields_ = [("k", _d), ("r0", _d), ("epsilon", _d), ("sigma", _d), ("delta", _d)]
 
 
+class HmHarmonicParams(C.Structure):
+    _fields_ = [("k", _d), ("r0", _d)]
+
+
+MLP_NK = 16
+MLP_NPARAM = 5313
+MLP_NBF16 = 10240
+MLP_GRAD_BLOCKS = 256
+
+
+class HmMlpModel(C.Structure):
+    _fields_ = [("r_cut", _d), ("width", _d), ("mu0", _d), ("dmu", _d),
+                ("gmean", _d * MLP_NK), ("gscale", _d * MLP_NK)]
+
+
+class HmMlpArgs(C.Structure):
+    _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("nlist", _p), ("nmax", _u),
+                ("n_neigh", _p), ("params", _p), ("params_bf16", _p), ("G", _p), ("A", _p),
+                ("Esite", _p), ("force", _p), ("virial", _p), ("target", _p), ("err", _p),
+                ("C", _p), ("loss_partial", _p), ("grad_partial", _p)]
+
+
+class HmMlpTrainArgs(C.Structure):
+    _fields_ = [("nframes", _u), ("frame_pos", _p), ("frame_force", _p), ("nsteps", _u),
+                ("order", _p), ("lr", _d), ("beta1", _d), ("beta2", _d), ("eps", _d),
+                ("loss", _p)]
+
+
 class HmBondArgs(C.Structure):
     _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("tag", _p), ("rtag", _p),
                 ("bond_row", _p), ("bond_other", _p), ("bond_type", _p), ("nbond_types", _u),
@@ -98,6 +126,16 @@ SIGNATURES = {
     "hm_pair_table": (_i, [C.POINTER(HmPairArgs), _p, _p]),
     "hm_pair_lj_cells": (_i, [C.POINTER(HmPairCellArgs), _p, _p]),
     "hm_bond_fene": (_i, [C.POINTER(HmBondArgs), _p, _p]),
+    "hm_bond_harmonic": (_i, [C.POINTER(HmBondArgs), _p, _p]),
+    "hm_mlp_cg_loss_blocks": (_u, [_u]),
+    "hm_mlp_cg_pack": (_i, [_p, _p, _p]),
+    "hm_mlp_cg_descriptor": (_i, [C.POINTER(HmMlpArgs), C.POINTER(HmMlpModel), _p]),
+    "hm_mlp_cg_forward": (_i, [C.POINTER(HmMlpArgs), C.POINTER(HmMlpModel), _p]),
+    "hm_mlp_cg_forces": (_i, [C.POINTER(HmMlpArgs), C.POINTER(HmMlpModel), _i, _p]),
+    "hm_mlp_cg_backward": (_i, [C.POINTER(HmMlpArgs), C.POINTER(HmMlpModel), _p]),
+    "hm_mlp_cg_loss": (_i, [_p, _u, _p, _p]),
+    "hm_mlp_cg_grad": (_i, [_p, _p, _p]),
+    "hm_mlp_cg_adam": (_i, [_p, _p, _p, _p, _p, _d, _d, _d, _d, _u, _p]),
     "hm_nvt_step_one": (_i, [_u, _p, _p, _p, _p, HmBox, _d, _d, _p]),
     "hm_nvt_step_two": (_i, [_u, _p, _p, _p, _u, _d, _d, _p]),
     "hm_langevin_step_two": (_i, [_u, _p, _p, _p, _p, _p, _u, _p, _u, _d, _d, _u64, _u16, _p]),
@@ -134,6 +172,17 @@ SIGNATURES = {
     "hm_sys_thermalize": (_i, [_p, _d]),
     "hm_sys_stream": (_p, [_p]),
     "hm_sys_time_pair": (_i, [_p, _i, _u, C.POINTER(_d), C.POINTER(_u64)]),
+    "hm_sys_add_bond_harmonic": (_i, [_p, _u, _p, _p, _u, _p]),
+    "hm_sys_update_pair_table": (_i, [_p, _i, _p, _p]),
+    "hm_sys_add_mlp_cg": (_i, [_p, C.POINTER(HmMlpModel), _p]),
+    "hm_sys_mlp_set_params": (_i, [_p, _i, _p]),
+    "hm_sys_mlp_get_params": (_i, [_p, _i, _p]),
+    "hm_sys_mlp_set_model": (_i, [_p, _i, C.POINTER(HmMlpModel)]),
+    "hm_sys_mlp_descriptors": (_i, [_p, _i, _p, _p]),
+    "hm_sys_mlp_train": (_i, [_p, _i, C.POINTER(HmMlpTrainArgs)]),
+    "hm_sys_mlp_loss_grad": (_i, [_p, _i, _p, _p, C.POINTER(_d), _p]),
+    "hm_sys_set_site_map": (_i, [_p, _u, _p, _p]),
+    "hm_sys_map_frame": (_i, [_p, _p, _p]),
 }
 
 

@@ -1,4 +1,5 @@
-"""hoomd.md subset: Integrator, nlist.Cell, pair.LJ, bond.FENEWCA, methods, compute."""
+"""hoomd.md subset: Integrator, nlist.Cell, pair.LJ/Table, bond.FENEWCA/Harmonic, methods,
+compute, and the learned coarse-grained site energy (mlcg)."""
 
-from . import bond, compute, methods, nlist, pair, tune  # noqa: F401
+from . import bond, compute, methods, mlcg, nlist, pair, tune  # noqa: F401
 from .integrate import Integrator  # noqa: F401

@@ -1,4 +1,5 @@
-"""Bond forces (hoomd/md/bond.py). ``FENEWCA`` drives the EvaluatorBondFENE kernel."""
+"""Bond forces (hoomd/md/bond.py). ``FENEWCA`` drives the EvaluatorBondFENE kernel and
+``Harmonic`` the EvaluatorBondHarmonic kernel."""
 
 import numpy as np
 
@@ -34,5 +35,35 @@ class FENEWCA(Force):
         tid = np.ascontiguousarray(b.typeid, np.uint32)
         fid = _lib.lib().hm_sys_add_bond_fene(h, b.N, g.ctypes.data, tid.ctypes.data, len(types), P)
         _lib.check(fid, "bond.FENEWCA")
+        self._force_id = fid
+        self._simulation = sim
+
+
+class Harmonic(Force):
+    """hoomd.md.bond.Harmonic(): params[bond_type] = dict(k, r0), U = k (r - r0)^2 / 2
+    (EvaluatorBondHarmonic.h:37-140, bond.py:57-106)."""
+
+    _log_namespace = ("md", "bond", "Harmonic")
+
+    def __init__(self):
+        super().__init__()
+        self.params = TypeParameter()
+
+    def _signature(self):
+        return ("Harmonic",
+                tuple(sorted((k, tuple(sorted(v.items()))) for k, v in self.params.items())))
+
+    def _attach(self, sim, h):
+        b = sim.state._snapshot.bonds
+        types = list(b.types) if b.types else ["A-A"]
+        P = (_lib.HmHarmonicParams * len(types))()
+        for q, t in enumerate(types):
+            p = self.params[t]
+            P[q].k, P[q].r0 = float(p["k"]), float(p["r0"])
+        g = np.ascontiguousarray(b.group, np.uint32).reshape(-1)
+        tid = np.ascontiguousarray(b.typeid, np.uint32)
+        fid = _lib.lib().hm_sys_add_bond_harmonic(h, b.N, g.ctypes.data, tid.ctypes.data,
+                                                  len(types), P)
+        _lib.check(fid, "bond.Harmonic")
         self._force_id = fid
         self._simulation = sim

@@ -103,11 +103,33 @@ class Table(Pair):
         return float(p["r_min"]), U, F
 
     def _signature(self):
+        # the layout (r_min and widths) only: new U and F values of an attached table are
+        # uploaded in place by _sync, so a ramped potential needs no new system
         tabs = []
         for k, v in sorted(self.params._d.items()):
             r_min, U, F = self._table(v)
-            tabs.append((k, r_min, U.tobytes(), F.tobytes()))
+            tabs.append((k, r_min, U.size))
         return ("Table", tuple(tabs), self.r_cut.signature(), self.nlist._signature())
+
+    def _values(self, types):
+        Us, Fs = [], []
+        for a in types:
+            for b in types:
+                _, U, F = self._table(self.params.get(a, b))
+                Us.append(U)
+                Fs.append(F)
+        return (np.ascontiguousarray(np.concatenate(Us)),
+                np.ascontiguousarray(np.concatenate(Fs)))
+
+    def _sync(self, sim):
+        U, F = self._values(sim.state.particle_types)
+        if getattr(self, "_uploaded", None) is not None and \
+                np.array_equal(self._uploaded[0], U) and np.array_equal(self._uploaded[1], F):
+            return
+        _lib.check(_lib.lib().hm_sys_update_pair_table(sim._system, self._force_id,
+                                                       U.ctypes.data, F.ctypes.data),
+                   "pair.Table values")
+        self._uploaded = (U, F)
 
     def _attach(self, sim, h):
         types = sim.state.particle_types
@@ -134,6 +156,7 @@ class Table(Pair):
         fid = _lib.lib().hm_sys_add_pair_table(h, rcf.ctypes.data, r_min.ctypes.data,
                                                width.ctypes.data, U.ctypes.data, F.ctypes.data)
         _lib.check(fid, "pair.Table")
+        self._uploaded = (U, F)
         self._force_id = fid
         self._simulation = sim
         self.nlist._simulation = sim

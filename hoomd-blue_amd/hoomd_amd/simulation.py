@@ -219,6 +219,9 @@ class Simulation:
         for c in self.operations.computes:
             c._simulation = self
         if self._system is not None and self._configured and self._attached_key == key:
+            for f in integ.forces:
+                if hasattr(f, "_sync"):
+                    f._sync(self)
             return
         if self._system is not None and self._configured:
             self._destroy_system()

@@ -241,6 +241,13 @@ typedef struct hm_fene_params
     double k, r0, epsilon, sigma, delta;
 } hm_fene_params;
 
+/* Harmonic bond parameters, EvaluatorBondHarmonic::param_type (hoomd/md/EvaluatorBondHarmonic.h):
+ * U = k (r - r0)^2 / 2. */
+typedef struct hm_harmonic_params
+{
+    double k, r0;
+} hm_harmonic_params;
+
 typedef struct hm_bond_args
 {
     unsigned int N;
@@ -263,6 +270,11 @@ typedef struct hm_bond_args
  * (hoomd/md/PotentialBondGPU.cuh:98-239). */
 int hm_bond_fene(const hm_bond_args* args, const hm_fene_params* params, hm_stream_t stream);
 
+/* Harmonic bond force with the same per-particle form and outputs as hm_bond_fene (flag is
+ * never set). Replaces gpu_compute_bond_forces<EvaluatorBondHarmonic>
+ * (hoomd/md/PotentialBondGPU.cuh:98-239, instantiated for md.bond.Harmonic). */
+int hm_bond_harmonic(const hm_bond_args* args, const hm_harmonic_params* params,
+                     hm_stream_t stream);
 /* Velocity-Verlet step one with velocity rescale alpha (1 for NVE): v = (v + a dt/2) * alpha;
  * r += v dt; wrap. Replaces gpu_nvt_rescale_step_one (hoomd/md/TwoStepConstantVolumeGPU.cu:
  * 35-100) and gpu_nve_step_one (hoomd/md/TwoStepNVEGPU.cu:46-126). */
@@ -305,6 +317,88 @@ int hm_sfc_order(unsigned int N, const double* pos, hm_box box, unsigned int gri
 int hm_sfc_traversal(unsigned int grid, unsigned int* traversal, hm_stream_t stream);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Learned coarse-grained site energy (no counterpart in the reference; the two-pass structure  */
+/* is the reference's EAM: hoomd/metal/EAMForceGPU.cu:20 density pass, :147 force pass)          */
+/* ------------------------------------------------------------------------------------------ */
+/* Site i of a coarse-grained system carries the 16 radial-basis densities
+ *   G_k(i) = sum_{j != i, r_ij < r_cut} exp(-(r_ij - mu_k)^2 / (2 width^2)) f_c(r_ij),
+ *   f_c(r) = (cos(pi r / r_cut) + 1) / 2,  mu_k = mu0 + k dmu,
+ * and the site energy E_i = w3 . silu(W2 silu(W1 x_i + b1) + b2) + b3 of x_i = (G(i) - gmean)
+ * gscale (16 -> 64 -> 64 -> 1). Forces come from the analytic backward:
+ *   F_i = -sum_j sum_k (dE_i/dG_k(i) + dE_j/dG_k(j)) d phi_k(r_ij)/dr (r_i - r_j) / r_ij.
+ * The matrix products run on bf16 MFMA with FP32 accumulation; weights are kept in FP32
+ * (params) with bf16 operand copies (params_bf16) refreshed by hm_mlp_cg_pack / _adam. */
+#define HM_MLP_NK 16
+#define HM_MLP_HIDDEN 64
+/* flat FP32 parameter vector: W1[64][16], b1[64], W2[64][64] (row = output unit), b2[64],
+ * w3[64], b3 */
+#define HM_MLP_NPARAM 5313
+#define HM_MLP_NBF16 10240     /* bf16 copies: W1, W1^T, W2, W2^T */
+#define HM_MLP_GRAD_BLOCKS 256 /* rows of partial gradients written by hm_mlp_cg_backward */
+
+typedef struct hm_mlp_cg_model
+{
+    double r_cut;                  /* cutoff of the basis (3.0) */
+    double width;                  /* Gaussian width (0.2) */
+    double mu0, dmu;               /* centres mu0 + k dmu, k < 16 (0, 0.2) */
+    double gmean[HM_MLP_NK];       /* input normalisation x = (G - gmean) * gscale */
+    double gscale[HM_MLP_NK];
+} hm_mlp_cg_model;
+
+typedef struct hm_mlp_cg_args
+{
+    unsigned int N;
+    const double* pos;             /* [4N] */
+    hm_box box;
+    const unsigned int* nlist;     /* [N*nmax] full list, row i = neighbour indices of i */
+    unsigned int nmax;
+    const unsigned int* n_neigh;   /* [N] */
+    const float* params;           /* [HM_MLP_NPARAM] */
+    const uint16_t* params_bf16;   /* [HM_MLP_NBF16] */
+    float* G;                      /* [16N] densities */
+    float* A;                      /* [16N] dE_i/dG(i) */
+    float* Esite;                  /* [N] site energies */
+    float* force;                  /* [4N] fx, fy, fz, site energy */
+    float* virial;                 /* [6N] rows of pitch N, or NULL */
+    /* training (force matching, loss = sum_i |F_i - T_i|^2 / (3N)) */
+    const float* target;           /* [4N] target forces (xyz used) */
+    float* err;                    /* [4N] dLoss/dF */
+    float* C;                      /* [16N] adjoint of the densities (times gscale) */
+    float* loss_partial;           /* [hm_mlp_cg_loss_blocks(N)] */
+    float* grad_partial;           /* [HM_MLP_GRAD_BLOCKS * HM_MLP_NPARAM] */
+} hm_mlp_cg_args;
+
+/* Entries of loss_partial for N sites. */
+unsigned int hm_mlp_cg_loss_blocks(unsigned int N);
+/* bf16 operand copies of the weights from the FP32 parameters. */
+int hm_mlp_cg_pack(const float* params, uint16_t* params_bf16, hm_stream_t stream);
+/* Pass 1: the densities G (EAMForceGPU.cu:20, the density sum of gpu_kernel_1). */
+int hm_mlp_cg_descriptor(const hm_mlp_cg_args* args, const hm_mlp_cg_model* model,
+                         hm_stream_t stream);
+/* Site MLP on bf16 MFMA: E_i into Esite and dE_i/dG(i) into A (the embedding derivative of
+ * gpu_kernel_1, EAMForceGPU.cu:20-145). */
+int hm_mlp_cg_forward(const hm_mlp_cg_args* args, const hm_mlp_cg_model* model,
+                      hm_stream_t stream);
+/* Pass 2: forces, site energies and (when virial != NULL) the per-site virial, half of each
+ * pair (EAMForceGPU.cu:147, gpu_kernel_2). With train != 0 also err = dLoss/dF against
+ * target and the block sums of |F - T|^2 into loss_partial. */
+int hm_mlp_cg_forces(const hm_mlp_cg_args* args, const hm_mlp_cg_model* model, int train,
+                     hm_stream_t stream);
+/* Training backward: the adjoint pass into C, then the parameter gradient of the loss as
+ * HM_MLP_GRAD_BLOCKS rows of partial sums in grad_partial (matrix products on bf16 MFMA). */
+int hm_mlp_cg_backward(const hm_mlp_cg_args* args, const hm_mlp_cg_model* model,
+                       hm_stream_t stream);
+/* out[0] = sum_i |F_i - T_i|^2 (FP64) from loss_partial of N sites. */
+int hm_mlp_cg_loss(const float* loss_partial, unsigned int N, double* out, hm_stream_t stream);
+/* grad = sum of the grad_partial rows (fixed order). */
+int hm_mlp_cg_grad(const float* grad_partial, float* grad, hm_stream_t stream);
+/* One Adam step t (t >= 1) on params with moments m1, m2 (each [HM_MLP_NPARAM]) from the
+ * grad_partial rows; refreshes params_bf16. */
+int hm_mlp_cg_adam(float* params, float* m1, float* m2, const float* grad_partial,
+                   uint16_t* params_bf16, double lr, double beta1, double beta2, double eps,
+                   unsigned int t, hm_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------ */
 /* System engine                                                                               */
 /* ------------------------------------------------------------------------------------------ */
 
@@ -333,6 +427,9 @@ int hm_sys_add_pair_lj(hm_system* s, const double* r_cut, const double* r_on,
  * r_cut <= 0 disables a pair. Returns the force id. */
 int hm_sys_add_pair_table(hm_system* s, const double* r_cut, const double* r_min,
                           const unsigned int* width, const double* U, const double* F);
+/* New U and F values of a Table force, same layout and widths as at hm_sys_add_pair_table
+ * (a ramped potential changes its values between runs without rebuilding the system). */
+int hm_sys_update_pair_table(hm_system* s, int force_id, const double* U, const double* F);
 /* FENE+WCA bonds (md.bond.FENEWCA): bonds by tag [2*nb], bond types [nb] (may be NULL),
  * params per bond type. Returns the force id. */
 int hm_sys_add_bond_fene(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
@@ -395,6 +492,51 @@ int hm_sys_stats(hm_system* s, uint64_t* out9);
 int hm_sys_thermalize(hm_system* s, double kT);
 /* Launch stream used by the engine (a hipStream_t). */
 hm_stream_t hm_sys_stream(hm_system* s);
+/* Harmonic bonds (md.bond.Harmonic): bonds by tag [2*nb], bond types [nb] (may be NULL),
+ * params per bond type. Returns the force id. */
+int hm_sys_add_bond_harmonic(hm_system* s, unsigned int nb, const unsigned int* bond_tags,
+                             const unsigned int* bond_typeid, unsigned int nbond_types,
+                             const hm_harmonic_params* params);
+/* Learned CG site energy as a force of the system (all type pairs use model->r_cut), from
+ * HM_MLP_NPARAM FP32 host parameters. It reads the system's neighbour list, so bonded pairs
+ * are left out when the list excludes bonds (hm_sys_set_nlist). Returns the force id. */
+int hm_sys_add_mlp_cg(hm_system* s, const hm_mlp_cg_model* model, const float* params);
+/* Replace the parameters (host, HM_MLP_NPARAM floats); resets the Adam moments. */
+int hm_sys_mlp_set_params(hm_system* s, int force_id, const float* params);
+int hm_sys_mlp_get_params(hm_system* s, int force_id, float* params);
+/* Replace the normalisation and basis constants (r_cut cannot change). */
+int hm_sys_mlp_set_model(hm_system* s, int force_id, const hm_mlp_cg_model* model);
+/* Densities G by tag [16N] (host) of the current state, or, when frame_pos is not NULL, of
+ * the positions frame_pos (device [N][3] FP32 by tag), which then replace the state's. */
+int hm_sys_mlp_descriptors(hm_system* s, int force_id, const float* frame_pos, float* G);
+
+/* Force-matching training on device-resident frames: step k loads frame order[k] (or
+ * k mod nframes), builds the list, evaluates the forces, the loss sum |F - T|^2 / (3N) and
+ * its gradient, and takes one Adam step. */
+typedef struct hm_mlp_train_args
+{
+    unsigned int nframes;
+    const float* frame_pos;        /* device [nframes][N][3] FP32 positions by tag */
+    const float* frame_force;      /* device [nframes][N][3] FP32 target forces by tag */
+    unsigned int nsteps;
+    const unsigned int* order;     /* host [nsteps] frame of each step, or NULL */
+    double lr, beta1, beta2, eps;
+    double* loss;                  /* host [nsteps] loss of each step before its update, or NULL */
+} hm_mlp_train_args;
+int hm_sys_mlp_train(hm_system* s, int force_id, const hm_mlp_train_args* args);
+/* Loss and gradient (host [HM_MLP_NPARAM], may be NULL) of one frame (device pointers as in
+ * hm_mlp_train_args), with no update. */
+int hm_sys_mlp_loss_grad(hm_system* s, int force_id, const float* frame_pos,
+                         const float* frame_force, double* loss, float* grad);
+
+/* Coarse-grained site map: site I holds the particles with tags tags[row[I]] ..
+ * tags[row[I+1]-1] (host arrays, row[nsites+1]). */
+int hm_sys_set_site_map(hm_system* s, unsigned int nsites, const unsigned int* row,
+                        const unsigned int* tags);
+/* Map the current state: d_pos[3*nsites] = mass-weighted site centres wrapped into the box,
+ * d_force[3*nsites] (may be NULL) = summed forces of all force computes (device FP32). */
+int hm_sys_map_frame(hm_system* s, float* d_pos, float* d_force);
+
 /* Benchmark helper (no counterpart in the reference): average duration in ms of the pair-force
  * kernel of force force_id over iters back-to-back launches on the engine stream, timed with
  * HIP events on that stream, and the number of neighbour-list entries it reads. */

@@ -54,3 +54,26 @@ def maxwell_boltzmann(N, kT, seed=7, mass=None):
     v = rng.normal(0.0, 1.0, size=(N, 3)) * np.sqrt(kT / m)[:, None]
     p = (m[:, None] * v).sum(axis=0) / N
     return v - p[None, :] / m[:, None]
+
+
+def random_walk_chains(n_chains, chain_len, rho, bond=0.97, seed=1):
+    """Linear chains as random walks with fixed bond length in a cubic box at bead density rho
+    (the Kremer-Grest melt's initial state: ideal chains that overlap freely). Returns
+    (positions[N,3] wrapped into [-L/2, L/2), images[N,3], bonds[M,2] by tag, L); tags run
+    chain by chain."""
+    N = n_chains * chain_len
+    L = (N / rho) ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    start = rng.uniform(-L / 2, L / 2, size=(n_chains, 1, 3))
+    v = rng.normal(size=(n_chains, chain_len - 1, 3))
+    v *= bond / np.linalg.norm(v, axis=2, keepdims=True)
+    steps = np.concatenate([np.zeros((n_chains, 1, 3)), np.cumsum(v, axis=1)], axis=1)
+    unwrapped = (start + steps).reshape(N, 3)
+    image = np.floor((unwrapped + L / 2) / L).astype(np.int32)
+    pos = unwrapped - image * L
+    up = pos >= L / 2
+    pos[up] -= L
+    image[up] += 1
+    first = (np.arange(n_chains)[:, None] * chain_len + np.arange(chain_len - 1)[None, :]).ravel()
+    bonds = np.stack([first, first + 1], axis=1).astype(np.uint32)
+    return pos, image, bonds, L

@@ -1,0 +1,215 @@
+"""The learned CG site energy through the engine and the hoomd_amd surface (md.mlcg): forces of
+a running system against the FP64 oracle, force-matching training, and the whole
+fine-grained -> coarse-grained workflow on a small Kremer-Grest melt, where the trained network
+must explain a clear part of the mapped non-bonded force that the bonds-only model leaves
+unexplained.
+
+Tolerances: engine forces and site energies against the bf16-rounding oracle 5e-4 relative RMS
+(the kernel-level tests state why); loss 1e-4 and gradient 3e-3 relative RMS against the same
+oracle after training moved the parameters."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import mlp_cg as M
+from test_gpu_mlp import cg_config, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hoomd():
+    from hoomd_amd import _lib
+    if _lib.lib().hm_device_count() < 1:
+        pytest.skip("no GPU")
+    import hoomd_amd
+    return hoomd_amd
+
+
+def cg_sim(hoomd, pos, L, bonds=None, params=None, gmean=None, gscale=None, exclusions=()):
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = len(pos)
+    snap.particles.types = ["S"]
+    snap.particles.position[:] = pos
+    forces = []
+    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=exclusions)
+    mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, params=params)
+    if gmean is not None:
+        mlp.normalization = (gmean, gscale)
+    forces.append(mlp)
+    harm = None
+    if bonds is not None:
+        snap.bonds.N = len(bonds)
+        snap.bonds.types = ["cg"]
+        snap.bonds.group[:] = bonds
+        harm = hoomd.md.bond.Harmonic()
+        harm.params["cg"] = dict(k=5.0, r0=1.6)
+        forces.append(harm)
+    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=3)
+    sim.create_state_from_snapshot(snap)
+    lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
+    sim.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=forces, methods=[lang])
+    sim.operations.tuners.clear()
+    return sim, mlp, harm
+
+
+def stats_model(pos, L, nl, nn):
+    G = M.descriptor(M.Model(), L, pos, nl, nn)
+    gmean, gscale = G.mean(axis=0), 1.0 / (G.std(axis=0) + 1e-3)
+    return M.Model(gmean=gmean, gscale=gscale, bf16=True), gmean, gscale
+
+
+def test_engine_forces_match_oracle(hoomd):
+    print("engine mlp forces: N=2500", flush=True)
+    pos, L = cg_config(2500, seed=11)
+    nl, nn = O.nlist_binned(O.Box(L), pos, 3.0, 0.0, full=True)
+    om, gmean, gscale = stats_model(pos, L, nl, nn)
+    p = M.init_params(21)
+    sim, mlp, _ = cg_sim(hoomd, pos, L, params=p.astype(np.float32), gmean=gmean, gscale=gscale)
+    sim.run(0)
+    _, oF, oE, _, _ = M.forces(om, p.astype(np.float32).astype(np.float64), L, pos, nl, nn)
+    eF, eE = rel_rms(mlp.forces, oF), rel_rms(mlp.energies, oE)
+    print(f"engine forces rel RMS {eF:.2e}, site energies {eE:.2e}", flush=True)
+    assert eF < 5e-4 and eE < 5e-4
+    # one step later (the list with its buffer, new positions) still matches
+    sim.run(5)
+    s = sim.state.get_snapshot()
+    p2 = s.particles.position
+    nl2, nn2 = O.nlist_binned(O.Box(L), p2, 3.0, 0.0, full=True)
+    _, oF2, _, _, _ = M.forces(om, p.astype(np.float32).astype(np.float64), L, p2, nl2, nn2)
+    assert rel_rms(mlp.forces, oF2) < 5e-4
+
+
+def test_engine_bonds_and_exclusions(hoomd):
+    print("engine mlp + harmonic bonds", flush=True)
+    pos, L = cg_config(1200, seed=12)
+    # bond each even site to its nearest odd partner within 2.6
+    bonds = []
+    used = set()
+    for i in range(0, len(pos), 2):
+        d = pos[1::2] - pos[i]
+        d -= L * np.rint(d / L)
+        r = np.sqrt((d * d).sum(axis=1))
+        j = int(np.argmin(r)) * 2 + 1
+        if r.min() < 2.6 and j not in used:
+            bonds.append((i, j))
+            used.add(j)
+    bonds = np.asarray(bonds, np.uint32)
+    excl = [[] for _ in range(len(pos))]
+    for a, b in bonds:
+        excl[a].append(int(b))
+        excl[b].append(int(a))
+    nl, nn = O.nlist_binned(O.Box(L), pos, 3.0, 0.0, full=True, excl=excl)
+    om, gmean, gscale = stats_model(pos, L, nl, nn)
+    p = M.init_params(22).astype(np.float32)
+    sim, mlp, harm = cg_sim(hoomd, pos, L, bonds=bonds, params=p, gmean=gmean, gscale=gscale,
+                            exclusions=("bond",))
+    sim.run(0)
+    _, oF, _, _, _ = M.forces(om, p.astype(np.float64), L, pos, nl, nn)
+    assert rel_rms(mlp.forces, oF) < 5e-4
+    from hoomd_amd.md.mlcg import harmonic_bond_forces
+    hb = harmonic_bond_forces(pos, L, bonds, 5.0, 1.6)
+    assert rel_rms(harm.forces, hb) < 1e-5
+
+
+def test_training_teacher_student(hoomd):
+    """Targets from a teacher network (FP64 oracle forces); a student from other random
+    parameters learns them: the loss falls by more than 20x, and the engine's loss and gradient
+    at the trained parameters match the oracle."""
+    from hoomd_amd.md.mlcg import FrameSet
+    N, nfr = 800, 6
+    confs = [cg_config(N, seed=30 + k) for k in range(nfr)]
+    L = confs[0][1]
+    nl0, nn0 = O.nlist_binned(O.Box(L), confs[0][0], 3.0, 0.0, full=True)
+    om, gmean, gscale = stats_model(confs[0][0], L, nl0, nn0)
+    om64 = M.Model(gmean=gmean, gscale=gscale)
+    teacher = M.init_params(40)
+    fs = FrameSet(nfr, N)
+    for k, (p, _) in enumerate(confs):
+        nl, nn = O.nlist_binned(O.Box(L), p, 3.0, 0.0, full=True)
+        _, F, _, _, _ = M.forces(om64, teacher, L, p, nl, nn)
+        fs.set(k, positions=p, forces=F)
+    sim, mlp, _ = cg_sim(hoomd, confs[0][0], L, params=M.init_params(41).astype(np.float32),
+                         gmean=gmean, gscale=gscale)
+    loss = mlp.train(fs, 600, lr=3e-3)
+    first, last = loss[:nfr].mean(), loss[-nfr:].mean()
+    print(f"teacher-student loss {first:.4e} -> {last:.4e}", flush=True)
+    assert np.isfinite(loss).all() and last < first / 20
+    # engine loss and gradient at the trained parameters against the oracle (frame 2)
+    pt = mlp.params.astype(np.float64)
+    l_gpu, g_gpu = mlp.loss_grad(fs, 2)
+    p2 = confs[2][0]
+    nl, nn = O.nlist_binned(O.Box(L), p2, 3.0, 0.0, full=True)
+    T = fs.forces(2).astype(np.float64)
+    l_or, g_or, _ = M.loss_grad(om, pt, L, fs.positions(2).astype(np.float64), nl, nn, T)
+    print(f"frame 2: loss {l_gpu:.6e} vs oracle {l_or:.6e}, grad rel RMS "
+          f"{rel_rms(g_gpu, g_or):.2e}", flush=True)
+    assert abs(l_gpu - l_or) <= 1e-4 * l_or
+    assert rel_rms(g_gpu, g_or) < 3e-3
+
+
+def test_kg_to_cg_network_gains(hoomd):
+    """Fine-grained Kremer-Grest melt (40 chains x 50 beads, random walks pushed off, WCA +
+    FENE, Langevin) mapped 5:1 on the GPU; CG bonds from Boltzmann inversion; the MLP trained on
+    the mapped non-bonded force. On held-out frames the trained network must leave a smaller
+    force residual than the bonds-only model (the same CG model without the network)."""
+    from kg_workload import pushed_off_melt
+    from hoomd_amd.md import mlcg
+    n_chains, chain_len, per = 40, 50, 5
+    print("kg -> cg: push-off and equilibration", flush=True)
+    sim, L = pushed_off_melt(hoomd, n_chains, chain_len, pushoff_steps=2000, ramp_updates=20,
+                             seed=5)
+    sim.run(3000)
+    smap = mlcg.SiteMap.linear_chains(n_chains, chain_len, per)
+    smap.attach(sim)
+    nfr, every = 60, 40
+    frames = mlcg.FrameSet(nfr, smap.nsites)
+    for k in range(nfr):
+        sim.run(every)
+        frames.record(sim, k)
+    cgb = smap.bonds()
+    r = np.concatenate([mlcg.bond_lengths(frames.positions(k).astype(np.float64), L, cgb)
+                        for k in range(nfr)])
+    kb, r0 = mlcg.boltzmann_harmonic(r, kT=1.0, nbins=40, min_count=10)
+    print(f"CG bonds by Boltzmann inversion: k = {kb:.3f}, r0 = {r0:.3f}", flush=True)
+    assert 0.5 < r0 < 4.0 and kb > 0
+    target = mlcg.FrameSet(nfr, smap.nsites)
+    for k in range(nfr):
+        p = frames.positions(k).astype(np.float64)
+        T = frames.forces(k).astype(np.float64) - mlcg.harmonic_bond_forces(p, L, cgb, kb, r0)
+        target.set(k, positions=p, forces=T)
+    # the CG system: sites, CG bonds, the network (bonded pairs excluded)
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = smap.nsites
+    snap.particles.types = ["S"]
+    snap.particles.position[:] = frames.positions(0)
+    snap.bonds.N = len(cgb)
+    snap.bonds.types = ["cg"]
+    snap.bonds.group[:] = cgb
+    cg = hoomd.Simulation(device=hoomd.device.GPU(), seed=7)
+    cg.create_state_from_snapshot(snap)
+    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=("bond",))
+    mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=3)
+    harm = hoomd.md.bond.Harmonic()
+    harm.params["cg"] = dict(k=kb, r0=r0)
+    lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
+    cg.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp, harm], methods=[lang])
+    cg.operations.tuners.clear()
+    G = np.concatenate([mlp.descriptors(target, k) for k in range(0, nfr, 6)])
+    mlp.fit_normalization(G)
+    train, held = np.arange(0, nfr - 12), np.arange(nfr - 12, nfr)
+    zero = np.zeros(mlcg.NPARAM, np.float32)
+    mlp.params = zero
+    base = np.mean([mlp.loss_grad(target, int(k))[0] for k in held])
+    mlp.params = mlcg.init_params(3)
+    order = np.random.default_rng(0).choice(train, size=1500)
+    loss = mlp.train(target, 1500, lr=2e-3, order=order)
+    trained = np.mean([mlp.loss_grad(target, int(k))[0] for k in held])
+    print(f"held-out force residual: bonds only {base:.4f}, with the trained network "
+          f"{trained:.4f} ({trained / base:.3f}); training loss {loss[:50].mean():.4f} -> "
+          f"{loss[-50:].mean():.4f}", flush=True)
+    assert np.isfinite(loss).all()
+    assert trained < 0.9 * base
