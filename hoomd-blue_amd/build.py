@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "hoomd_amd", "libhoomd_amd.so")
 ARCH = os.environ.get("HM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "sort.hip", "capi.hip", "engine.hip", "mlp.hip"]
+SOURCES = ["kernels.hip", "sort.hip", "capi.hip", "engine.hip", "mlp.hip", "batch.hip", "pair_ref.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",
@@ -49,14 +49,26 @@ def _filter(out):
     return "\n".join(lines)
 
 
+def _includes(path, seen):
+    """Quoted includes of path found in csrc/ or include/, recursively."""
+    import re
+    for name in re.findall(r'^\s*#\s*include\s+"([^"]+)"', open(path).read(), re.M):
+        for d in (os.path.dirname(path), CSRC, os.path.join(ROOT, "include")):
+            q = os.path.join(d, name)
+            if os.path.exists(q) and q not in seen:
+                seen.add(q)
+                _includes(q, seen)
+                break
+    return seen
+
+
 def _up_to_date(src, obj):
-    """The object is newer than its source and every header of csrc/ and include/."""
+    """The object is newer than its source and every header it includes (transitively)."""
     if not os.path.exists(obj):
         return False
     t = os.path.getmtime(obj)
-    deps = [os.path.join(CSRC, src)]
-    for d in (CSRC, os.path.join(ROOT, "include")):
-        deps += [os.path.join(d, f) for f in os.listdir(d) if f.endswith(".h")]
+    path = os.path.join(CSRC, src)
+    deps = [path] + sorted(_includes(path, set()))
     return all(os.path.getmtime(p) < t for p in deps)
 
 
@@ -72,18 +84,37 @@ def compile_one(src, incremental=False):
     return obj
 
 
+# Out-of-library evaluator plugins (each its own shared library, as a user's plugin would be)
+PLUGINS = [os.path.join(HERE, "plugins", "pair_example", "pair_example.hip")]
+
+
+def build_plugin(src, incremental=False):
+    out = os.path.join(os.path.dirname(src), "lib" + os.path.basename(src)[:-4] + ".so")
+    deps = [src] + sorted(_includes(src, set()))
+    if incremental and os.path.exists(out) and \
+            all(os.path.getmtime(p) < os.path.getmtime(out) for p in deps):
+        return out
+    cmd = [hipcc()] + CFLAGS + ["-shared", src, "-o", out + ".tmp"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"plugin build failed: {src}\n{_filter(p.stdout + p.stderr)}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(jobs=4, verbose=False, incremental=False):
     """Compile every source (or, with incremental, only those older than their inputs) and
-    link the library."""
+    link the library; then build the evaluator plugins."""
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: compile_one(s, incremental), SOURCES))
+        plugins = list(ex.map(lambda s: build_plugin(s, incremental), PLUGINS))
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed\n{_filter(p.stdout + p.stderr)}")
     os.replace(OUT + ".tmp", OUT)
     if verbose:
-        print("built", OUT)
+        print("built", OUT, *plugins)
     return OUT
 
 

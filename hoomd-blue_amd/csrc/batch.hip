@@ -1,0 +1,1280 @@
+// batch.hip -- many independent LJ systems ("replicas") of uneven size advanced together.
+//
+// The reference advances one system per Simulation; a study of many small systems (state
+// points, seeds) runs them one after another, each step a launch sequence sized for one small
+// system that cannot fill the chip. Here every replica lives in one set of concatenated arrays
+// (replica r owns particles [off_r, off_r + n_r) and cells [cell_off_r, cell_off_r + dim_r^3))
+// and every phase of the timestep is ONE launch over all replicas:
+//
+//   step one   (tiles)      Bussi factor of the tile's replica, velocity-Verlet first half,
+//                           wrap, distance check -> the replica's rebuild flag
+//   cell clear (cells)      only cells of replicas that rebuild
+//   cell bin   (tiles)      only particles of replicas that rebuild
+//   list build (tiles)      full neighbour rows, only replicas that rebuild
+//   pair       (particles)  LJ force/energy, FP32 arithmetic on FP64 minimum-image separations
+//   step two   (tiles)      second half kick, kinetic-energy partial per tile
+//
+// A tile is up to BTILE consecutive particles of one replica, so per-tile reductions never mix
+// replicas and every per-replica sum (kinetic energy) is a fixed-order sum of its tiles'
+// partials: results do not depend on launch shape or on the other replicas. Every decision is
+// taken on the device (per-replica flags), so chunks of steps are captured as hipGraphs.
+//
+// Per-replica algorithms are the reference's: TwoStepConstantVolume step one/two
+// (hoomd/md/TwoStepConstantVolume.cc:7-80,213-260) with the Bussi factor of
+// hoomd/md/Thermostat.h:357-446 (RNG instance = tag 0 of the replica), NeighborList's distance
+// check (hoomd/md/NeighborList.cc:1220-1289, rebuild_check_delay 1), the binned full list
+// (hoomd/md/NeighborListGPUBinned.cu:52-298: j != i, r^2 <= r_list^2) over the unique adjacent
+// cells of CellList::initializeCellAdj (hoomd/CellList.cc:346-402), and PotentialPair with
+// EvaluatorPairLJ (hoomd/md/PotentialPair.h:587-794, EvaluatorPairLJ.h:190-211).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "capi_util.h"
+#include "evaluators.h"
+#include "kernels.h"
+
+using namespace hm;
+
+namespace
+{
+
+constexpr unsigned int BTILE = 256;  // particles per tile (= block of the tile kernels)
+
+struct BError
+{
+    int code;
+    std::string what;
+};
+
+inline void bchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw BError{HM_ERR_LAUNCH, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template<class T> T* balloc(size_t n)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess)
+        throw BError{HM_ERR_ALLOC, "device allocation failed"};
+    return (T*)p;
+}
+
+template<class T> void bfree(T*& p)
+{
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// per-replica constants
+struct Rep
+{
+    double L, Linv, hi;         // cubic box edge, 1/L, L/2
+    double kT, ndof;            // Bussi target and translational degrees of freedom
+    double cell_inv;            // dim / L
+    unsigned int off, n;        // particle range
+    unsigned int cell_off, dim; // cell range (dim^3 cells)
+    unsigned int tile_off, ntile;
+    unsigned int seed;          // 16-bit user seed
+    unsigned int pad;
+};
+
+// per-replica device state
+struct RepState
+{
+    unsigned int rebuild;      // rebuild before the forces of the current step
+    unsigned int n_builds;
+    unsigned int error;        // ERR_ZERO_KE
+    unsigned int pad;
+};
+
+struct BatchStep
+{
+    unsigned long long timestep; // step that the next step-one kernel starts
+};
+
+struct BCond
+{
+    unsigned int cell_need;
+    unsigned int nlist_need;
+    unsigned int nan_idx;
+    unsigned int pad;
+};
+
+struct PairConst
+{
+    float lj1, lj2;
+    double rcutsq;
+    float rcutsq_f;
+};
+
+__device__ __forceinline__ double block_reduce(double v, double* sh)
+{
+    // BTILE = 256 threads = 4 wavefronts; fixed-order sums
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    const unsigned int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0)
+        sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void tile_range(const Rep& R, unsigned int t, unsigned int& first,
+                                           unsigned int& count)
+{
+    const unsigned int k = t - R.tile_off;
+    first = R.off + k * BTILE;
+    count = min(BTILE, R.n - k * BTILE);
+}
+
+// -------------------------------------------------------------------------- step one
+// TwoStepConstantVolume::integrateStepOne with the Bussi factor of the replica's kinetic energy
+// (sum of its tiles' partials, fixed order, computed redundantly by every tile of the replica),
+// then the distance check against the positions of the last build.
+__global__ __launch_bounds__(BTILE) void bstep_one_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
+    RepState* __restrict__ rst, const BatchStep* __restrict__ bst,
+    const double* __restrict__ ke_partial, double4* __restrict__ pos, double4* __restrict__ vel,
+    const d3* __restrict__ accel, i3* __restrict__ image, const d3* __restrict__ last,
+    BCond* __restrict__ cond, double dt, double tau, int bussi, double maxsq)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    __shared__ double s_alpha;
+    __shared__ unsigned int s_move;
+    const unsigned int t = blockIdx.x;
+    const unsigned int r = tile_rep[t];
+    const Rep R = reps[r];
+    unsigned int first, count;
+    tile_range(R, t, first, count);
+    if (threadIdx.x == 0)
+        s_move = 0;
+    double alpha = 1.0;
+    if (bussi)
+    {
+        double acc = 0.0;
+        if (threadIdx.x < R.ntile)
+            acc = ke_partial[R.tile_off + threadIdx.x];
+        // replicas of more than BTILE tiles: remaining partials in a fixed order
+        for (unsigned int q = threadIdx.x + BTILE; q < R.ntile; q += BTILE)
+            acc += ke_partial[R.tile_off + q];
+        const double tot = block_reduce(acc, sh);
+        if (threadIdx.x == 0)
+        {
+            const double K = tot * 0.5;
+            double a = 1.0;
+            if (K == 0.0)
+            {
+                if (t == R.tile_off)
+                    atomicOr(&rst[r].error, ERR_ZERO_KE);
+            }
+            else
+                a = bussi_alpha(K, R.ndof, dt, tau, R.kT, bst->timestep, (uint16_t)R.seed, 0u);
+            s_alpha = a;
+        }
+        __syncthreads();
+        alpha = s_alpha;
+    }
+    unsigned int moved = 0;
+    if (threadIdx.x < count)
+    {
+        const unsigned int p = first + threadIdx.x;
+        double4 x = pos[p];
+        double4 v = vel[p];
+        const d3 a = accel[p];
+        v.x = (v.x + (1.0 / 2.0) * a.x * dt) * alpha;
+        v.y = (v.y + (1.0 / 2.0) * a.y * dt) * alpha;
+        v.z = (v.z + (1.0 / 2.0) * a.z * dt) * alpha;
+        x.x += dt * v.x;
+        x.y += dt * v.y;
+        x.z += dt * v.z;
+        i3 im = image[p];
+        wrap_axis(x.x, im.x, -R.hi, R.hi, R.L);
+        wrap_axis(x.y, im.y, -R.hi, R.hi, R.L);
+        wrap_axis(x.z, im.z, -R.hi, R.hi, R.L);
+        pos[p] = x;
+        vel[p] = v;
+        image[p] = im;
+        if (x.x != x.x || x.y != x.y || x.z != x.z)
+            atomicMax(&cond->nan_idx, p + 1);
+        const d3 l = last[p];
+        double dx = x.x - l.x, dy = x.y - l.y, dz = x.z - l.z;
+        dz -= R.L * rint(dz * R.Linv);
+        dy -= R.L * rint(dy * R.Linv);
+        dx -= R.L * rint(dx * R.Linv);
+        moved = (dx * dx + dy * dy + dz * dz >= maxsq) ? 1u : 0u;
+    }
+    if (wave_ballot(moved != 0) != 0 && lane_id() == 0)
+        s_move = 1;  // benign race: every writer stores 1
+    __syncthreads();
+    if (threadIdx.x == 0 && s_move)
+        atomicOr(&rst[r].rebuild, 1u);
+}
+
+// -------------------------------------------------------------------------- cells
+__global__ __launch_bounds__(256) void bcell_clear_kernel(const Rep* __restrict__ reps,
+                                                          const unsigned int* __restrict__ cell_rep,
+                                                          RepState* __restrict__ rst,
+                                                          unsigned int ncell,
+                                                          unsigned int* __restrict__ cell_size)
+{
+    const unsigned int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell)
+        return;
+    const unsigned int r = cell_rep[c];
+    if (!rst[r].rebuild)
+        return;
+    cell_size[c] = 0;
+    if (c == reps[r].cell_off)
+        rst[r].n_builds++;
+}
+
+__device__ __forceinline__ unsigned int cell_coord(double x, const Rep& R)
+{
+    // CellList::computeCellList bin (hoomd/CellList.cc:466-490) for a cubic box at the origin:
+    // floor(fractional * dim), the upper edge folded back
+    int c = (int)((x + R.hi) * R.cell_inv);
+    if (c >= (int)R.dim)
+        c = 0;
+    if (c < 0)
+        c = 0;
+    return (unsigned int)c;
+}
+
+__global__ __launch_bounds__(BTILE) void bcell_bin_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
+    const RepState* __restrict__ rst, const double4* __restrict__ pos, d3* __restrict__ last,
+    unsigned int cmax, unsigned int* __restrict__ cell_size, unsigned int* __restrict__ cell_idx,
+    BCond* __restrict__ cond)
+{
+    const unsigned int t = blockIdx.x;
+    const unsigned int r = tile_rep[t];
+    if (!rst[r].rebuild)
+        return;
+    const Rep R = reps[r];
+    unsigned int first, count;
+    tile_range(R, t, first, count);
+    if (threadIdx.x >= count)
+        return;
+    const unsigned int p = first + threadIdx.x;
+    const double4 x = pos[p];
+    last[p] = d3{x.x, x.y, x.z};
+    const unsigned int c = R.cell_off
+                           + (cell_coord(x.z, R) * R.dim + cell_coord(x.y, R)) * R.dim
+                           + cell_coord(x.x, R);
+    const unsigned int slot = atomicAdd(&cell_size[c], 1u);
+    if (slot < cmax)
+        cell_idx[(size_t)c * cmax + slot] = p;
+    else
+        atomicMax(&cond->cell_need, slot + 1);
+}
+
+// -------------------------------------------------------------------------- list build
+// One lane per particle over the unique adjacent cells: offsets {-1, 0, 1} per axis when the
+// axis has at least 3 cells, {0, 1} for 2 cells, {0} for 1 (CellList::initializeCellAdj keeps
+// each adjacent cell once). Candidate j != i is kept when its FP64 minimum-image r^2 <= r_list^2.
+__global__ __launch_bounds__(BTILE) void bnlist_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
+    const RepState* __restrict__ rst, const double4* __restrict__ pos, unsigned int cmax,
+    const unsigned int* __restrict__ cell_size, const unsigned int* __restrict__ cell_idx,
+    double rlistsq, unsigned int nmax, unsigned int* __restrict__ nlist,
+    unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond)
+{
+#pragma clang fp contract(off)
+    const unsigned int t = blockIdx.x;
+    const unsigned int r = tile_rep[t];
+    if (!rst[r].rebuild)
+        return;
+    const Rep R = reps[r];
+    unsigned int first, count;
+    tile_range(R, t, first, count);
+    if (threadIdx.x >= count)
+        return;
+    const unsigned int p = first + threadIdx.x;
+    const double4 xi = pos[p];
+    const int cx = (int)cell_coord(xi.x, R), cy = (int)cell_coord(xi.y, R),
+              cz = (int)cell_coord(xi.z, R);
+    const int d = (int)R.dim;
+    const int lo = d >= 3 ? -1 : 0;
+    const int hi = d >= 2 ? 1 : 0;
+    unsigned int k = 0;
+    unsigned int* row = nlist + (size_t)p * nmax;
+    for (int oz = lo; oz <= hi; oz++)
+    {
+        const int z = (cz + oz + d) % d;
+        for (int oy = lo; oy <= hi; oy++)
+        {
+            const int y = (cy + oy + d) % d;
+            for (int ox = lo; ox <= hi; ox++)
+            {
+                const int x = (cx + ox + d) % d;
+                const unsigned int c = R.cell_off + ((unsigned int)z * R.dim + y) * R.dim + x;
+                const unsigned int sz = min(cell_size[c], cmax);
+                const unsigned int* cl = cell_idx + (size_t)c * cmax;
+                for (unsigned int q = 0; q < sz; q++)
+                {
+                    const unsigned int j = cl[q];
+                    if (j == p)
+                        continue;
+                    const double4 xj = pos[j];
+                    double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    dz -= R.L * rint(dz * R.Linv);
+                    dy -= R.L * rint(dy * R.Linv);
+                    dx -= R.L * rint(dx * R.Linv);
+                    if (dx * dx + dy * dy + dz * dz <= rlistsq)
+                    {
+                        if (k < nmax)
+                            row[k] = j;
+                        k++;
+                    }
+                }
+            }
+        }
+    }
+    n_neigh[p] = min(k, nmax);
+    if (k > nmax)
+        atomicMax(&cond->nlist_need, k);
+}
+
+// -------------------------------------------------------------------------- pair forces
+// tpp lanes per particle split its row; FP64 separation and exact cutoff test, FP32 evaluator
+// arithmetic and FP32 per-particle sums (force, pair energy halved as PotentialPair.h:739-772).
+template<unsigned int TPP>
+__global__ __launch_bounds__(256) void bpair_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ part_rep, unsigned int N,
+    const double4* __restrict__ pos, const unsigned int* __restrict__ nlist, unsigned int nmax,
+    const unsigned int* __restrict__ n_neigh, PairConst pc, float4* __restrict__ force)
+{
+#pragma clang fp contract(off)
+    const unsigned int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int p = g / TPP;
+    const unsigned int sub = g % TPP;
+    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
+    if (p < N)
+    {
+        const Rep R = reps[part_rep[p]];
+        const double4 xi = pos[p];
+        const unsigned int nn = n_neigh[p];
+        const unsigned int* row = nlist + (size_t)p * nmax;
+        for (unsigned int k = sub; k < nn; k += TPP)
+        {
+            const unsigned int j = row[k];
+            const double4 xj = pos[j];
+            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            dz -= R.L * rint(dz * R.Linv);
+            dy -= R.L * rint(dy * R.Linv);
+            dx -= R.L * rint(dx * R.Linv);
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq < pc.rcutsq)
+            {
+                const float r2inv = recip((float)rsq);
+                const float r6inv = r2inv * r2inv * r2inv;
+                const float fdr = r2inv * r6inv * (12.f * pc.lj1 * r6inv - 6.f * pc.lj2);
+                e += r6inv * (pc.lj1 * r6inv - pc.lj2);
+                fx += (float)dx * fdr;
+                fy += (float)dy * fdr;
+                fz += (float)dz * fdr;
+            }
+        }
+    }
+    for (unsigned int o = TPP / 2; o > 0; o >>= 1)
+    {
+        fx += __shfl_xor(fx, o, 64);
+        fy += __shfl_xor(fy, o, 64);
+        fz += __shfl_xor(fz, o, 64);
+        e += __shfl_xor(e, o, 64);
+    }
+    if (p < N && sub == 0)
+        force[p] = make_float4(fx, fy, fz, 0.5f * e);
+}
+
+// -------------------------------------------------------------------------- step two
+__global__ __launch_bounds__(BTILE) void bstep_two_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
+    RepState* __restrict__ rst, BatchStep* __restrict__ bst, const float4* __restrict__ force,
+    double4* __restrict__ vel, d3* __restrict__ accel, double* __restrict__ ke_partial, double dt,
+    int advance)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    const unsigned int t = blockIdx.x;
+    const unsigned int r = tile_rep[t];
+    const Rep R = reps[r];
+    unsigned int first, count;
+    tile_range(R, t, first, count);
+    double ke = 0.0;
+    if (threadIdx.x < count)
+    {
+        const unsigned int p = first + threadIdx.x;
+        double4 v = vel[p];
+        const float4 f = force[p];
+        const double minv = 1.0 / v.w;
+        d3 a{(double)f.x * minv, (double)f.y * minv, (double)f.z * minv};
+        if (advance)
+        {
+            v.x += (1.0 / 2.0) * dt * a.x;
+            v.y += (1.0 / 2.0) * dt * a.y;
+            v.z += (1.0 / 2.0) * dt * a.z;
+            vel[p] = v;
+        }
+        accel[p] = a;
+        ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    const double tot = block_reduce(ke, sh);
+    if (threadIdx.x == 0)
+    {
+        ke_partial[t] = tot;
+        if (t == R.tile_off)
+            rst[r].rebuild = 0;
+        if (t == 0 && advance)
+            bst->timestep++;
+    }
+}
+
+__global__ void bset_rebuild_kernel(RepState* rst, unsigned int nrep)
+{
+    const unsigned int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrep)
+        rst[r].rebuild = 1;
+}
+
+} // namespace
+
+// ============================================================================ host object
+struct hm_batch
+{
+    unsigned int nrep = 0, N = 0, ntile = 0, ncell = 0;
+    std::vector<Rep> hrep;
+    double dt = 0.005, tau = 0.0, r_cut = 2.5, r_buff = 0.4, rlistsq = 0.0, maxsq = 0.0;
+    PairConst pc{};
+    unsigned int cmax = 0, nmax = 0;
+    unsigned int pair_tpp = 4, pair_block = 256;
+    bool tuned = false;
+    unsigned int chunk = 100;
+    uint64_t timestep = 0, steps_run = 0, rollbacks = 0;
+    bool prepared = false;
+    hipStream_t stream = nullptr;
+
+    Rep* reps = nullptr;
+    RepState* rst = nullptr;
+    BatchStep* bst = nullptr;
+    BCond* cond = nullptr;
+    BCond* h_cond = nullptr;
+    unsigned int* tile_rep = nullptr;
+    unsigned int* cell_rep = nullptr;
+    unsigned int* part_rep = nullptr;
+    double4* pos = nullptr;
+    double4* vel = nullptr;
+    d3* accel = nullptr;
+    i3* image = nullptr;
+    d3* last = nullptr;
+    float4* force = nullptr;
+    double* ke_partial = nullptr;
+    unsigned int* cell_size = nullptr;
+    unsigned int* cell_idx = nullptr;
+    unsigned int* nlist = nullptr;
+    unsigned int* n_neigh = nullptr;
+    // chunk-start snapshot for overflow replay
+    double4 *s_pos = nullptr, *s_vel = nullptr;
+    d3 *s_accel = nullptr, *s_last = nullptr;
+    i3* s_image = nullptr;
+    double* s_ke = nullptr;
+    RepState* s_rst = nullptr;
+    BatchStep* s_bst = nullptr;
+
+    std::map<unsigned int, hipGraphExec_t> graphs;
+    std::vector<float> tune_log; // rows of (tpp, block, ms)
+
+    ~hm_batch()
+    {
+        clear_graphs();
+        for (void** p : {(void**)&reps, (void**)&rst, (void**)&bst, (void**)&cond,
+                         (void**)&tile_rep, (void**)&cell_rep, (void**)&part_rep, (void**)&pos,
+                         (void**)&vel, (void**)&accel, (void**)&image, (void**)&last,
+                         (void**)&force, (void**)&ke_partial, (void**)&cell_size,
+                         (void**)&cell_idx, (void**)&nlist, (void**)&n_neigh, (void**)&s_pos,
+                         (void**)&s_vel, (void**)&s_accel, (void**)&s_last, (void**)&s_image,
+                         (void**)&s_ke, (void**)&s_rst, (void**)&s_bst})
+            if (*p)
+            {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        if (h_cond)
+            (void)hipHostFree(h_cond);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    void clear_graphs()
+    {
+        for (auto& kv : graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
+    void alloc_lists()
+    {
+        bfree(cell_idx);
+        bfree(nlist);
+        cell_idx = balloc<unsigned int>((size_t)ncell * cmax);
+        nlist = balloc<unsigned int>((size_t)N * nmax);
+        clear_graphs();
+    }
+
+    // ---------------------------------------------------------------- launches
+    void launch_pair()
+    {
+        const unsigned int nthreads = N * pair_tpp;
+        const dim3 grid((nthreads + pair_block - 1) / pair_block), block(pair_block);
+        switch (pair_tpp)
+        {
+        case 1:
+            hipLaunchKernelGGL(bpair_kernel<1>, grid, block, 0, stream, reps, part_rep, N, pos,
+                               nlist, nmax, n_neigh, pc, force);
+            break;
+        case 2:
+            hipLaunchKernelGGL(bpair_kernel<2>, grid, block, 0, stream, reps, part_rep, N, pos,
+                               nlist, nmax, n_neigh, pc, force);
+            break;
+        case 4:
+            hipLaunchKernelGGL(bpair_kernel<4>, grid, block, 0, stream, reps, part_rep, N, pos,
+                               nlist, nmax, n_neigh, pc, force);
+            break;
+        case 8:
+            hipLaunchKernelGGL(bpair_kernel<8>, grid, block, 0, stream, reps, part_rep, N, pos,
+                               nlist, nmax, n_neigh, pc, force);
+            break;
+        default:
+            hipLaunchKernelGGL(bpair_kernel<16>, grid, block, 0, stream, reps, part_rep, N, pos,
+                               nlist, nmax, n_neigh, pc, force);
+            break;
+        }
+        bchk(hipGetLastError(), "batch pair");
+    }
+
+    void launch_rebuild()
+    {
+        hipLaunchKernelGGL(bcell_clear_kernel, dim3((ncell + 255) / 256), dim3(256), 0, stream,
+                           reps, cell_rep, rst, ncell, cell_size);
+        hipLaunchKernelGGL(bcell_bin_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
+                           rst, pos, last, cmax, cell_size, cell_idx, cond);
+        hipLaunchKernelGGL(bnlist_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep, rst,
+                           pos, cmax, cell_size, cell_idx, rlistsq, nmax, nlist, n_neigh, cond);
+        bchk(hipGetLastError(), "batch rebuild");
+    }
+
+    void launch_step()
+    {
+        hipLaunchKernelGGL(bstep_one_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
+                           rst, bst, ke_partial, pos, vel, accel, image, last, cond, dt, tau, 1,
+                           maxsq);
+        bchk(hipGetLastError(), "batch step one");
+        launch_rebuild();
+        launch_pair();
+        hipLaunchKernelGGL(bstep_two_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
+                           rst, bst, force, vel, accel, ke_partial, dt, 1);
+        bchk(hipGetLastError(), "batch step two");
+    }
+
+    // forces, accelerations and kinetic-energy partials of the current state (all lists rebuilt)
+    void prep()
+    {
+        hipLaunchKernelGGL(bset_rebuild_kernel, dim3((nrep + 255) / 256), dim3(256), 0, stream,
+                           rst, nrep);
+        bchk(hipMemsetAsync(cond, 0, sizeof(BCond), stream), "clear conditions");
+        launch_rebuild();
+        launch_pair();
+        hipLaunchKernelGGL(bstep_two_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
+                           rst, bst, force, vel, accel, ke_partial, dt, 0);
+        bchk(hipGetLastError(), "batch prep");
+        read_cond();
+        if (h_cond->cell_need > cmax || h_cond->nlist_need > nmax)
+        {
+            cmax = std::max(cmax, round_up(h_cond->cell_need + 8, 8));
+            nmax = std::max(nmax, round_up(h_cond->nlist_need + 16, 16));
+            alloc_lists();
+            return prep();
+        }
+        prepared = true;
+    }
+
+    void read_cond()
+    {
+        bchk(hipMemcpyAsync(h_cond, cond, sizeof(BCond), hipMemcpyDeviceToHost, stream), "cond");
+        bchk(hipStreamSynchronize(stream), "sync");
+    }
+
+    hipGraphExec_t capture_steps(unsigned int K)
+    {
+        hipGraph_t g;
+        bchk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try
+        {
+            for (unsigned int k = 0; k < K; k++)
+                launch_step();
+        }
+        catch (...)
+        {
+            (void)hipStreamEndCapture(stream, &g);
+            throw;
+        }
+        bchk(hipStreamEndCapture(stream, &g), "end capture");
+        hipGraphExec_t ex;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        bchk(e, "graph instantiate");
+        return ex;
+    }
+
+    void snapshot()
+    {
+        if (!s_pos)
+        {
+            s_pos = balloc<double4>(N);
+            s_vel = balloc<double4>(N);
+            s_accel = balloc<d3>(N);
+            s_last = balloc<d3>(N);
+            s_image = balloc<i3>(N);
+            s_ke = balloc<double>(ntile);
+            s_rst = balloc<RepState>(nrep);
+            s_bst = balloc<BatchStep>(1);
+        }
+        auto cp = [&](void* d, const void* s, size_t b) {
+            bchk(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream), "snapshot");
+        };
+        cp(s_pos, pos, (size_t)N * sizeof(double4));
+        cp(s_vel, vel, (size_t)N * sizeof(double4));
+        cp(s_accel, accel, (size_t)N * sizeof(d3));
+        cp(s_last, last, (size_t)N * sizeof(d3));
+        cp(s_image, image, (size_t)N * sizeof(i3));
+        cp(s_ke, ke_partial, (size_t)ntile * sizeof(double));
+        cp(s_rst, rst, (size_t)nrep * sizeof(RepState));
+        cp(s_bst, bst, sizeof(BatchStep));
+    }
+
+    void restore()
+    {
+        auto cp = [&](void* d, const void* s, size_t b) {
+            bchk(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream), "restore");
+        };
+        cp(pos, s_pos, (size_t)N * sizeof(double4));
+        cp(vel, s_vel, (size_t)N * sizeof(double4));
+        cp(accel, s_accel, (size_t)N * sizeof(d3));
+        cp(last, s_last, (size_t)N * sizeof(d3));
+        cp(image, s_image, (size_t)N * sizeof(i3));
+        cp(ke_partial, s_ke, (size_t)ntile * sizeof(double));
+        cp(rst, s_rst, (size_t)nrep * sizeof(RepState));
+        cp(bst, s_bst, sizeof(BatchStep));
+    }
+
+    uint64_t chunk_len(uint64_t t, uint64_t end) const
+    {
+        return std::min<uint64_t>(chunk ? chunk : 1, end - t);
+    }
+
+    int prime(uint64_t nsteps)
+    {
+        if (!prepared)
+            prep();
+        int added = 0;
+        for (uint64_t t = timestep, end = timestep + nsteps; t < end;)
+        {
+            const unsigned int K = (unsigned int)chunk_len(t, end);
+            if (graphs.find(K) == graphs.end())
+            {
+                graphs.emplace(K, capture_steps(K));
+                added++;
+            }
+            t += K;
+        }
+        return added;
+    }
+
+    int run(uint64_t nsteps)
+    {
+        if (!prepared)
+            prep();
+        const uint64_t end = timestep + nsteps;
+        int retries = 0;
+        while (timestep < end)
+        {
+            const unsigned int K = (unsigned int)chunk_len(timestep, end);
+            bchk(hipMemsetAsync(cond, 0, sizeof(BCond), stream), "clear conditions");
+            snapshot();
+            auto it = graphs.find(K);
+            if (it == graphs.end())
+                it = graphs.emplace(K, capture_steps(K)).first;
+            bchk(hipGraphLaunch(it->second, stream), "graph launch");
+            read_cond();
+            const BCond c = *h_cond;
+            if (c.cell_need > cmax || c.nlist_need > nmax)
+            {
+                // regrow and replay the chunk from its start (the reference's regrow-and-repeat,
+                // NeighborList.cc:411-427, CellList.cc:183-196)
+                restore();
+                if (c.cell_need > cmax)
+                    cmax = round_up(c.cell_need + 8, 8);
+                if (c.nlist_need > nmax)
+                    nmax = round_up(c.nlist_need + 16, 16);
+                alloc_lists();
+                hipLaunchKernelGGL(bset_rebuild_kernel, dim3((nrep + 255) / 256), dim3(256), 0,
+                                   stream, rst, nrep);
+                bchk(hipGetLastError(), "force rebuild");
+                rollbacks++;
+                if (++retries > 16)
+                    throw BError{HM_ERR_LAUNCH, "batch: neighbour list kept overflowing"};
+                continue;
+            }
+            if (c.nan_idx)
+            {
+                restore();
+                bchk(hipStreamSynchronize(stream), "sync");
+                throw BError{HM_ERR_NAN, "batch: particle position is NaN"};
+            }
+            retries = 0;
+            timestep += K;
+            steps_run += K;
+        }
+        std::vector<RepState> h(nrep);
+        bchk(hipMemcpy(h.data(), rst, sizeof(RepState) * nrep, hipMemcpyDeviceToHost), "state");
+        for (unsigned int r = 0; r < nrep; r++)
+            if (h[r].error & ERR_ZERO_KE)
+                throw BError{HM_ERR_THERMO,
+                             "batch: Bussi thermostat requires non-zero initial momenta"};
+        return HM_OK;
+    }
+
+    // ---------------------------------------------------------------- autotuner
+    // Times every (tpp, block) of the pair kernel on the current lists (median of 5 launches)
+    // and keeps the fastest. The choice is cached per (total particles, mean density) bucket for
+    // the life of the process, so batches of the same shape skip the sweep.
+    float time_pair(unsigned int iters)
+    {
+        hipEvent_t a, b;
+        bchk(hipEventCreate(&a), "event");
+        bchk(hipEventCreate(&b), "event");
+        launch_pair();
+        bchk(hipEventRecord(a, stream), "event");
+        for (unsigned int k = 0; k < iters; k++)
+            launch_pair();
+        bchk(hipEventRecord(b, stream), "event");
+        bchk(hipEventSynchronize(b), "event sync");
+        float ms = 0.f;
+        bchk(hipEventElapsedTime(&ms, a, b), "elapsed");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return ms / (float)iters;
+    }
+
+    void tune();
+};
+
+namespace
+{
+struct TuneKey
+{
+    unsigned int n_bucket, rho_bucket;
+    bool operator<(const TuneKey& o) const
+    {
+        return n_bucket != o.n_bucket ? n_bucket < o.n_bucket : rho_bucket < o.rho_bucket;
+    }
+};
+std::map<TuneKey, std::pair<unsigned int, unsigned int>> g_batch_tune_cache;
+
+TuneKey tune_key(const hm_batch& b)
+{
+    double vol = 0.0;
+    for (const Rep& R : b.hrep)
+        vol += R.L * R.L * R.L;
+    const double rho = b.N / vol;
+    return TuneKey{(unsigned int)std::lround(std::log2((double)b.N) * 2.0),
+                   (unsigned int)std::lround(rho * 20.0)};
+}
+} // namespace
+
+void hm_batch::tune()
+{
+    if (!prepared)
+        prep();
+    const TuneKey key = tune_key(*this);
+    auto it = g_batch_tune_cache.find(key);
+    tune_log.clear();
+    if (it != g_batch_tune_cache.end())
+    {
+        pair_tpp = it->second.first;
+        pair_block = it->second.second;
+        tuned = true;
+        return;
+    }
+    float best = 1e30f;
+    unsigned int bt = pair_tpp, bb = pair_block;
+    for (unsigned int tpp : {1u, 2u, 4u, 8u, 16u})
+        for (unsigned int blk : {64u, 128u, 256u})
+        {
+            pair_tpp = tpp;
+            pair_block = blk;
+            float s[5];
+            for (float& v : s)
+                v = time_pair(2);
+            std::sort(s, s + 5);
+            tune_log.insert(tune_log.end(), {(float)tpp, (float)blk, s[2]});
+            if (s[2] < best)
+            {
+                best = s[2];
+                bt = tpp;
+                bb = blk;
+            }
+        }
+    pair_tpp = bt;
+    pair_block = bb;
+    tuned = true;
+    g_batch_tune_cache[key] = {bt, bb};
+    clear_graphs();
+}
+
+// ============================================================================ C ABI
+namespace
+{
+template<class F> int bguard(F&& f)
+{
+    try
+    {
+        return f();
+    }
+    catch (const BError& e)
+    {
+        return fail(e.code, e.what.c_str());
+    }
+    catch (const std::bad_alloc&)
+    {
+        return fail(HM_ERR_ALLOC, "host allocation failed");
+    }
+}
+} // namespace
+
+extern "C"
+{
+
+hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double* L,
+                          const double* kT, const uint16_t* seed, const double* pos3,
+                          const double* vel3, const double* mass, double epsilon, double sigma,
+                          double r_cut, double r_buff, double dt, double tau)
+{
+    if (nrep == 0 || !n || !L || !kT || !seed || !pos3 || !(r_cut > 0.0) || !(r_buff >= 0.0)
+        || !(dt > 0.0) || !(tau >= 0.0))
+    {
+        fail(HM_ERR_ARGS, "hm_batch_create: invalid arguments");
+        return nullptr;
+    }
+    hm_batch* b = new (std::nothrow) hm_batch();
+    if (!b)
+        return nullptr;
+    int r = bguard([&]() {
+        const double rlist = r_cut + r_buff;
+        b->nrep = nrep;
+        b->dt = dt;
+        b->tau = tau;
+        b->r_cut = r_cut;
+        b->r_buff = r_buff;
+        b->rlistsq = rlist * rlist;
+        b->maxsq = distance_check_maxsq(r_cut, r_buff);
+        // EvaluatorPairLJ parameters (lj1 = 4 eps sigma^12, lj2 = 4 eps sigma^6)
+        const double s6 = sigma * sigma * sigma * sigma * sigma * sigma;
+        b->pc.lj1 = (float)(4.0 * epsilon * s6 * s6);
+        b->pc.lj2 = (float)(4.0 * epsilon * s6);
+        b->pc.rcutsq = r_cut * r_cut;
+        b->pc.rcutsq_f = (float)(r_cut * r_cut);
+        b->hrep.resize(nrep);
+        unsigned long long off = 0, coff = 0, toff = 0;
+        double rho_max = 0.0;
+        for (unsigned int q = 0; q < nrep; q++)
+        {
+            Rep& R = b->hrep[q];
+            if (n[q] == 0 || !(L[q] > 0.0) || !(kT[q] >= 0.0))
+                throw BError{HM_ERR_ARGS, "hm_batch_create: replica with no particles or box"};
+            // NeighborList::checkBoxSize (hoomd/md/NeighborList.cc:538-561): the list range
+            // must be below half the box
+            if (L[q] <= 2.0 * rlist)
+                throw BError{HM_ERR_BOXSMALL,
+                             "hm_batch_create: simulation box too small for the neighbour list"};
+            R.L = L[q];
+            R.Linv = 1.0 / L[q];
+            R.hi = L[q] / 2.0;
+            R.kT = kT[q];
+            R.ndof = 3.0 * n[q] - 3.0;
+            // CellList::computeDimensions (hoomd/CellList.cc:83-131): floor(L / r_list) cells
+            R.dim = std::max(1u, (unsigned int)std::floor(L[q] / rlist));
+            R.cell_inv = R.dim / L[q];
+            R.off = (unsigned int)off;
+            R.n = n[q];
+            R.cell_off = (unsigned int)coff;
+            R.tile_off = (unsigned int)toff;
+            R.ntile = (n[q] + BTILE - 1) / BTILE;
+            R.seed = seed[q];
+            off += n[q];
+            coff += (unsigned long long)R.dim * R.dim * R.dim;
+            toff += R.ntile;
+            rho_max = std::max(rho_max, n[q] / (L[q] * L[q] * L[q]));
+            if (off >= (1ull << 31) || coff >= (1ull << 31))
+                throw BError{HM_ERR_ARGS, "hm_batch_create: too many particles or cells"};
+        }
+        b->N = (unsigned int)off;
+        b->ncell = (unsigned int)coff;
+        b->ntile = (unsigned int)toff;
+        // first list sizes from the densest replica; overflow regrows them
+        const double cellvol = rlist * rlist * rlist;
+        b->cmax = round_up((unsigned int)(rho_max * cellvol * 1.5) + 16, 8);
+        b->nmax = round_up((unsigned int)(4.18879 * rlist * rlist * rlist * rho_max * 1.4) + 32,
+                           16);
+        bchk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+        const unsigned int N = b->N;
+        b->reps = balloc<Rep>(nrep);
+        b->rst = balloc<RepState>(nrep);
+        b->bst = balloc<BatchStep>(1);
+        b->cond = balloc<BCond>(1);
+        if (hipHostMalloc((void**)&b->h_cond, sizeof(BCond), hipHostMallocDefault) != hipSuccess)
+            throw BError{HM_ERR_ALLOC, "pinned allocation failed"};
+        b->tile_rep = balloc<unsigned int>(b->ntile);
+        b->cell_rep = balloc<unsigned int>(b->ncell);
+        b->part_rep = balloc<unsigned int>(N);
+        b->pos = balloc<double4>(N);
+        b->vel = balloc<double4>(N);
+        b->accel = balloc<d3>(N);
+        b->image = balloc<i3>(N);
+        b->last = balloc<d3>(N);
+        b->force = balloc<float4>(N);
+        b->ke_partial = balloc<double>(b->ntile);
+        b->cell_size = balloc<unsigned int>(b->ncell);
+        b->n_neigh = balloc<unsigned int>(N);
+        b->alloc_lists();
+        std::vector<unsigned int> trep(b->ntile), crep(b->ncell), prep_(N);
+        std::vector<double4> hp(N), hv(N);
+        for (unsigned int q = 0; q < nrep; q++)
+        {
+            const Rep& R = b->hrep[q];
+            for (unsigned int t = 0; t < R.ntile; t++)
+                trep[R.tile_off + t] = q;
+            for (unsigned int c = 0; c < R.dim * R.dim * R.dim; c++)
+                crep[R.cell_off + c] = q;
+            for (unsigned int i = 0; i < R.n; i++)
+            {
+                const size_t p = R.off + i;
+                double x[3];
+                for (int d = 0; d < 3; d++)
+                {
+                    // positions are wrapped into [-L/2, L/2) as the snapshot reader does
+                    double w = pos3[3 * p + d];
+                    w -= R.L * std::floor((w + R.hi) * R.Linv);
+                    if (w >= R.hi)
+                        w -= R.L;
+                    x[d] = w;
+                }
+                hp[p] = make_double4(x[0], x[1], x[2], 0.0);
+                hv[p] = make_double4(vel3 ? vel3[3 * p] : 0.0, vel3 ? vel3[3 * p + 1] : 0.0,
+                                     vel3 ? vel3[3 * p + 2] : 0.0, mass ? mass[p] : 1.0);
+                prep_[p] = q;
+            }
+        }
+        auto up = [&](void* d, const void* s, size_t bytes) {
+            bchk(hipMemcpy(d, s, bytes, hipMemcpyHostToDevice), "upload");
+        };
+        up(b->reps, b->hrep.data(), sizeof(Rep) * nrep);
+        up(b->tile_rep, trep.data(), sizeof(unsigned int) * b->ntile);
+        up(b->cell_rep, crep.data(), sizeof(unsigned int) * b->ncell);
+        up(b->part_rep, prep_.data(), sizeof(unsigned int) * N);
+        up(b->pos, hp.data(), sizeof(double4) * N);
+        up(b->vel, hv.data(), sizeof(double4) * N);
+        bchk(hipMemset(b->image, 0, sizeof(i3) * N), "clear");
+        bchk(hipMemset(b->accel, 0, sizeof(d3) * N), "clear");
+        bchk(hipMemset(b->rst, 0, sizeof(RepState) * nrep), "clear");
+        bchk(hipMemset(b->bst, 0, sizeof(BatchStep)), "clear");
+        bchk(hipMemset(b->cell_size, 0, sizeof(unsigned int) * b->ncell), "clear");
+        return HM_OK;
+    });
+    if (r != HM_OK)
+    {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+void hm_batch_destroy(hm_batch* b)
+{
+    delete b;
+}
+
+int hm_batch_thermalize(hm_batch* b, uint64_t timestep)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_thermalize: NULL batch");
+    return bguard([&]() {
+        // ParticleGroup::thermalizeParticleMomenta per replica: tags 0..n-1 of the replica, its
+        // seed and kT, its own COM momentum removed
+        unsigned int nmax_rep = 0;
+        for (const Rep& R : b->hrep)
+            nmax_rep = std::max(nmax_rep, R.n);
+        unsigned int* tags = balloc<unsigned int>(nmax_rep);
+        double* scratch = balloc<double>(THERMO_BLOCKS * 3);
+        std::vector<unsigned int> ht(nmax_rep);
+        for (unsigned int i = 0; i < nmax_rep; i++)
+            ht[i] = i;
+        hipError_t e = hipMemcpy(tags, ht.data(), sizeof(unsigned int) * nmax_rep,
+                                 hipMemcpyHostToDevice);
+        for (const Rep& R : b->hrep)
+        {
+            if (e != hipSuccess)
+                break;
+            e = launch_thermalize(R.n, b->vel + R.off, tags, R.kT, timestep, (uint16_t)R.seed,
+                                  scratch, b->stream);
+        }
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(b->stream);
+        bfree(tags);
+        bfree(scratch);
+        bchk(e, "thermalize");
+        b->prepared = false;
+        return HM_OK;
+    });
+}
+
+int hm_batch_set_timestep(hm_batch* b, uint64_t timestep)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_set_timestep: NULL batch");
+    return bguard([&]() {
+        BatchStep s{timestep};
+        bchk(hipMemcpy(b->bst, &s, sizeof(s), hipMemcpyHostToDevice), "timestep");
+        b->timestep = timestep;
+        return HM_OK;
+    });
+}
+
+int hm_batch_run(hm_batch* b, uint64_t nsteps)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_run: NULL batch");
+    return bguard([&]() { return b->run(nsteps); });
+}
+
+int hm_batch_prime(hm_batch* b, uint64_t nsteps)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_prime: NULL batch");
+    return bguard([&]() { return b->prime(nsteps); });
+}
+
+int hm_batch_set_chunk(hm_batch* b, unsigned int steps)
+{
+    if (!b || steps == 0)
+        return fail(HM_ERR_ARGS, "hm_batch_set_chunk: invalid arguments");
+    b->chunk = steps;
+    return HM_OK;
+}
+
+int hm_batch_tune(hm_batch* b, float* log3, unsigned int max_rows)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_tune: NULL batch");
+    return bguard([&]() {
+        b->tune();
+        const unsigned int rows = (unsigned int)(b->tune_log.size() / 3);
+        if (log3)
+            std::memcpy(log3, b->tune_log.data(), sizeof(float) * 3 * std::min(rows, max_rows));
+        return (int)rows;
+    });
+}
+
+int hm_batch_set_pair_launch(hm_batch* b, unsigned int tpp, unsigned int block)
+{
+    if (!b || (tpp != 1 && tpp != 2 && tpp != 4 && tpp != 8 && tpp != 16)
+        || (block != 64 && block != 128 && block != 256))
+        return fail(HM_ERR_ARGS, "hm_batch_set_pair_launch: tpp in {1,2,4,8,16}, block in "
+                                 "{64,128,256}");
+    b->pair_tpp = tpp;
+    b->pair_block = block;
+    b->clear_graphs();
+    return HM_OK;
+}
+
+int hm_batch_config(const hm_batch* b, unsigned int out[6])
+{
+    if (!b || !out)
+        return fail(HM_ERR_ARGS, "hm_batch_config: invalid arguments");
+    out[0] = b->pair_tpp;
+    out[1] = b->pair_block;
+    out[2] = b->tuned ? 1u : 0u;
+    out[3] = b->cmax;
+    out[4] = b->nmax;
+    out[5] = b->ncell;
+    return HM_OK;
+}
+
+int hm_batch_get_state(hm_batch* b, double* pos3, double* vel3, int* image3)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_get_state: NULL batch");
+    return bguard([&]() {
+        const unsigned int N = b->N;
+        std::vector<double4> p(N), v(N);
+        std::vector<i3> im(N);
+        bchk(hipStreamSynchronize(b->stream), "sync");
+        bchk(hipMemcpy(p.data(), b->pos, sizeof(double4) * N, hipMemcpyDeviceToHost), "pos");
+        bchk(hipMemcpy(v.data(), b->vel, sizeof(double4) * N, hipMemcpyDeviceToHost), "vel");
+        bchk(hipMemcpy(im.data(), b->image, sizeof(i3) * N, hipMemcpyDeviceToHost), "image");
+        for (unsigned int i = 0; i < N; i++)
+        {
+            if (pos3)
+            {
+                pos3[3 * i] = p[i].x;
+                pos3[3 * i + 1] = p[i].y;
+                pos3[3 * i + 2] = p[i].z;
+            }
+            if (vel3)
+            {
+                vel3[3 * i] = v[i].x;
+                vel3[3 * i + 1] = v[i].y;
+                vel3[3 * i + 2] = v[i].z;
+            }
+            if (image3)
+            {
+                image3[3 * i] = im[i].x;
+                image3[3 * i + 1] = im[i].y;
+                image3[3 * i + 2] = im[i].z;
+            }
+        }
+        return HM_OK;
+    });
+}
+
+int hm_batch_get_forces(hm_batch* b, double* force4)
+{
+    if (!b || !force4)
+        return fail(HM_ERR_ARGS, "hm_batch_get_forces: invalid arguments");
+    return bguard([&]() {
+        if (!b->prepared)
+            b->prep();
+        std::vector<float4> f(b->N);
+        bchk(hipStreamSynchronize(b->stream), "sync");
+        bchk(hipMemcpy(f.data(), b->force, sizeof(float4) * b->N, hipMemcpyDeviceToHost),
+             "force");
+        for (unsigned int i = 0; i < b->N; i++)
+        {
+            force4[4 * i] = f[i].x;
+            force4[4 * i + 1] = f[i].y;
+            force4[4 * i + 2] = f[i].z;
+            force4[4 * i + 3] = f[i].w;
+        }
+        return HM_OK;
+    });
+}
+
+// per replica: out[r * 4 + 0] = kinetic energy, +1 = potential energy, +2 = list builds since
+// creation, +3 = neighbour-list entries
+int hm_batch_replica_stats(hm_batch* b, double* out4)
+{
+    if (!b || !out4)
+        return fail(HM_ERR_ARGS, "hm_batch_replica_stats: invalid arguments");
+    return bguard([&]() {
+        if (!b->prepared)
+            b->prep();
+        const unsigned int N = b->N;
+        std::vector<double4> v(N);
+        std::vector<float4> f(N);
+        std::vector<unsigned int> nn(N);
+        std::vector<RepState> rs(b->nrep);
+        bchk(hipStreamSynchronize(b->stream), "sync");
+        bchk(hipMemcpy(v.data(), b->vel, sizeof(double4) * N, hipMemcpyDeviceToHost), "vel");
+        bchk(hipMemcpy(f.data(), b->force, sizeof(float4) * N, hipMemcpyDeviceToHost), "force");
+        bchk(hipMemcpy(nn.data(), b->n_neigh, sizeof(unsigned int) * N, hipMemcpyDeviceToHost),
+             "n_neigh");
+        bchk(hipMemcpy(rs.data(), b->rst, sizeof(RepState) * b->nrep, hipMemcpyDeviceToHost),
+             "state");
+        for (unsigned int q = 0; q < b->nrep; q++)
+        {
+            const Rep& R = b->hrep[q];
+            double ke = 0.0, pe = 0.0, ent = 0.0;
+            for (unsigned int i = R.off; i < R.off + R.n; i++)
+            {
+                ke += 0.5 * v[i].w * (v[i].x * v[i].x + v[i].y * v[i].y + v[i].z * v[i].z);
+                pe += f[i].w;
+                ent += nn[i];
+            }
+            out4[4 * q] = ke;
+            out4[4 * q + 1] = pe;
+            out4[4 * q + 2] = rs[q].n_builds;
+            out4[4 * q + 3] = ent;
+        }
+        return HM_OK;
+    });
+}
+
+// totals: out[0] particles, out[1] steps run, out[2] overflow replays, out[3] list builds (all
+// replicas)
+int hm_batch_stats(hm_batch* b, uint64_t out4[4])
+{
+    if (!b || !out4)
+        return fail(HM_ERR_ARGS, "hm_batch_stats: invalid arguments");
+    return bguard([&]() {
+        std::vector<RepState> rs(b->nrep);
+        bchk(hipStreamSynchronize(b->stream), "sync");
+        bchk(hipMemcpy(rs.data(), b->rst, sizeof(RepState) * b->nrep, hipMemcpyDeviceToHost),
+             "state");
+        uint64_t nb = 0;
+        for (const RepState& s : rs)
+            nb += s.n_builds;
+        out4[0] = b->N;
+        out4[1] = b->steps_run;
+        out4[2] = b->rollbacks;
+        out4[3] = nb;
+        return HM_OK;
+    });
+}
+
+// Replace the cell and row capacities (test hook: capacities below the need make the next
+// rebuild overflow, which the run loop answers by replaying the chunk with grown lists). Every
+// replica rebuilds its lists before the forces of the next step.
+int hm_batch_set_list_sizes(hm_batch* b, unsigned int cmax, unsigned int nmax)
+{
+    if (!b || cmax == 0 || nmax == 0)
+        return fail(HM_ERR_ARGS, "hm_batch_set_list_sizes: invalid arguments");
+    return bguard([&]() {
+        if (!b->prepared)
+            b->prep();
+        b->cmax = cmax;
+        b->nmax = nmax;
+        b->alloc_lists();
+        hipLaunchKernelGGL(bset_rebuild_kernel, dim3((b->nrep + 255) / 256), dim3(256), 0,
+                           b->stream, b->rst, b->nrep);
+        bchk(hipGetLastError(), "force rebuild");
+        bchk(hipStreamSynchronize(b->stream), "sync");
+        return HM_OK;
+    });
+}
+
+// Average duration of the pair launch over iters back-to-back launches on the batch stream.
+int hm_batch_time_pair(hm_batch* b, unsigned int iters, double* avg_ms)
+{
+    if (!b || !avg_ms || iters == 0)
+        return fail(HM_ERR_ARGS, "hm_batch_time_pair: invalid arguments");
+    return bguard([&]() {
+        if (!b->prepared)
+            b->prep();
+        *avg_ms = b->time_pair(iters);
+        return HM_OK;
+    });
+}
+
+} // extern "C"

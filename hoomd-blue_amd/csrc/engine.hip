@@ -1149,27 +1149,60 @@ struct hm_system
         }
         auto it = graphs.find(K);
         if (it == graphs.end())
-        {
-            hipGraph_t g;
-            hchk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            try
-            {
-                for (unsigned int k = 0; k < K; k++)
-                    launch_step();
-            }
-            catch (...)
-            {
-                (void)hipStreamEndCapture(stream, &g);
-                throw;
-            }
-            hchk(hipStreamEndCapture(stream, &g), "end capture");
-            hipGraphExec_t ex;
-            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            hchk(e, "graph instantiate");
-            it = graphs.emplace(K, ex).first;
-        }
+            it = graphs.emplace(K, capture_steps(K)).first;
         hchk(hipGraphLaunch(it->second, stream), "graph launch");
+    }
+
+    // K steps captured from the stream into an instantiated graph (nothing is launched)
+    hipGraphExec_t capture_steps(unsigned int K)
+    {
+        hipGraph_t g;
+        hchk(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try
+        {
+            for (unsigned int k = 0; k < K; k++)
+                launch_step();
+        }
+        catch (...)
+        {
+            (void)hipStreamEndCapture(stream, &g);
+            throw;
+        }
+        hchk(hipStreamEndCapture(stream, &g), "end capture");
+        hipGraphExec_t ex;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hchk(e, "graph instantiate");
+        return ex;
+    }
+
+    // Capture and instantiate the graphs of every chunk length that run(nsteps) from the
+    // current step would launch, without launching anything, so that a timed run that follows
+    // spends no time in capture or instantiation. Returns the number of graphs added.
+    int prime(uint64_t nsteps)
+    {
+        if (!prepared)
+            prep();
+        if (!use_graph)
+            return 0;
+        int added = 0;
+        uint64_t t = timestep, end = timestep + nsteps;
+        while (t < end)
+        {
+            uint64_t K = std::min<uint64_t>(chunk ? chunk : 1, end - t);
+            if (sort_period)
+            {
+                uint64_t next_sort = (t / sort_period + 1) * sort_period;
+                K = std::min<uint64_t>(K, next_sort - t);
+            }
+            if (graphs.find((unsigned int)K) == graphs.end())
+            {
+                graphs.emplace((unsigned int)K, capture_steps((unsigned int)K));
+                added++;
+            }
+            t += K;
+        }
+        return added;
     }
 
     int run(uint64_t nsteps)
@@ -1737,6 +1770,13 @@ int hm_sys_run(hm_system* s, uint64_t nsteps)
     if (!s)
         return fail(HM_ERR_ARGS, "hm_sys_run: NULL system");
     return guarded([&]() { return s->run(nsteps); });
+}
+
+int hm_sys_prime(hm_system* s, uint64_t nsteps)
+{
+    if (!s)
+        return fail(HM_ERR_ARGS, "hm_sys_prime: NULL system");
+    return guarded([&]() { return s->prime(nsteps); });
 }
 
 uint64_t hm_sys_timestep(const hm_system* s)
@@ -2349,6 +2389,46 @@ int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg
                 tot += std::min(v, s->nmax);
             *nlist_entries = tot;
         }
+        return HM_OK;
+    });
+}
+
+int hm_sys_time_mlp(hm_system* s, int force_id, unsigned int iters, double* avg_ms3)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size()
+        || s->forces[force_id].kind != 3 || iters == 0 || !avg_ms3)
+        return fail(HM_ERR_ARGS, "hm_sys_time_mlp: invalid arguments");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        const MlpArgs a = s->mlp_args(s->forces[force_id], s->compute_virial);
+        hipEvent_t e[4];
+        for (auto& x : e)
+            hchk(hipEventCreate(&x), "event");
+        // one untimed pass, then iters launches of each kernel in turn (each pass reads the
+        // previous one's output, as in a step)
+        hchk(launch_mlp_descriptor(a, s->stream), "mlp descriptor");
+        hchk(launch_mlp_site(a, s->stream), "mlp site energies");
+        hchk(launch_mlp_force(a, false, s->stream), "mlp forces");
+        hchk(hipEventRecord(e[0], s->stream), "event");
+        for (unsigned int q = 0; q < iters; q++)
+            hchk(launch_mlp_descriptor(a, s->stream), "mlp descriptor");
+        hchk(hipEventRecord(e[1], s->stream), "event");
+        for (unsigned int q = 0; q < iters; q++)
+            hchk(launch_mlp_site(a, s->stream), "mlp site energies");
+        hchk(hipEventRecord(e[2], s->stream), "event");
+        for (unsigned int q = 0; q < iters; q++)
+            hchk(launch_mlp_force(a, false, s->stream), "mlp forces");
+        hchk(hipEventRecord(e[3], s->stream), "event");
+        hchk(hipEventSynchronize(e[3]), "event");
+        for (int k = 0; k < 3; k++)
+        {
+            float ms = 0.f;
+            hchk(hipEventElapsedTime(&ms, e[k], e[k + 1]), "event");
+            avg_ms3[k] = (double)ms / iters;
+        }
+        for (auto& x : e)
+            (void)hipEventDestroy(x);
         return HM_OK;
     });
 }

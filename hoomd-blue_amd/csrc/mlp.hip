@@ -827,6 +827,42 @@ __global__ __launch_bounds__(256) void map_sites_kernel(SiteMapArgs a)
     }
 }
 
+// Frame forces minus harmonic bond forces, per (frame, site): out_i = F_i - sum over the bonded
+// partners j of k (r0 / r - 1) d, d = minimum image of p_i - p_j (PotentialBond with
+// EvaluatorBondHarmonic, hoomd/md/EvaluatorBondHarmonic.h:105-123, force on the first endpoint;
+// a zero distance gives no force, as the evaluator zeroes a non-finite force_divr).
+__global__ __launch_bounds__(256) void frames_sub_harmonic_kernel(
+    unsigned int nframes, unsigned int nsites, const float* __restrict__ fpos,
+    const float* fforce, float* out, const unsigned int* __restrict__ row,
+    const unsigned int* __restrict__ partner, double k, double r0, DevBox box)
+{
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)nframes * nsites)
+        return;
+    const size_t f0 = g - g % nsites;  // first site of this frame
+    const unsigned int i = (unsigned int)(g % nsites);
+    const float* pf = fpos + 3 * f0;
+    double fx = fforce[3 * g], fy = fforce[3 * g + 1], fz = fforce[3 * g + 2];
+    for (unsigned int q = row[i]; q < row[i + 1]; q++)
+    {
+        const unsigned int j = partner[q];
+        double dx = (double)pf[3 * (size_t)i] - pf[3 * (size_t)j];
+        double dy = (double)pf[3 * (size_t)i + 1] - pf[3 * (size_t)j + 1];
+        double dz = (double)pf[3 * (size_t)i + 2] - pf[3 * (size_t)j + 2];
+        min_image(box, dx, dy, dz);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double fdr = k * (r0 / r - 1.0);
+        if (!isfinite(fdr))
+            fdr = 0.0;
+        fx -= fdr * dx;
+        fy -= fdr * dy;
+        fz -= fdr * dz;
+    }
+    out[3 * g] = (float)fx;
+    out[3 * g + 1] = (float)fy;
+    out[3 * g + 2] = (float)fz;
+}
+
 unsigned int site_blocks(unsigned int N)
 {
     // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront
@@ -1078,6 +1114,22 @@ int hm_mlp_cg_adam(float* params, float* m1, float* m2, const float* grad_partia
     return check(launch_mlp_adam(params, m1, m2, grad_partial, params_bf16, (float)lr,
                                  (float)beta1, (float)beta2, (float)eps, t, (hipStream_t)stream),
                  "hm_mlp_cg_adam");
+}
+
+int hm_frames_sub_harmonic(unsigned int nframes, unsigned int nsites, const float* frame_pos,
+                           const float* frame_force, float* out_force, const unsigned int* row,
+                           const unsigned int* partner, double k, double r0, hm_box box,
+                           hm_stream_t stream)
+{
+    if ((nframes && nsites) && (!frame_pos || !frame_force || !out_force || !row || !partner))
+        return fail(HM_ERR_ARGS, "hm_frames_sub_harmonic: invalid arguments");
+    const size_t n = (size_t)nframes * nsites;
+    if (n == 0)
+        return HM_OK;
+    hipLaunchKernelGGL(frames_sub_harmonic_kernel, dim3((unsigned int)((n + 255) / 256)),
+                       dim3(256), 0, (hipStream_t)stream, nframes, nsites, frame_pos,
+                       frame_force, out_force, row, partner, k, r0, make_devbox(box));
+    return check(hipGetLastError(), "hm_frames_sub_harmonic");
 }
 
 } // extern "C"

@@ -161,6 +161,7 @@ SIGNATURES = {
     "hm_sys_tune_log": (_i, [_p, _p, _u]),
     "hm_sys_set_chunk": (_i, [_p, _u, _i]),
     "hm_sys_run": (_i, [_p, _u64]),
+    "hm_sys_prime": (_i, [_p, _u64]),
     "hm_sys_timestep": (_u64, [_p]),
     "hm_sys_set_timestep": (_i, [_p, _u64]),
     "hm_sys_get_state": (_i, [_p, _p, _p, _p, _p]),
@@ -183,6 +184,27 @@ SIGNATURES = {
     "hm_sys_mlp_loss_grad": (_i, [_p, _i, _p, _p, C.POINTER(_d), _p]),
     "hm_sys_set_site_map": (_i, [_p, _u, _p, _p]),
     "hm_sys_map_frame": (_i, [_p, _p, _p]),
+    "hm_sys_time_mlp": (_i, [_p, _i, _u, _p]),
+    "hm_frames_sub_harmonic": (_i, [_u, _u, _p, _p, _p, _p, _p, _d, _d, HmBox, _p]),
+    # reference-layout pair driver (include/hoomd_amd_pair.h)
+    "hm_compute_pair_forces_lj": (_i, [_p, _p, _p]),
+    # batched replicas
+    "hm_batch_create": (_p, [_u, _p, _p, _p, _p, _p, _p, _p, _d, _d, _d, _d, _d, _d]),
+    "hm_batch_destroy": (None, [_p]),
+    "hm_batch_thermalize": (_i, [_p, _u64]),
+    "hm_batch_set_timestep": (_i, [_p, _u64]),
+    "hm_batch_run": (_i, [_p, _u64]),
+    "hm_batch_prime": (_i, [_p, _u64]),
+    "hm_batch_set_chunk": (_i, [_p, _u]),
+    "hm_batch_tune": (_i, [_p, _p, _u]),
+    "hm_batch_set_pair_launch": (_i, [_p, _u, _u]),
+    "hm_batch_config": (_i, [_p, _p]),
+    "hm_batch_get_state": (_i, [_p, _p, _p, _p]),
+    "hm_batch_get_forces": (_i, [_p, _p]),
+    "hm_batch_replica_stats": (_i, [_p, _p]),
+    "hm_batch_stats": (_i, [_p, _p]),
+    "hm_batch_time_pair": (_i, [_p, _u, C.POINTER(_d)]),
+    "hm_batch_set_list_sizes": (_i, [_p, _u, _u]),
 }
 
 

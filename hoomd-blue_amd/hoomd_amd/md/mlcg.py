@@ -250,6 +250,21 @@ class FrameSet:
     def forces(self, k):
         return self._get(self.ptr_force(k))
 
+    def subtract_harmonic(self, bonds, k, r0, L):
+        """Replace every frame's forces by the forces minus those of harmonic bonds (k, r0)
+        between the site pairs ``bonds`` [M, 2] (on the device; cubic box edge L)."""
+        b = np.asarray(bonds, np.int64).reshape(-1, 2)
+        ends = np.concatenate([b, b[:, ::-1]])
+        ends = ends[np.lexsort((ends[:, 1], ends[:, 0]))]
+        row = np.searchsorted(ends[:, 0], np.arange(self.nsites + 1)).astype(np.uint32)
+        partner = np.ascontiguousarray(ends[:, 1], np.uint32)
+        d_row = _lib.DeviceArray.from_numpy(row)
+        d_partner = _lib.DeviceArray.from_numpy(partner if len(partner) else np.zeros(1, np.uint32))
+        _lib.check(_lib.lib().hm_frames_sub_harmonic(
+            self.nframes, self.nsites, self.d_pos.ptr, self.d_force.ptr, self.d_force.ptr,
+            d_row.ptr, d_partner.ptr, float(k), float(r0), _lib.box(L), None), "bond forces")
+        _lib.check(_lib.lib().hm_synchronize(), "bond forces")
+
     def set(self, k, positions=None, forces=None):
         for arr, ptr in ((positions, self.ptr_pos(k)), (forces, self.ptr_force(k))):
             if arr is not None:

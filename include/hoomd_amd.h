@@ -468,6 +468,10 @@ int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
 /* Advance nsteps timesteps. Returns HM_OK or a negative error (state is left at the last good
  * chunk boundary on data errors). */
 int hm_sys_run(hm_system* s, uint64_t nsteps);
+/* Capture and instantiate the hipGraphs of every chunk length that hm_sys_run(s, nsteps) from
+ * the current step would launch, launching nothing (prepares the system first if needed).
+ * Returns the number of graphs added (>= 0) or a negative error. */
+int hm_sys_prime(hm_system* s, uint64_t nsteps);
 uint64_t hm_sys_timestep(const hm_system* s);
 /* Set the current timestep (before the first run, or to continue a trajectory). */
 int hm_sys_set_timestep(hm_system* s, uint64_t timestep);
@@ -537,11 +541,79 @@ int hm_sys_set_site_map(hm_system* s, unsigned int nsites, const unsigned int* r
  * d_force[3*nsites] (may be NULL) = summed forces of all force computes (device FP32). */
 int hm_sys_map_frame(hm_system* s, float* d_pos, float* d_force);
 
+/* Benchmark helper (no counterpart in the reference): average durations in ms of the learned
+ * site energy's three kernels (densities, site MLP on bf16 MFMA, forces) of force force_id over
+ * iters launches each, HIP events on the engine stream. */
+int hm_sys_time_mlp(hm_system* s, int force_id, unsigned int iters, double* avg_ms3);
+/* Force-matching targets (no counterpart in the reference; device arrays): out_force = frame
+ * forces minus the harmonic bond forces k (r - r0)^2 / 2 of each site's bonded partners
+ * (partner[row[i] .. row[i+1]-1] for site i), frames [nframes][nsites][3] FP32. out_force may
+ * be frame_force. */
+int hm_frames_sub_harmonic(unsigned int nframes, unsigned int nsites, const float* frame_pos,
+                           const float* frame_force, float* out_force, const unsigned int* row,
+                           const unsigned int* partner, double k, double r0, hm_box box,
+                           hm_stream_t stream);
+
 /* Benchmark helper (no counterpart in the reference): average duration in ms of the pair-force
  * kernel of force force_id over iters back-to-back launches on the engine stream, timed with
  * HIP events on that stream, and the number of neighbour-list entries it reads. */
 int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg_ms,
                      uint64_t* nlist_entries);
+
+
+/* ---------------------------------------------------------------------------------------------
+ * Batched replicas: many independent LJ systems of uneven size in one set of device arrays, each
+ * phase of the timestep one launch over all of them (no counterpart in the reference, which runs
+ * one System per Simulation; each replica follows ConstantVolume + Bussi, the binned full list
+ * with distance-check rebuilds and PotentialPair<EvaluatorPairLJ> with mode "none", as the
+ * single-system path does: hoomd/md/TwoStepConstantVolume.cc:7-260, Thermostat.h:357-446,
+ * NeighborList.cc:1220-1289, NeighborListGPUBinned.cu:52-298, PotentialPair.h:587-794).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct hm_batch hm_batch;
+/* nrep replicas; replica r holds n[r] particles in a cubic periodic box of edge L[r] and is
+ * thermostatted at kT[r] with RNG seed seed[r] (16 bits, as Simulation.seed). pos3/vel3/mass
+ * hold all replicas' particles concatenated in replica order (vel3, mass may be NULL: zero, 1).
+ * LJ epsilon/sigma/r_cut and buffer r_buff are shared; dt and Bussi tau (0: tau = 0) too.
+ * Returns NULL on error (hm_last_error; HM_ERR_BOXSMALL when L[r] <= 2 (r_cut + r_buff)). */
+hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double* L,
+                          const double* kT, const uint16_t* seed, const double* pos3,
+                          const double* vel3, const double* mass, double epsilon, double sigma,
+                          double r_cut, double r_buff, double dt, double tau);
+void hm_batch_destroy(hm_batch* b);
+/* Maxwell-Boltzmann velocities per replica (ParticleGroup::thermalizeParticleMomenta with the
+ * replica's tags 0..n-1, seed and kT; COM momentum of each replica removed). */
+int hm_batch_thermalize(hm_batch* b, uint64_t timestep);
+int hm_batch_set_timestep(hm_batch* b, uint64_t timestep);
+/* Advance every replica nsteps steps (chunks captured as hipGraphs). */
+int hm_batch_run(hm_batch* b, uint64_t nsteps);
+/* Capture the graphs hm_batch_run(b, nsteps) would launch, launching nothing. */
+int hm_batch_prime(hm_batch* b, uint64_t nsteps);
+/* Steps per captured chunk (default 100). */
+int hm_batch_set_chunk(hm_batch* b, unsigned int steps);
+/* Autotune the pair launch (tpp x block, median of 5) on the current lists; cached per
+ * (particles, density) bucket for the process. log3 receives rows (tpp, block, ms); returns
+ * the number of rows of the sweep (0 when the cache answered). */
+int hm_batch_tune(hm_batch* b, float* log3, unsigned int max_rows);
+/* Fixed pair launch: tpp in {1,2,4,8,16}, block in {64,128,256}. */
+int hm_batch_set_pair_launch(hm_batch* b, unsigned int tpp, unsigned int block);
+/* out = {pair tpp, pair block, autotuned, cmax, nmax, total cells}. */
+int hm_batch_config(const hm_batch* b, unsigned int out[6]);
+/* State of all particles in replica order (any pointer may be NULL). */
+int hm_batch_get_state(hm_batch* b, double* pos3, double* vel3, int* image3);
+/* Pair force (fx, fy, fz, energy) of every particle [4N] for the current state. */
+int hm_batch_get_forces(hm_batch* b, double* force4);
+/* Per replica r: out4[4r] kinetic energy, [4r+1] pair energy, [4r+2] list builds, [4r+3]
+ * neighbour-list entries. */
+int hm_batch_replica_stats(hm_batch* b, double* out4);
+/* out4 = {particles, steps run, overflow replays, list builds over all replicas}. */
+int hm_batch_stats(hm_batch* b, uint64_t out4[4]);
+/* Replace the cell and row capacities and rebuild every list at the next step (capacities below
+ * the need exercise the overflow replay: the chunk restarts from its first step with grown
+ * lists, as the reference regrows and repeats, NeighborList.cc:411-427). */
+int hm_batch_set_list_sizes(hm_batch* b, unsigned int cmax, unsigned int nmax);
+/* Benchmark helper: average pair-kernel launch duration (ms) over iters launches, HIP events on
+ * the batch stream. */
+int hm_batch_time_pair(hm_batch* b, unsigned int iters, double* avg_ms);
 
 #ifdef __cplusplus
 }

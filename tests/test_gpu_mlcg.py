@@ -177,9 +177,14 @@ def test_kg_to_cg_network_gains(hoomd):
     assert 0.5 < r0 < 4.0 and kb > 0
     target = mlcg.FrameSet(nfr, smap.nsites)
     for k in range(nfr):
+        target.set(k, positions=frames.positions(k), forces=frames.forces(k))
+    # targets: mapped force minus the CG bond force, on the device; checked against the host
+    # restatement (FP64 from the same FP32 frames) on the first and last frames
+    target.subtract_harmonic(cgb, kb, r0, L)
+    for k in (0, nfr - 1):
         p = frames.positions(k).astype(np.float64)
         T = frames.forces(k).astype(np.float64) - mlcg.harmonic_bond_forces(p, L, cgb, kb, r0)
-        target.set(k, positions=p, forces=T)
+        assert rel_rms(target.forces(k), T) < 1e-6
     # the CG system: sites, CG bonds, the network (bonded pairs excluded)
     snap = hoomd.Snapshot()
     snap.configuration.box = [L, L, L, 0, 0, 0]
