@@ -101,6 +101,35 @@ def pair_roofline(sim, lj, N, kind, period=500, samples=10, launches=20):
     return ms_avg, ent_avg, B, achieved
 
 
+def nlist_pair_fraction(sim, N, entries, pair_ms, steps, builds):
+    """The nlist + pair roofline fraction with the list cost amortised over k = steps per build
+    measured in the timed steps:
+        (B_pair + (B_nl + B_cl) / k) / (t_pair + (t_nl + t_cl) / k) / peak
+    with the algorithmic bytes of FP64 position records (P = 32 bytes):
+        B_pair = 4 sum n + N (4 + P + 16)          (row entries, n_neigh, own position, force)
+        B_nl   = N (P + 16 + 4 + 4 + P) + 4 sum n  (own position, cell record, cell size,
+                                                    n_neigh, last position; row entries)
+        B_cl   = N (P + 16 + 4) + 4 cells          (position, cell record, slot; cell sizes)
+    t_cl and t_nl are complete rebuilds at the current state (HIP events, 5 rebuilds)."""
+    import ctypes as C
+
+    from hoomd_amd import _lib
+    t = (C.c_double * 2)()
+    _lib.check(_lib.lib().hm_sys_time_build(sim._system, 5, t), "time build")
+    P = 32
+    k = steps / max(builds, 1)
+    b_pair = 4.0 * entries + N * (4 + P + 16)
+    b_nl = N * (P + 16 + 4 + 4 + P) + 4.0 * entries
+    cells = int(np.prod(np.floor(np.asarray(sim.state.box[:3]) / (2.5 + 0.4))))
+    b_cl = N * (P + 16 + 4) + 4.0 * cells
+    t_s = pair_ms * 1e-3 + (t[0] + t[1]) * 1e-3 / k
+    achieved = (b_pair + (b_nl + b_cl) / k) / t_s / 1e9
+    return {"k": k, "t_pair_us": pair_ms * 1e3, "t_cell_us": t[0] * 1e3, "t_nlist_us": t[1] * 1e3,
+            "bytes_per_step": b_pair + (b_nl + b_cl) / k, "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "target_frac": 0.60}
+
+
 def load_traffic(N):
     """HBM bytes per pair launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
     by a copy-kernel calibration), committed under profiles/ for this workload."""
@@ -111,23 +140,50 @@ def load_traffic(N):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def host_cpu():
+    """(CPU model, threads used for the CPU baseline): the host's cores, at most 16 (the share
+    of one GPU on these machines)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, max(1, min(16, n))
+
+
 def cpu_baseline(n_cells, rho, seed, steps):
-    """Oracle (FP64 C restatement of the reference CPU path: half list, 3rd law, Bussi) on one
-    host core, on the same 1M-particle state for a bounded number of steps."""
+    """Oracle (FP64 C restatement of the reference CPU path: Bussi, binned lists) on the host's
+    cores (full lists on P threads; the serial form keeps the half list with the third law), on
+    the same 1M-particle state for a bounded number of steps."""
     import oracle as O
     from lattices import fcc_lattice
     pos, L = fcc_lattice(n_cells, rho)
     N = len(pos)
+    O.set_threads(host_cpu()[1])
     vel = O.thermalize(np.ones(N), np.arange(N), 1.2, 0, seed)
     s = O.System(pos, vel, L, r_cut=2.5, mode="none", buffer=0.4, dt=0.005, seed=seed)
     s.set_bussi(1.2, 0.0)
     s.set_sorter(500, 256)
     s.run(1)  # initial build, first step and sort outside the timed sample
+    # a short probe sizes the sample to about 15 s of host work (at least `steps` steps)
+    t0 = time.perf_counter()
+    s.run(3)
+    per = (time.perf_counter() - t0) / 3
+    steps = max(steps, min(2000, int(15.0 / max(per, 1e-6))))
     b0 = s.n_builds
     t0 = time.perf_counter()
     s.run(steps)
     dt = time.perf_counter() - t0
-    return steps / dt, dt, int(s.n_builds - b0)
+    O.set_threads(1)
+    return steps / dt, dt, int(s.n_builds - b0), steps
 
 
 # sphinx-doc/howto/determine-the-most-efficient-device.rst: one A100; one EPYC 7742 MPI rank
@@ -438,6 +494,9 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     rdf_edges_max, nb = 4.0, 80
     g_map = np.mean([MC.rdf(fs.positions(k), L, rdf_edges_max, nb, exclude_bonded=cgb)[1]
                      for k in sample], axis=0)
+    r_bins = (np.arange(nb) + 0.5) * rdf_edges_max / nb
+    prior_params, rc_prior = MC.rdf_prior_table(r_bins, g_map)
+    log(f"[mlcg] RDF prior: cut-off {rc_prior:.3f}")
     fs.subtract_harmonic(cgb, kb, r0, L)
     # CG system
     snap = hoomd.Snapshot()
@@ -455,10 +514,14 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=seed)
     harm = hoomd.md.bond.Harmonic()
     harm.params["cg"] = dict(k=kb, r0=r0)
+    prior = hoomd.md.pair.Table(nlist=nl, default_r_cut=rc_prior)
+    prior.params[("S", "S")] = prior_params
     lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
     lang.gamma["S"] = float(per)
-    cg.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp, harm], methods=[lang])
+    cg.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp, harm, prior],
+                                                   methods=[lang])
     cg.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
+    fs.subtract_pair(cg, prior)
     G = np.concatenate([mlp.descriptors(fs, int(k)) for k in sample[:5]])
     mlp.fit_normalization(G)
     ntrain = max(1, int(nframes * 0.9))
@@ -486,23 +549,33 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     _lib.check(_lib.lib().hm_synchronize())
     tps_cg = steps / (time.perf_counter() - t0)
     log(f"[mlcg] CG run: {tps_cg:.1f} steps/s at {smap.nsites} sites")
-    # structure of the CG run
-    g_cg, p_cg, r_cg = [], [], []
-    for k in range(20):
-        cg.run(250)
-        pc = cg.state.get_snapshot().particles.position
-        g_cg.append(MC.rdf(pc, L, rdf_edges_max, nb, exclude_bonded=cgb)[1])
-        r_cg.append(MC.bond_lengths(pc, L, cgb))
-        p_cg.append(thermo.pressure)
-    g_cg = np.mean(g_cg, axis=0)
-    r_cg = np.concatenate(r_cg)
+    # structure of the CG run, with the network and without it (bonds + prior)
+    def structure(nsamp=20, every=250):
+        gs, ps, rs = [], [], []
+        for k in range(nsamp):
+            cg.run(every)
+            pc = cg.state.get_snapshot().particles.position
+            gs.append(MC.rdf(pc, L, rdf_edges_max, nb, exclude_bonded=cgb)[1])
+            rs.append(MC.bond_lengths(pc, L, cgb))
+            ps.append(thermo.pressure)
+        return np.mean(gs, axis=0), np.mean(ps), np.concatenate(rs)
+
+    g_cg, p_cg_mean, r_cg = structure()
+    trained_params = mlp.params
+    mlp.params = np.zeros(MC.NPARAM, np.float32)
+    cg.run(2000)
+    g_none, p_none, r_none = structure()
+    mlp.params = trained_params
     dr = rdf_edges_max / nb
     rdf_l1 = float(np.abs(g_cg - g_map).sum() / g_map.sum())
     rdf_l2 = float(np.sqrt(((g_cg - g_map) ** 2).sum() / (g_map ** 2).sum()))
+    rdf_l1_none = float(np.abs(g_none - g_map).sum() / g_map.sum())
     hb_e = np.linspace(min(r_map.min(), r_cg.min()), max(r_map.max(), r_cg.max()), 61)
     hm_, _ = np.histogram(r_map, hb_e)
     hc_, _ = np.histogram(r_cg, hb_e)
     bond_tv = float(0.5 * np.abs(hm_ / hm_.sum() - hc_ / hc_.sum()).sum())
+    hn_, _ = np.histogram(r_none, hb_e)
+    bond_tv_none = float(0.5 * np.abs(hm_ / hm_.sum() - hn_ / hn_.sum()).sum())
     ms3 = (Cc.c_double * 3)()
     _lib.check(_lib.lib().hm_sys_time_mlp(cg._system, mlp._force_id, 20, ms3), "time mlp")
     site_flops = MLP_SITE_FLOP * smap.nsites
@@ -516,7 +589,7 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
         "plain": {"timesteps_per_s": tps_fg, "particles": N, "dt": 0.01,
                   "tau_per_s": tps_fg * 0.01, "pressure": float(np.mean(p_fg))},
         "cg": {"timesteps_per_s": tps_cg, "sites": smap.nsites, "dt": 0.02,
-               "tau_per_s": tps_cg * 0.02, "pressure": float(np.mean(p_cg)),
+               "tau_per_s": tps_cg * 0.02, "pressure": float(p_cg_mean),
                "kernel_ms": {"densities": ms3[0], "site_mlp": ms3[1], "forces": ms3[2]},
                "mfma": {"site_flop": MLP_SITE_FLOP, "achieved_tflops": site_flops /
                         (ms3[1] * 1e-3) / 1e12, "peak_tflops_bf16": 2500.0,
@@ -527,10 +600,13 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
                      "heldout_residual_bonds_only": base, "heldout_residual_trained": trained},
         "cg_bonds": {"k": kb, "r0": r0},
         "errors": {"rdf_rel_l1": rdf_l1, "rdf_rel_l2": rdf_l2, "bond_length_tv": bond_tv,
-                   "pressure_rel": float(abs(np.mean(p_cg) - np.mean(p_fg)) /
-                                         abs(np.mean(p_fg)))},
+                   "pressure_rel": float(abs(p_cg_mean - np.mean(p_fg)) / abs(np.mean(p_fg)))},
+        "without_network": {"rdf_rel_l1": rdf_l1_none, "bond_length_tv": bond_tv_none,
+                            "pressure": float(p_none)},
+        "prior": {"r_cut": rc_prior},
         "rdf": {"r": [float((i + 0.5) * dr) for i in range(nb)],
-                "mapped": g_map.tolist(), "cg": g_cg.tolist()},
+                "mapped": g_map.tolist(), "cg": g_cg.tolist(), "cg_without_network":
+                g_none.tolist()},
         "dtype": "bf16 MFMA (fp32 accumulate) MLP, f32 densities/forces, f64 integration",
         "data": "synthetic (seeded random-walk melt, pushed off)",
         "config": {"workload": f"KG melt {n_chains} x {chain_len} -> 5:1 CG "
@@ -667,15 +743,27 @@ def main():
     ms, entries, B, achieved = pair_roofline(sim, lj, N, a.pair_kind)
     log(f"[bench] pair kernel {ms * 1e3:.1f} us, {B / 1e9:.3f} GB algorithmic, {achieved:.0f} GB/s")
     traffic = load_traffic(N)
+    nlp = nlist_pair_fraction(sim, N, entries, ms, a.steps, builds)
+    log(f"[bench] nlist+pair: cell {nlp['t_cell_us']:.1f} us + list {nlp['t_nlist_us']:.1f} us "
+        f"per build, k = {nlp['k']:.2f} steps per build -> {nlp['achieved']:.0f} GB/s "
+        f"({nlp['frac']:.3f} of peak)")
+    nl_out = np.zeros(3, np.int32)
+    nl_log = np.zeros((32, 3), np.float32)
+    rows = _lib.lib().hm_sys_nlist_launch(sim._system, nl_out.ctypes.data, nl_log.ctypes.data, 32)
+    for r in nl_log[:max(rows, 0)]:
+        log(f"[bench] list-build tuner shape={int(r[0])} grid={int(r[1])} {r[2] * 1e3:.1f} us")
+    launch["nlist_shape"], launch["nlist_grid"] = int(nl_out[0]), int(nl_out[1])
 
     cpu = None
     if not a.no_cpu:
-        log(f"[bench] CPU baseline: oracle, {a.cpu_steps} steps of the same workload, 1 core")
-        cps, cdt, cbuilds = cpu_baseline(n_cells, rho, a.seed, a.cpu_steps)
-        cpu = {"value": cps, "unit": "timesteps/s", "cores": 1, "kind": "port",
-               "sample": f"N={N} LJ Bussi workload, {a.cpu_steps} steps after the first "
-                         f"(includes {cbuilds} list rebuilds), {cdt:.1f} s on one host core; "
-                         f"{cps * N:.3e} particle-steps/s",
+        model, P = host_cpu()
+        log(f"[bench] CPU baseline: oracle, {a.cpu_steps} steps of the same workload, {P} threads")
+        cps, cdt, cbuilds, csteps = cpu_baseline(n_cells, rho, a.seed, a.cpu_steps)
+        cpu = {"value": cps, "unit": "timesteps/s", "cores": P, "kind": "port",
+               "cpu_model": model,
+               "sample": f"N={N} LJ Bussi workload, {csteps} steps after the first four "
+                         f"(includes {cbuilds} list rebuilds), {cdt:.1f} s on {P} host threads "
+                         f"({model}); {cps * N:.3e} particle-steps/s",
                "particle_steps_per_s": cps * N}
         log(f"[bench] CPU {cps:.3f} steps/s")
 
@@ -703,6 +791,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": ms * 1e3, "algorithmic_bytes": B,
                      "nlist_entries": int(entries)},
+        "roofline_nlist_pair": nlp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

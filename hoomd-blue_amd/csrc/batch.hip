@@ -10,6 +10,7 @@
 //                           wrap, distance check -> the replica's rebuild flag
 //   cell clear (cells)      only cells of replicas that rebuild
 //   cell bin   (tiles)      only particles of replicas that rebuild
+//   cell rank  (slots)      each rebuilt cell's slots ordered by particle index
 //   list build (tiles)      full neighbour rows, only replicas that rebuild
 //   pair       (particles)  LJ force/energy, FP32 arithmetic on FP64 minimum-image separations
 //   step two   (tiles)      second half kick, kinetic-energy partial per tile
@@ -252,11 +253,14 @@ __device__ __forceinline__ unsigned int cell_coord(double x, const Rep& R)
     return (unsigned int)c;
 }
 
+// Also zeroes the particle's neighbour count: the list build writes the counts of the particles
+// it finds in the cells, so a particle left out of an overflowed cell keeps an empty row (the
+// overflow then replays the chunk) instead of a count from an earlier build.
 __global__ __launch_bounds__(BTILE) void bcell_bin_kernel(
     const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
     const RepState* __restrict__ rst, const double4* __restrict__ pos, d3* __restrict__ last,
-    unsigned int cmax, unsigned int* __restrict__ cell_size, unsigned int* __restrict__ cell_idx,
-    BCond* __restrict__ cond)
+    unsigned int cmax, unsigned int* __restrict__ cell_size, unsigned int* __restrict__ cell_raw,
+    unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond)
 {
     const unsigned int t = blockIdx.x;
     const unsigned int r = tile_rep[t];
@@ -270,81 +274,163 @@ __global__ __launch_bounds__(BTILE) void bcell_bin_kernel(
     const unsigned int p = first + threadIdx.x;
     const double4 x = pos[p];
     last[p] = d3{x.x, x.y, x.z};
+    n_neigh[p] = 0;
     const unsigned int c = R.cell_off
                            + (cell_coord(x.z, R) * R.dim + cell_coord(x.y, R)) * R.dim
                            + cell_coord(x.x, R);
     const unsigned int slot = atomicAdd(&cell_size[c], 1u);
     if (slot < cmax)
-        cell_idx[(size_t)c * cmax + slot] = p;
+        cell_raw[(size_t)c * cmax + slot] = p;
     else
         atomicMax(&cond->cell_need, slot + 1);
 }
 
+// Order every rebuilt cell's slots by particle index (slots were claimed by atomics in arrival
+// order): slot k's particle goes to its rank among the cell's particles. Rows, and so the FP32
+// force sums, then do not depend on scheduling: a run is reproducible and a replica follows the
+// same trajectory alone or in any batch.
+__global__ __launch_bounds__(256) void bcell_rank_kernel(const unsigned int* __restrict__ cell_rep,
+                                                         const RepState* __restrict__ rst,
+                                                         unsigned int ncell, unsigned int cmax,
+                                                         const unsigned int* __restrict__ cell_size,
+                                                         const unsigned int* __restrict__ cell_raw,
+                                                         unsigned int* __restrict__ cell_idx)
+{
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)ncell * cmax)
+        return;
+    const unsigned int c = (unsigned int)(g / cmax), k = (unsigned int)(g % cmax);
+    if (!rst[cell_rep[c]].rebuild)
+        return;
+    const unsigned int n = min(cell_size[c], cmax);
+    if (k >= n)
+        return;
+    const unsigned int* raw = cell_raw + (size_t)c * cmax;
+    const unsigned int my = raw[k];
+    unsigned int rank = 0;
+    for (unsigned int m = 0; m < n; m++)
+        rank += raw[m] < my ? 1u : 0u;
+    cell_idx[(size_t)c * cmax + rank] = my;
+}
+
 // -------------------------------------------------------------------------- list build
-// One lane per particle over the unique adjacent cells: offsets {-1, 0, 1} per axis when the
-// axis has at least 3 cells, {0, 1} for 2 cells, {0} for 1 (CellList::initializeCellAdj keeps
-// each adjacent cell once). Candidate j != i is kept when its FP64 minimum-image r^2 <= r_list^2.
-__global__ __launch_bounds__(BTILE) void bnlist_kernel(
-    const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
+// One wavefront per home cell of a rebuilding replica. The candidates (the particles of the
+// unique adjacent cells, in cell order: offsets {-1, 0, 1} per axis where the axis has at least
+// 3 cells, {0, 1} for 2, {0} for 1, as CellList::initializeCellAdj keeps each adjacent cell
+// once) are staged in LDS, NL_CAP at a time; then for each home particle the 64 lanes test 64
+// candidates at once (FP64 minimum image, kept when j != i and r^2 <= r_list^2) and a ballot
+// places the hits contiguously in the particle's row, in candidate order
+// (NeighborListGPUBinned.cu:52-298 with the lanes over candidates instead of over particles).
+constexpr unsigned int NL_CAP = 512;
+constexpr unsigned int NL_HOME_MAX = 256;
+
+__global__ __launch_bounds__(64) void bnlist_kernel(
+    const Rep* __restrict__ reps, const unsigned int* __restrict__ cell_rep,
     const RepState* __restrict__ rst, const double4* __restrict__ pos, unsigned int cmax,
     const unsigned int* __restrict__ cell_size, const unsigned int* __restrict__ cell_idx,
     double rlistsq, unsigned int nmax, unsigned int* __restrict__ nlist,
     unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond)
 {
 #pragma clang fp contract(off)
-    const unsigned int t = blockIdx.x;
-    const unsigned int r = tile_rep[t];
+    __shared__ double sx[NL_CAP], sy[NL_CAP], sz[NL_CAP];
+    __shared__ unsigned int sj[NL_CAP];
+    __shared__ unsigned int acell[27], aoff[28];
+    __shared__ unsigned int hcnt[NL_HOME_MAX];
+    const unsigned int c = blockIdx.x;
+    const unsigned int r = cell_rep[c];
     if (!rst[r].rebuild)
         return;
     const Rep R = reps[r];
-    unsigned int first, count;
-    tile_range(R, t, first, count);
-    if (threadIdx.x >= count)
-        return;
-    const unsigned int p = first + threadIdx.x;
-    const double4 xi = pos[p];
-    const int cx = (int)cell_coord(xi.x, R), cy = (int)cell_coord(xi.y, R),
-              cz = (int)cell_coord(xi.z, R);
+    const unsigned int lane = threadIdx.x;
+    const unsigned int lc = c - R.cell_off;
     const int d = (int)R.dim;
+    const int cx = (int)(lc % R.dim), cy = (int)((lc / R.dim) % R.dim), cz = (int)(lc / (R.dim * R.dim));
     const int lo = d >= 3 ? -1 : 0;
-    const int hi = d >= 2 ? 1 : 0;
-    unsigned int k = 0;
-    unsigned int* row = nlist + (size_t)p * nmax;
-    for (int oz = lo; oz <= hi; oz++)
+    const int span = d >= 3 ? 3 : (d == 2 ? 2 : 1);
+    const unsigned int nadj = (unsigned int)(span * span * span);
+    // adjacent cells in (z, y, x) offset order and their sizes, prefix sums by a wave scan
+    unsigned int asz = 0;
+    if (lane < nadj)
     {
-        const int z = (cz + oz + d) % d;
-        for (int oy = lo; oy <= hi; oy++)
+        const int ox = lo + (int)(lane % span), oy = lo + (int)((lane / span) % span),
+                  oz = lo + (int)(lane / (span * span));
+        const int x = (cx + ox + d) % d, y = (cy + oy + d) % d, z = (cz + oz + d) % d;
+        const unsigned int ac = R.cell_off + ((unsigned int)z * R.dim + y) * R.dim + x;
+        acell[lane] = ac;
+        asz = min(cell_size[ac], cmax);
+    }
+    unsigned int incl = asz;
+    for (unsigned int o = 1; o < 32; o <<= 1)
+    {
+        const unsigned int t = __shfl_up(incl, o, 64);
+        if (lane >= o)
+            incl += t;
+    }
+    if (lane < nadj)
+        aoff[lane + 1] = incl;
+    if (lane == 0)
+        aoff[0] = 0;
+    const unsigned int total = __shfl(incl, (int)nadj - 1, 64);
+    const unsigned int nhome = min(min(cell_size[c], cmax), NL_HOME_MAX);
+    const unsigned int* home = cell_idx + (size_t)c * cmax;
+    for (unsigned int h = lane; h < nhome; h += 64)
+        hcnt[h] = 0;
+    __syncthreads();
+    for (unsigned int base = 0; base < total; base += NL_CAP)
+    {
+        const unsigned int npass = min(NL_CAP, total - base);
+        for (unsigned int k = lane; k < npass; k += 64)
         {
-            const int y = (cy + oy + d) % d;
-            for (int ox = lo; ox <= hi; ox++)
+            const unsigned int g = base + k;
+            unsigned int seg = 0;
+            while (aoff[seg + 1] <= g)
+                seg++;
+            const unsigned int j = cell_idx[(size_t)acell[seg] * cmax + (g - aoff[seg])];
+            const double4 xj = pos[j];
+            sj[k] = j;
+            sx[k] = xj.x;
+            sy[k] = xj.y;
+            sz[k] = xj.z;
+        }
+        __syncthreads();
+        for (unsigned int h = 0; h < nhome; h++)
+        {
+            const unsigned int i = home[h];
+            const double4 xi = pos[i];
+            unsigned int cnt = hcnt[h];
+            unsigned int* row = nlist + (size_t)i * nmax;
+            for (unsigned int k0 = 0; k0 < npass; k0 += 64)
             {
-                const int x = (cx + ox + d) % d;
-                const unsigned int c = R.cell_off + ((unsigned int)z * R.dim + y) * R.dim + x;
-                const unsigned int sz = min(cell_size[c], cmax);
-                const unsigned int* cl = cell_idx + (size_t)c * cmax;
-                for (unsigned int q = 0; q < sz; q++)
+                const unsigned int k = k0 + lane;
+                bool hit = false;
+                unsigned int j = 0;
+                if (k < npass)
                 {
-                    const unsigned int j = cl[q];
-                    if (j == p)
-                        continue;
-                    const double4 xj = pos[j];
-                    double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    j = sj[k];
+                    double dx = xi.x - sx[k], dy = xi.y - sy[k], dz = xi.z - sz[k];
                     dz -= R.L * rint(dz * R.Linv);
                     dy -= R.L * rint(dy * R.Linv);
                     dx -= R.L * rint(dx * R.Linv);
-                    if (dx * dx + dy * dy + dz * dz <= rlistsq)
-                    {
-                        if (k < nmax)
-                            row[k] = j;
-                        k++;
-                    }
+                    hit = j != i && dx * dx + dy * dy + dz * dz <= rlistsq;
                 }
+                const uint64_t m = wave_ballot(hit);
+                const unsigned int at = cnt + lanes_below(m);
+                if (hit && at < nmax)
+                    row[at] = j;
+                cnt += (unsigned int)__popcll(m);
             }
+            if (lane == 0)
+                hcnt[h] = cnt;
         }
+        __syncthreads();
     }
-    n_neigh[p] = min(k, nmax);
-    if (k > nmax)
-        atomicMax(&cond->nlist_need, k);
+    for (unsigned int h = lane; h < nhome; h += 64)
+    {
+        const unsigned int cnt = hcnt[h];
+        n_neigh[home[h]] = min(cnt, nmax);
+        if (cnt > nmax)
+            atomicMax(&cond->nlist_need, cnt);
+    }
 }
 
 // -------------------------------------------------------------------------- pair forces
@@ -365,26 +451,43 @@ __global__ __launch_bounds__(256) void bpair_kernel(
     {
         const Rep R = reps[part_rep[p]];
         const double4 xi = pos[p];
-        const unsigned int nn = n_neigh[p];
+        const unsigned int nn = min(n_neigh[p], nmax);
         const unsigned int* row = nlist + (size_t)p * nmax;
-        for (unsigned int k = sub; k < nn; k += TPP)
+        // UNR neighbours per lane per pass: every index, then every position load is issued
+        // before the first is used, so several gathers are in flight per lane
+        constexpr unsigned int UNR = 4;
+        for (unsigned int k0 = sub; k0 < nn; k0 += TPP * UNR)
         {
-            const unsigned int j = row[k];
-            const double4 xj = pos[j];
-            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            dz -= R.L * rint(dz * R.Linv);
-            dy -= R.L * rint(dy * R.Linv);
-            dx -= R.L * rint(dx * R.Linv);
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq < pc.rcutsq)
+            unsigned int j[UNR];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
             {
-                const float r2inv = recip((float)rsq);
-                const float r6inv = r2inv * r2inv * r2inv;
-                const float fdr = r2inv * r6inv * (12.f * pc.lj1 * r6inv - 6.f * pc.lj2);
-                e += r6inv * (pc.lj1 * r6inv - pc.lj2);
-                fx += (float)dx * fdr;
-                fy += (float)dy * fdr;
-                fz += (float)dz * fdr;
+                const unsigned int k = k0 + u * TPP;
+                j[u] = k < nn ? row[k] : p;
+            }
+            double4 xj[UNR];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
+                xj[u] = pos[j[u]];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
+            {
+                double dx = xi.x - xj[u].x, dy = xi.y - xj[u].y, dz = xi.z - xj[u].z;
+                dz -= R.L * rint(dz * R.Linv);
+                dy -= R.L * rint(dy * R.Linv);
+                dx -= R.L * rint(dx * R.Linv);
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                // the particle itself stands in for missing entries: rsq = 0 is excluded
+                if (rsq < pc.rcutsq && j[u] != p)
+                {
+                    const float r2inv = recip((float)rsq);
+                    const float r6inv = r2inv * r2inv * r2inv;
+                    const float fdr = r2inv * r6inv * (12.f * pc.lj1 * r6inv - 6.f * pc.lj2);
+                    e += r6inv * (pc.lj1 * r6inv - pc.lj2);
+                    fx += (float)dx * fdr;
+                    fy += (float)dy * fdr;
+                    fz += (float)dz * fdr;
+                }
             }
         }
     }
@@ -442,6 +545,53 @@ __global__ __launch_bounds__(BTILE) void bstep_two_kernel(
     }
 }
 
+// -------------------------------------------------------------------------- cell order
+// The particles of each replica re-laid in cell order (cells in index order, each cell's
+// particles by index), so that the particles a wavefront gathers in the list build and the pair
+// kernel share cache lines; the reference's SFCPackTuner does this with a Hilbert order
+// (hoomd/SFCPackTuner.cc:119-285) for the same reason. One wavefront per replica turns its cell
+// sizes into first slots (exclusive prefix sums, offset by the replica's first particle).
+__global__ __launch_bounds__(64) void bcell_start_kernel(const Rep* __restrict__ reps,
+                                                         const unsigned int* __restrict__ cell_size,
+                                                         unsigned int cmax,
+                                                         unsigned int* __restrict__ cell_start)
+{
+    const Rep R = reps[blockIdx.x];
+    const unsigned int ncell = R.dim * R.dim * R.dim;
+    const unsigned int lane = threadIdx.x;
+    unsigned int base = R.off;
+    for (unsigned int c0 = 0; c0 < ncell; c0 += 64)
+    {
+        const unsigned int c = c0 + lane;
+        const unsigned int n = c < ncell ? min(cell_size[R.cell_off + c], cmax) : 0u;
+        unsigned int incl = n;
+        for (unsigned int o = 1; o < 64; o <<= 1)
+        {
+            const unsigned int t = __shfl_up(incl, o, 64);
+            if (lane >= o)
+                incl += t;
+        }
+        if (c < ncell)
+            cell_start[R.cell_off + c] = base + incl - n;
+        base += __shfl(incl, 63, 64);
+    }
+}
+
+// order[new index] = old index
+__global__ __launch_bounds__(256) void bcell_order_kernel(unsigned int ncell, unsigned int cmax,
+                                                          const unsigned int* __restrict__ cell_size,
+                                                          const unsigned int* __restrict__ cell_start,
+                                                          const unsigned int* __restrict__ cell_idx,
+                                                          unsigned int* __restrict__ order)
+{
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)ncell * cmax)
+        return;
+    const unsigned int c = (unsigned int)(g / cmax), k = (unsigned int)(g % cmax);
+    if (k < min(cell_size[c], cmax))
+        order[cell_start[c] + k] = cell_idx[g];
+}
+
 __global__ void bset_rebuild_kernel(RepState* rst, unsigned int nrep)
 {
     const unsigned int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,14 +632,24 @@ struct hm_batch
     float4* force = nullptr;
     double* ke_partial = nullptr;
     unsigned int* cell_size = nullptr;
+    unsigned int* cell_raw = nullptr;
     unsigned int* cell_idx = nullptr;
     unsigned int* nlist = nullptr;
     unsigned int* n_neigh = nullptr;
+    // cell-order sort: replica-local tag of each particle (its index at creation, minus the
+    // replica's first index), the new order and a staging buffer for the permutation
+    unsigned int* tag = nullptr;
+    unsigned int* cell_start = nullptr;
+    unsigned int* order = nullptr;
+    void* stage = nullptr;
+    unsigned int sort_period = 100;
+    uint64_t last_sort = ~0ull, n_sorts = 0;
     // chunk-start snapshot for overflow replay
     double4 *s_pos = nullptr, *s_vel = nullptr;
     d3 *s_accel = nullptr, *s_last = nullptr;
     i3* s_image = nullptr;
     double* s_ke = nullptr;
+    unsigned int* s_tag = nullptr;
     RepState* s_rst = nullptr;
     BatchStep* s_bst = nullptr;
 
@@ -503,9 +663,9 @@ struct hm_batch
                          (void**)&tile_rep, (void**)&cell_rep, (void**)&part_rep, (void**)&pos,
                          (void**)&vel, (void**)&accel, (void**)&image, (void**)&last,
                          (void**)&force, (void**)&ke_partial, (void**)&cell_size,
-                         (void**)&cell_idx, (void**)&nlist, (void**)&n_neigh, (void**)&s_pos,
+                         (void**)&cell_raw, (void**)&cell_idx, (void**)&nlist, (void**)&n_neigh, (void**)&tag, (void**)&cell_start, (void**)&order, (void**)&stage, (void**)&s_pos,
                          (void**)&s_vel, (void**)&s_accel, (void**)&s_last, (void**)&s_image,
-                         (void**)&s_ke, (void**)&s_rst, (void**)&s_bst})
+                         (void**)&s_ke, (void**)&s_tag, (void**)&s_rst, (void**)&s_bst})
             if (*p)
             {
                 (void)hipFree(*p);
@@ -526,10 +686,19 @@ struct hm_batch
 
     void alloc_lists()
     {
+        if (cmax > NL_HOME_MAX)
+            throw BError{HM_ERR_LAUNCH, "batch: more than 256 particles in one cell"};
+        bfree(cell_raw);
         bfree(cell_idx);
         bfree(nlist);
+        cell_raw = balloc<unsigned int>((size_t)ncell * cmax);
         cell_idx = balloc<unsigned int>((size_t)ncell * cmax);
         nlist = balloc<unsigned int>((size_t)N * nmax);
+        // zero-filled: every row entry and count is a valid index / length before any build
+        bchk(hipMemset(nlist, 0, sizeof(unsigned int) * (size_t)N * nmax), "clear rows");
+        bchk(hipMemset(cell_idx, 0, sizeof(unsigned int) * (size_t)ncell * cmax), "clear cells");
+        if (n_neigh)
+            bchk(hipMemset(n_neigh, 0, sizeof(unsigned int) * N), "clear counts");
         clear_graphs();
     }
 
@@ -569,8 +738,11 @@ struct hm_batch
         hipLaunchKernelGGL(bcell_clear_kernel, dim3((ncell + 255) / 256), dim3(256), 0, stream,
                            reps, cell_rep, rst, ncell, cell_size);
         hipLaunchKernelGGL(bcell_bin_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
-                           rst, pos, last, cmax, cell_size, cell_idx, cond);
-        hipLaunchKernelGGL(bnlist_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep, rst,
+                           rst, pos, last, cmax, cell_size, cell_raw, n_neigh, cond);
+        const size_t nslot = (size_t)ncell * cmax;
+        hipLaunchKernelGGL(bcell_rank_kernel, dim3((unsigned int)((nslot + 255) / 256)), dim3(256),
+                           0, stream, cell_rep, rst, ncell, cmax, cell_size, cell_raw, cell_idx);
+        hipLaunchKernelGGL(bnlist_kernel, dim3(ncell), dim3(64), 0, stream, reps, cell_rep, rst,
                            pos, cmax, cell_size, cell_idx, rlistsq, nmax, nlist, n_neigh, cond);
         bchk(hipGetLastError(), "batch rebuild");
     }
@@ -648,6 +820,7 @@ struct hm_batch
             s_last = balloc<d3>(N);
             s_image = balloc<i3>(N);
             s_ke = balloc<double>(ntile);
+            s_tag = balloc<unsigned int>(N);
             s_rst = balloc<RepState>(nrep);
             s_bst = balloc<BatchStep>(1);
         }
@@ -660,6 +833,7 @@ struct hm_batch
         cp(s_last, last, (size_t)N * sizeof(d3));
         cp(s_image, image, (size_t)N * sizeof(i3));
         cp(s_ke, ke_partial, (size_t)ntile * sizeof(double));
+        cp(s_tag, tag, (size_t)N * sizeof(unsigned int));
         cp(s_rst, rst, (size_t)nrep * sizeof(RepState));
         cp(s_bst, bst, sizeof(BatchStep));
     }
@@ -675,13 +849,82 @@ struct hm_batch
         cp(last, s_last, (size_t)N * sizeof(d3));
         cp(image, s_image, (size_t)N * sizeof(i3));
         cp(ke_partial, s_ke, (size_t)ntile * sizeof(double));
+        cp(tag, s_tag, (size_t)N * sizeof(unsigned int));
         cp(rst, s_rst, (size_t)nrep * sizeof(RepState));
         cp(bst, s_bst, sizeof(BatchStep));
     }
 
     uint64_t chunk_len(uint64_t t, uint64_t end) const
     {
-        return std::min<uint64_t>(chunk ? chunk : 1, end - t);
+        uint64_t K = std::min<uint64_t>(chunk ? chunk : 1, end - t);
+        if (sort_period)
+            K = std::min<uint64_t>(K, (t / sort_period + 1) * sort_period - t);
+        return K;
+    }
+
+    // Re-lay every replica's particles in cell order (between chunks, outside any graph): a
+    // fresh binning of all replicas, the per-replica cell starts, the order, the permutation of
+    // every per-particle array, then the kinetic-energy partials of the new tiles and a forced
+    // list build before the next forces.
+    void sort_cells()
+    {
+        if (!cell_start)
+        {
+            cell_start = balloc<unsigned int>(ncell);
+            order = balloc<unsigned int>(N);
+            stage = balloc<double4>(N);
+        }
+        for (int attempt = 0;; attempt++)
+        {
+            bchk(hipMemsetAsync(cond, 0, sizeof(BCond), stream), "clear conditions");
+            hipLaunchKernelGGL(bset_rebuild_kernel, dim3((nrep + 255) / 256), dim3(256), 0,
+                               stream, rst, nrep);
+            hipLaunchKernelGGL(bcell_clear_kernel, dim3((ncell + 255) / 256), dim3(256), 0,
+                               stream, reps, cell_rep, rst, ncell, cell_size);
+            hipLaunchKernelGGL(bcell_bin_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps,
+                               tile_rep, rst, pos, last, cmax, cell_size, cell_raw, n_neigh,
+                               cond);
+            const size_t nslot = (size_t)ncell * cmax;
+            hipLaunchKernelGGL(bcell_rank_kernel, dim3((unsigned int)((nslot + 255) / 256)),
+                               dim3(256), 0, stream, cell_rep, rst, ncell, cmax, cell_size,
+                               cell_raw, cell_idx);
+            bchk(hipGetLastError(), "sort binning");
+            read_cond();
+            if (h_cond->cell_need <= cmax)
+                break;
+            if (attempt > 4)
+                throw BError{HM_ERR_LAUNCH, "batch sort: cell list kept overflowing"};
+            cmax = round_up(h_cond->cell_need + 8, 8);
+            alloc_lists();
+        }
+        const size_t nslot = (size_t)ncell * cmax;
+        hipLaunchKernelGGL(bcell_start_kernel, dim3(nrep), dim3(64), 0, stream, reps, cell_size,
+                           cmax, cell_start);
+        hipLaunchKernelGGL(bcell_order_kernel, dim3((unsigned int)((nslot + 255) / 256)),
+                           dim3(256), 0, stream, ncell, cmax, cell_size, cell_start, cell_idx,
+                           order);
+        bchk(hipGetLastError(), "sort order");
+        auto permute = [&](void* arr, unsigned int bytes) {
+            bchk(gather_bytes(arr, stage, order, N, bytes, stream), "permute");
+            bchk(hipMemcpyAsync(arr, stage, (size_t)N * bytes, hipMemcpyDeviceToDevice, stream),
+                 "permute copy");
+        };
+        permute(pos, sizeof(double4));
+        permute(vel, sizeof(double4));
+        permute(accel, sizeof(d3));
+        permute(image, sizeof(i3));
+        permute(last, sizeof(d3));
+        permute(force, sizeof(float4));
+        permute(tag, sizeof(unsigned int));
+        // kinetic-energy partials of the new tiles (accelerations rewritten unchanged), then
+        // every replica rebuilds its lists (rows hold old indices) before the next forces
+        hipLaunchKernelGGL(bstep_two_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
+                           rst, bst, force, vel, accel, ke_partial, dt, 0);
+        hipLaunchKernelGGL(bset_rebuild_kernel, dim3((nrep + 255) / 256), dim3(256), 0, stream,
+                           rst, nrep);
+        bchk(hipGetLastError(), "sort finish");
+        last_sort = timestep;
+        n_sorts++;
     }
 
     int prime(uint64_t nsteps)
@@ -710,6 +953,8 @@ struct hm_batch
         int retries = 0;
         while (timestep < end)
         {
+            if (sort_period && timestep % sort_period == 0 && last_sort != timestep)
+                sort_cells();
             const unsigned int K = (unsigned int)chunk_len(timestep, end);
             bchk(hipMemsetAsync(cond, 0, sizeof(BCond), stream), "clear conditions");
             snapshot();
@@ -779,6 +1024,17 @@ struct hm_batch
     }
 
     void tune();
+
+    // index at creation of the particle now at each position (replica offset + its tag)
+    std::vector<unsigned int> creation_index()
+    {
+        std::vector<unsigned int> t(N), pr(N);
+        bchk(hipMemcpy(t.data(), tag, sizeof(unsigned int) * N, hipMemcpyDeviceToHost), "tag");
+        for (const Rep& R : hrep)
+            for (unsigned int k = 0; k < R.n; k++)
+                t[R.off + k] += R.off;
+        return t;
+    }
 };
 
 namespace
@@ -898,7 +1154,7 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         b->pc.rcutsq_f = (float)(r_cut * r_cut);
         b->hrep.resize(nrep);
         unsigned long long off = 0, coff = 0, toff = 0;
-        double rho_max = 0.0;
+        double rho_max = 0.0, cell_fill_max = 0.0;
         for (unsigned int q = 0; q < nrep; q++)
         {
             Rep& R = b->hrep[q];
@@ -926,6 +1182,10 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
             off += n[q];
             coff += (unsigned long long)R.dim * R.dim * R.dim;
             toff += R.ntile;
+            // densest cell: the replica's density times its real cell volume (w = L / dim can
+            // be up to twice r_list)
+            const double w = L[q] / R.dim;
+            cell_fill_max = std::max(cell_fill_max, n[q] / (L[q] * L[q] * L[q]) * w * w * w);
             rho_max = std::max(rho_max, n[q] / (L[q] * L[q] * L[q]));
             if (off >= (1ull << 31) || coff >= (1ull << 31))
                 throw BError{HM_ERR_ARGS, "hm_batch_create: too many particles or cells"};
@@ -934,8 +1194,7 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         b->ncell = (unsigned int)coff;
         b->ntile = (unsigned int)toff;
         // first list sizes from the densest replica; overflow regrows them
-        const double cellvol = rlist * rlist * rlist;
-        b->cmax = round_up((unsigned int)(rho_max * cellvol * 1.5) + 16, 8);
+        b->cmax = std::min(NL_HOME_MAX, round_up((unsigned int)(cell_fill_max * 1.5) + 16, 8));
         b->nmax = round_up((unsigned int)(4.18879 * rlist * rlist * rlist * rho_max * 1.4) + 32,
                            16);
         bchk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
@@ -958,8 +1217,9 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         b->ke_partial = balloc<double>(b->ntile);
         b->cell_size = balloc<unsigned int>(b->ncell);
         b->n_neigh = balloc<unsigned int>(N);
+        b->tag = balloc<unsigned int>(N);
         b->alloc_lists();
-        std::vector<unsigned int> trep(b->ntile), crep(b->ncell), prep_(N);
+        std::vector<unsigned int> trep(b->ntile), crep(b->ncell), prep_(N), htag(N);
         std::vector<double4> hp(N), hv(N);
         for (unsigned int q = 0; q < nrep; q++)
         {
@@ -985,6 +1245,7 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
                 hv[p] = make_double4(vel3 ? vel3[3 * p] : 0.0, vel3 ? vel3[3 * p + 1] : 0.0,
                                      vel3 ? vel3[3 * p + 2] : 0.0, mass ? mass[p] : 1.0);
                 prep_[p] = q;
+                htag[p] = i;
             }
         }
         auto up = [&](void* d, const void* s, size_t bytes) {
@@ -994,6 +1255,7 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         up(b->tile_rep, trep.data(), sizeof(unsigned int) * b->ntile);
         up(b->cell_rep, crep.data(), sizeof(unsigned int) * b->ncell);
         up(b->part_rep, prep_.data(), sizeof(unsigned int) * N);
+        up(b->tag, htag.data(), sizeof(unsigned int) * N);
         up(b->pos, hp.data(), sizeof(double4) * N);
         up(b->vel, hv.data(), sizeof(double4) * N);
         bchk(hipMemset(b->image, 0, sizeof(i3) * N), "clear");
@@ -1023,26 +1285,17 @@ int hm_batch_thermalize(hm_batch* b, uint64_t timestep)
     return bguard([&]() {
         // ParticleGroup::thermalizeParticleMomenta per replica: tags 0..n-1 of the replica, its
         // seed and kT, its own COM momentum removed
-        unsigned int nmax_rep = 0;
-        for (const Rep& R : b->hrep)
-            nmax_rep = std::max(nmax_rep, R.n);
-        unsigned int* tags = balloc<unsigned int>(nmax_rep);
         double* scratch = balloc<double>(THERMO_BLOCKS * 3);
-        std::vector<unsigned int> ht(nmax_rep);
-        for (unsigned int i = 0; i < nmax_rep; i++)
-            ht[i] = i;
-        hipError_t e = hipMemcpy(tags, ht.data(), sizeof(unsigned int) * nmax_rep,
-                                 hipMemcpyHostToDevice);
+        hipError_t e = hipSuccess;
         for (const Rep& R : b->hrep)
         {
             if (e != hipSuccess)
                 break;
-            e = launch_thermalize(R.n, b->vel + R.off, tags, R.kT, timestep, (uint16_t)R.seed,
-                                  scratch, b->stream);
+            e = launch_thermalize(R.n, b->vel + R.off, b->tag + R.off, R.kT, timestep,
+                                  (uint16_t)R.seed, scratch, b->stream);
         }
         if (e == hipSuccess)
             e = hipStreamSynchronize(b->stream);
-        bfree(tags);
         bfree(scratch);
         bchk(e, "thermalize");
         b->prepared = false;
@@ -1074,6 +1327,15 @@ int hm_batch_prime(hm_batch* b, uint64_t nsteps)
     if (!b)
         return fail(HM_ERR_ARGS, "hm_batch_prime: NULL batch");
     return bguard([&]() { return b->prime(nsteps); });
+}
+
+int hm_batch_set_sort_period(hm_batch* b, unsigned int period)
+{
+    if (!b)
+        return fail(HM_ERR_ARGS, "hm_batch_set_sort_period: NULL batch");
+    b->sort_period = period;
+    b->clear_graphs();
+    return HM_OK;
 }
 
 int hm_batch_set_chunk(hm_batch* b, unsigned int steps)
@@ -1134,25 +1396,27 @@ int hm_batch_get_state(hm_batch* b, double* pos3, double* vel3, int* image3)
         bchk(hipMemcpy(p.data(), b->pos, sizeof(double4) * N, hipMemcpyDeviceToHost), "pos");
         bchk(hipMemcpy(v.data(), b->vel, sizeof(double4) * N, hipMemcpyDeviceToHost), "vel");
         bchk(hipMemcpy(im.data(), b->image, sizeof(i3) * N, hipMemcpyDeviceToHost), "image");
-        for (unsigned int i = 0; i < N; i++)
+        const std::vector<unsigned int> where = b->creation_index();
+        for (unsigned int n = 0; n < N; n++)
         {
+            const unsigned int i = where[n];
             if (pos3)
             {
-                pos3[3 * i] = p[i].x;
-                pos3[3 * i + 1] = p[i].y;
-                pos3[3 * i + 2] = p[i].z;
+                pos3[3 * i] = p[n].x;
+                pos3[3 * i + 1] = p[n].y;
+                pos3[3 * i + 2] = p[n].z;
             }
             if (vel3)
             {
-                vel3[3 * i] = v[i].x;
-                vel3[3 * i + 1] = v[i].y;
-                vel3[3 * i + 2] = v[i].z;
+                vel3[3 * i] = v[n].x;
+                vel3[3 * i + 1] = v[n].y;
+                vel3[3 * i + 2] = v[n].z;
             }
             if (image3)
             {
-                image3[3 * i] = im[i].x;
-                image3[3 * i + 1] = im[i].y;
-                image3[3 * i + 2] = im[i].z;
+                image3[3 * i] = im[n].x;
+                image3[3 * i + 1] = im[n].y;
+                image3[3 * i + 2] = im[n].z;
             }
         }
         return HM_OK;
@@ -1170,12 +1434,14 @@ int hm_batch_get_forces(hm_batch* b, double* force4)
         bchk(hipStreamSynchronize(b->stream), "sync");
         bchk(hipMemcpy(f.data(), b->force, sizeof(float4) * b->N, hipMemcpyDeviceToHost),
              "force");
-        for (unsigned int i = 0; i < b->N; i++)
+        const std::vector<unsigned int> where = b->creation_index();
+        for (unsigned int n = 0; n < b->N; n++)
         {
-            force4[4 * i] = f[i].x;
-            force4[4 * i + 1] = f[i].y;
-            force4[4 * i + 2] = f[i].z;
-            force4[4 * i + 3] = f[i].w;
+            const unsigned int i = where[n];
+            force4[4 * i] = f[n].x;
+            force4[4 * i + 1] = f[n].y;
+            force4[4 * i + 2] = f[n].z;
+            force4[4 * i + 3] = f[n].w;
         }
         return HM_OK;
     });

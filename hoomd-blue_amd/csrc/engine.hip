@@ -14,6 +14,7 @@
 // (NeighborList.cc:411-427, CellList.cc:183-196) without a per-step host sync.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <new>
@@ -106,9 +107,41 @@ bool is_pair(const ForceSlot& f)
 {
     return f.kind == 0 || f.kind == 2 || f.kind == 3;
 }
+
+// out[3 tag + d] -= f[i].d for every particle i (frames are stored by tag)
+__global__ __launch_bounds__(256) void subtract_by_tag_kernel(unsigned int N,
+                                                              const unsigned int* __restrict__ tag,
+                                                              const float4* __restrict__ f,
+                                                              float* __restrict__ out)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const float4 v = f[i];
+    const size_t t = tag[i];
+    out[3 * t] -= v.x;
+    out[3 * t + 1] -= v.y;
+    out[3 * t + 2] -= v.z;
+}
 bool is_bond(const ForceSlot& f)
 {
     return f.kind == 1 || f.kind == 4;
+}
+} // namespace
+
+namespace
+{
+struct TuneChoice
+{
+    PairLaunch pair;
+    int nl_shape;
+    unsigned int nl_grid;
+};
+// launch shapes chosen by the autotuner, per kind of system, for the life of the process
+std::map<std::string, TuneChoice>& g_tune_cache()
+{
+    static std::map<std::string, TuneChoice> cache;
+    return cache;
 }
 } // namespace
 
@@ -192,6 +225,14 @@ struct hm_system
     int pair_kernel_kind = 2;
     bool autotune = true;             // sweep the pair launch configuration at the next prep
     bool pair_tuned = false;
+    // list-build launch shape (0: from the occupancy) and persistent grid, from the autotuner
+    int nl_shape = 0;
+    unsigned int nl_grid = 0;
+    // second sweep once the system has run this many steps since its first prep (the lattice
+    // start is more ordered than the liquid it melts into)
+    uint64_t retune_after = 500;
+    bool retuned = false;
+    bool tune_cached = false;         // the last sweep was answered by the shape cache
     unsigned int chunk = 100;
     bool use_graph = true;
     std::map<unsigned int, hipGraphExec_t> graphs;
@@ -494,6 +535,8 @@ struct hm_system
         k.skip_index_rows = pair_kind() == 3 ? 1 : 0;
         k.offset_codes = pair_kind() == 2 ? 1 : 0;
         k.cond = &cond->nlist_need;
+        k.shape = nl_shape;
+        k.grid = nl_grid;
         k.gate = gated ? st : nullptr;
         k.counter = st;
         k.check_delay = check_delay;
@@ -934,6 +977,117 @@ struct hm_system
     // particle {1, 2, 4, 8, 16} x block size {64, 128, 256, 512, 1024} for the gather kernels,
     // lanes per particle {4, 8, 16} for the cell-tiled kernel. The pair launches only rewrite
     // the force arrays with the same values, so tuning does not change the state.
+    // Sweep of the list build's workgroup shape and persistent grid on the lists of the current
+    // state (cells already built; every candidate writes the same rows), median of 3 launches.
+    struct NlSample
+    {
+        int shape;
+        unsigned int grid;
+        float ms;
+    };
+    std::vector<NlSample> nl_log;
+    void tune_nlist()
+    {
+        nl_log.clear();
+        if (rcut_max <= 0.0 || !nlist)
+            return;
+        std::vector<int> shapes{0};
+        if (cmax <= 64)
+            shapes = {NL_SHAPE_TINY, NL_SHAPE_SMALL, NL_SHAPE_MID, NL_SHAPE_BIG};
+        hipEvent_t ev[4];
+        for (auto& e : ev)
+            hchk(hipEventCreate(&e), "event");
+        float best = 1e30f;
+        int bs = nl_shape;
+        unsigned int bg = nl_grid;
+        for (int sh : shapes)
+        {
+            const bool one_wave = sh == NL_SHAPE_TINY || sh == NL_SHAPE_SMALL;
+            for (unsigned int g : one_wave ? std::vector<unsigned int>{2048u, 4096u, 8192u, 16384u}
+                                           : std::vector<unsigned int>{1024u, 2048u, 4096u, 8192u})
+            {
+                if (g > round_up(ncell, 8) && g != 1024u && g != 2048u)
+                    continue;
+                nl_shape = sh;
+                nl_grid = g;
+                NlistKernelArgs k;
+                fill_nlist_args(k, false);
+                k.counter = nullptr;
+                hchk(launch_nlist_build(k, stream), "nlist tune");
+                for (int r = 0; r < 3; r++)
+                {
+                    hchk(hipEventRecord(ev[r], stream), "event");
+                    hchk(launch_nlist_build(k, stream), "nlist tune");
+                }
+                hchk(hipEventRecord(ev[3], stream), "event");
+                hchk(hipEventSynchronize(ev[3]), "event");
+                float t[3];
+                for (int r = 0; r < 3; r++)
+                    hchk(hipEventElapsedTime(&t[r], ev[r], ev[r + 1]), "event");
+                std::sort(t, t + 3);
+                nl_log.push_back({sh, g, t[1]});
+                if (t[1] < best)
+                {
+                    best = t[1];
+                    bs = sh;
+                    bg = g;
+                }
+            }
+        }
+        for (auto& e : ev)
+            (void)hipEventDestroy(e);
+        nl_shape = bs;
+        nl_grid = bg;
+        clear_graphs();
+    }
+
+    // the whole sweep: the shape cache (same kind of system seen before in this process), else
+    // the list build then the pair kernel, and the result stored in the cache
+    void tune_all(bool use_cache)
+    {
+        const std::string key = tune_key();
+        auto it = g_tune_cache().find(key);
+        if (use_cache && it != g_tune_cache().end())
+        {
+            const TuneChoice& c = it->second;
+            pair_launch_cfg() = c.pair;
+            nl_shape = c.nl_shape;
+            nl_grid = c.nl_grid;
+            pair_tuned = true;
+            tune_cached = true;
+            tune_log.clear();
+            nl_log.clear();
+            clear_graphs();
+            return;
+        }
+        tune_cached = false;
+        tune_nlist();
+        tune_pair();
+        g_tune_cache()[key] = TuneChoice{pair_launch_cfg(), nl_shape, nl_grid};
+    }
+
+    PairLaunch& pair_launch_cfg()
+    {
+        const int kind = pair_kind();
+        return kind == 0 ? pair_cell_cfg : (kind == 3 ? pair_tile_cfg : pair_cfg);
+    }
+
+    // bucket of systems that share launch shapes: pair kind, force kinds, types, exclusions,
+    // log2 N in half steps, density, list range and cells per axis (coarse)
+    std::string tune_key() const
+    {
+        const double vol = hbox.L[0] * hbox.L[1] * hbox.L[2];
+        char buf[256];
+        std::string forces_kinds;
+        for (auto& f : forces)
+            forces_kinds += char('0' + f.kind);
+        std::snprintf(buf, sizeof(buf), "k%d f%s t%u x%d n%ld r%ld l%ld c%u", pair_kind(),
+                      forces_kinds.c_str(), ntypes, (max_excl && exclude_bonds) ? 1 : 0,
+                      std::lround(2.0 * std::log2((double)N)), std::lround(N / vol * 40.0),
+                      std::lround((rcut_max + r_buff) * 10.0), std::min(dim[0], 64u) / 8);
+        return buf;
+    }
+
     void tune_pair()
     {
         ForceSlot* f = nullptr;
@@ -1031,7 +1185,7 @@ struct hm_system
         if (rcut_max > 0.0)
             build_nlist_now();
         if (autotune && !pair_tuned)
-            tune_pair();
+            tune_all(true);
         launch_forces(compute_virial);
         const float4* fp[4];
         unsigned int nf = force_ptrs(fp);
@@ -1195,6 +1349,9 @@ struct hm_system
                 uint64_t next_sort = (t / sort_period + 1) * sort_period;
                 K = std::min<uint64_t>(K, next_sort - t);
             }
+            const uint64_t ran = steps_run + (t - timestep);
+            if (autotune && !retuned && !tune_cached && ran < retune_after)
+                K = std::min<uint64_t>(K, retune_after - ran);
             if (graphs.find((unsigned int)K) == graphs.end())
             {
                 graphs.emplace((unsigned int)K, capture_steps((unsigned int)K));
@@ -1225,6 +1382,15 @@ struct hm_system
         int retries = 0;
         while (timestep < end)
         {
+            if (autotune && !retuned && steps_run >= retune_after && !tune_cached)
+            {
+                // second sweep on the state the run has reached (outside any captured chunk)
+                retuned = true;
+                tune_all(false);
+                // the sweep's list builds reused the last cell binning: rebuild both before the
+                // forces of the next step
+                hchk(launch_set_rebuild(st, stream), "force rebuild");
+            }
             if (sort_period && timestep % sort_period == 0 && last_sort_step != timestep)
                 do_sort();
             uint64_t K = std::min<uint64_t>(chunk ? chunk : 1, end - timestep);
@@ -1233,6 +1399,8 @@ struct hm_system
                 uint64_t next_sort = (timestep / sort_period + 1) * sort_period;
                 K = std::min<uint64_t>(K, next_sort - timestep);
             }
+            if (autotune && !retuned && !tune_cached && steps_run < retune_after)
+                K = std::min<uint64_t>(K, retune_after - steps_run);
             hchk(hipMemsetAsync(cond, 0, sizeof(Conditions), stream), "clear conditions");
             snapshot();
             launch_chunk((unsigned int)K);
@@ -1755,6 +1923,28 @@ int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows)
     return (int)s->tune_log.size();
 }
 
+void hm_tune_cache_clear(void)
+{
+    g_tune_cache().clear();
+}
+
+int hm_sys_nlist_launch(const hm_system* s, int out[3], float* log3, unsigned int max_rows)
+{
+    if (!s || !out || (!log3 && max_rows))
+        return fail(HM_ERR_ARGS, "hm_sys_nlist_launch: invalid arguments");
+    out[0] = s->nl_shape;
+    out[1] = (int)s->nl_grid;
+    out[2] = s->tune_cached ? 1 : 0;
+    const unsigned int n = (unsigned int)std::min<size_t>(max_rows, s->nl_log.size());
+    for (unsigned int q = 0; q < n; q++)
+    {
+        log3[3 * q + 0] = (float)s->nl_log[q].shape;
+        log3[3 * q + 1] = (float)s->nl_log[q].grid;
+        log3[3 * q + 2] = s->nl_log[q].ms;
+    }
+    return (int)s->nl_log.size();
+}
+
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph)
 {
     if (!s)
@@ -2034,6 +2224,26 @@ int hm_sys_stats(hm_system* s, uint64_t* out9)
     });
 }
 
+int hm_sys_set_list_capacity(hm_system* s, unsigned int cmax, unsigned int nmax)
+{
+    if (!s || cmax == 0 || cmax > 256 || nmax == 0)
+        return fail(HM_ERR_ARGS, "hm_sys_set_list_capacity: 0 < cmax <= 256, nmax > 0");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        // fresh zero-filled tables (every slot and row index stays in range until the forced
+        // build below refills them), then a forced build before the forces of the next step
+        s->cmax = cmax;
+        s->alloc_cells();
+        s->nmax = (nmax + ROW_BLOCK - 1) & ~(ROW_BLOCK - 1);
+        s->alloc_rows();
+        s->clear_graphs();
+        hchk(launch_set_rebuild(s->st, s->stream), "force rebuild");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        return HM_OK;
+    });
+}
+
 int hm_sys_set_nlist_policy(hm_system* s, unsigned int rebuild_check_delay, int check_dist)
 {
     if (!s)
@@ -2234,6 +2444,30 @@ int hm_sys_mlp_descriptors(hm_system* s, int force_id, const float* frame_pos, f
     });
 }
 
+int hm_sys_frame_subtract_force(hm_system* s, int force_id, const float* frame_pos,
+                                float* frame_force)
+{
+    if (!s || force_id < 0 || force_id >= (int)s->forces.size() || !frame_pos || !frame_force
+        || !(is_pair(s->forces[force_id]) && s->forces[force_id].kind != 3))
+        return fail(HM_ERR_ARGS, "hm_sys_frame_subtract_force: a pair force and device frames");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        hchk(launch_mlp_load_frame(s->N, s->rtag, frame_pos, nullptr, s->box, s->pos, nullptr,
+                                   s->stream),
+             "load frame");
+        s->prepared = false;  // the next run starts from these positions: prepare again
+        s->build_nlist_now();
+        ForceSlot& f = s->forces[force_id];
+        s->launch_pair(f, 0);
+        hipLaunchKernelGGL(subtract_by_tag_kernel, dim3((s->N + 255) / 256), dim3(256), 0,
+                           s->stream, s->N, s->tag, f.f, frame_force);
+        hchk(hipGetLastError(), "subtract force");
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        return HM_OK;
+    });
+}
+
 int hm_sys_mlp_loss_grad(hm_system* s, int force_id, const float* frame_pos,
                          const float* frame_force, double* loss, float* grad)
 {
@@ -2389,6 +2623,65 @@ int hm_sys_time_pair(hm_system* s, int force_id, unsigned int iters, double* avg
                 tot += std::min(v, s->nmax);
             *nlist_entries = tot;
         }
+        return HM_OK;
+    });
+}
+
+int hm_sys_time_build(hm_system* s, unsigned int iters, double* avg_ms2)
+{
+    if (!s || iters == 0 || !avg_ms2)
+        return fail(HM_ERR_ARGS, "hm_sys_time_build: invalid arguments");
+    return guarded([&]() {
+        if (!s->prepared)
+            s->prep();
+        if (s->rcut_max <= 0.0)
+            throw HmError{HM_ERR_STATE, "hm_sys_time_build: no neighbour list"};
+        hipEvent_t e[3];
+        for (auto& x : e)
+            hchk(hipEventCreate(&x), "event");
+        float tc = 0.f, tn = 0.f;
+        for (unsigned int q = 0; q < iters; q++)
+        {
+            // a complete rebuild at the current positions, launched as a step launches it
+            hchk(hipEventRecord(e[0], s->stream), "event");
+            hchk(launch_cell_clear(s->ncell, s->cell_size, nullptr, s->stream), "cell clear");
+            if (s->N <= CELL_FUSED_MAX_N)
+                hchk(launch_cell_bin_rank(s->N, s->dim, s->cmax, s->pos, s->cell_size, s->last,
+                                          &s->cond->cell_need, s->ckey, s->chome, s->entries,
+                                          s->cell_idx, nullptr, s->stream, s->box,
+                                          s->rel_frame()),
+                     "cell build and order");
+            else
+            {
+                hchk(launch_cell_build(s->N, s->pos, s->box, s->dim, s->cmax, s->cell_size,
+                                       nullptr, s->last, &s->cond->cell_need, nullptr, s->stream,
+                                       s->ckey, s->chome),
+                     "cell build");
+                hchk(launch_cell_rank(s->N, s->cmax, s->pos, s->cell_size, s->ckey, s->chome,
+                                      s->entries, s->cell_idx, nullptr, s->stream, s->box,
+                                      s->rel_frame()),
+                     "cell order");
+            }
+            hchk(hipEventRecord(e[1], s->stream), "event");
+            NlistKernelArgs k;
+            s->fill_nlist_args(k, false);
+            k.counter = nullptr;
+            hchk(launch_nlist_build(k, s->stream), "nlist build");
+            hchk(hipEventRecord(e[2], s->stream), "event");
+            hchk(hipEventSynchronize(e[2]), "event");
+            float a = 0.f, b = 0.f;
+            hchk(hipEventElapsedTime(&a, e[0], e[1]), "event");
+            hchk(hipEventElapsedTime(&b, e[1], e[2]), "event");
+            tc += a;
+            tn += b;
+        }
+        for (auto& x : e)
+            (void)hipEventDestroy(x);
+        avg_ms2[0] = (double)tc / iters;
+        avg_ms2[1] = (double)tn / iters;
+        // the run continues with the regular build machinery
+        hchk(launch_set_rebuild(s->st, s->stream), "force rebuild");
+        hchk(hipStreamSynchronize(s->stream), "sync");
         return HM_OK;
     });
 }

@@ -196,6 +196,18 @@ struct NlistKernelArgs
     const unsigned int* sidx;
     const uint32_t* swrd;
     float cwf[3];             // cell widths rounded to FP32
+    // launch shape, 0 = chosen from the cell occupancy. shape: workgroup form of the FP32 path
+    // (NL_SHAPE_*; a form whose home-cell capacity is below cmax falls back to the default);
+    // grid: persistent workgroups (rounded up to a multiple of 8)
+    int shape;
+    unsigned int grid;
+};
+enum : int
+{
+    NL_SHAPE_TINY = 1,
+    NL_SHAPE_SMALL = 2,
+    NL_SHAPE_MID = 3,
+    NL_SHAPE_BIG = 4,
 };
 
 // Cell-tiled pair kernel: neighbours read from the home cell's LDS-staged candidates through

@@ -1080,12 +1080,21 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
     if (a_in.ncell == 0)
         return hipSuccess;
     NlistKernelArgs a = a_in;
-    const unsigned int grid = round_up(min(a.ncell, GATED_BLOCKS), 8);
+    const unsigned int grid = round_up(a.grid ? a.grid : min(a.ncell, GATED_BLOCKS), 8);
     // small cells: a few particles per cell on average, and every home cell fits the
     // one-wavefront workgroup (cmax <= 64)
-    const bool small = a.cmax <= (unsigned int)NlSmall::HOME && a.N <= 8ull * a.ncell;
-    const bool tiny = small && a.N <= 4ull * a.ncell;
-    const unsigned int grid_small = round_up(min(a.ncell, 4 * GATED_BLOCKS), 8);
+    bool small = a.cmax <= (unsigned int)NlSmall::HOME && a.N <= 8ull * a.ncell;
+    bool tiny = small && a.N <= 4ull * a.ncell;
+    const unsigned int grid_small = round_up(a.grid ? a.grid : min(a.ncell, 4 * GATED_BLOCKS), 8);
+    // an explicit shape (the autotuner's) where its home-cell capacity holds cmax; the staging
+    // passes of every shape take any number of candidates
+    bool mid_ok = a.cmax <= (unsigned int)NlMid::HOME;
+    if (a.shape && a.cmax <= (unsigned int)NlSmall::HOME)
+    {
+        tiny = a.shape == NL_SHAPE_TINY;
+        small = tiny || a.shape == NL_SHAPE_SMALL;
+        mid_ok = a.shape == NL_SHAPE_MID;
+    }
     const bool mt = a.ntypes > 1;
     const bool ex = a.n_excl != nullptr && a.max_excl > 0;
     const bool fast = a.pos != nullptr && a.dim[0] >= 4 && a.dim[1] >= 4 && a.dim[2] >= 4;
@@ -1114,7 +1123,7 @@ hipError_t launch_nlist_build(const NlistKernelArgs& a_in, hipStream_t s)
             nlist_dispatch<true, NlTiny>(a, mt, ex, grid_small, s);
         else if (small)
             nlist_dispatch<true, NlSmall>(a, mt, ex, grid_small, s);
-        else if (a.cmax <= (unsigned int)NlMid::HOME)
+        else if (mid_ok)
             nlist_dispatch<true, NlMid>(a, mt, ex, grid, s);
         else
             nlist_dispatch<true, NlBig>(a, mt, ex, grid, s);

@@ -185,6 +185,11 @@ SIGNATURES = {
     "hm_sys_set_site_map": (_i, [_p, _u, _p, _p]),
     "hm_sys_map_frame": (_i, [_p, _p, _p]),
     "hm_sys_time_mlp": (_i, [_p, _i, _u, _p]),
+    "hm_sys_time_build": (_i, [_p, _u, _p]),
+    "hm_sys_set_list_capacity": (_i, [_p, _u, _u]),
+    "hm_sys_nlist_launch": (_i, [_p, _p, _p, _u]),
+    "hm_tune_cache_clear": (None, []),
+    "hm_sys_frame_subtract_force": (_i, [_p, _i, _p, _p]),
     "hm_frames_sub_harmonic": (_i, [_u, _u, _p, _p, _p, _p, _p, _d, _d, HmBox, _p]),
     # reference-layout pair driver (include/hoomd_amd_pair.h)
     "hm_compute_pair_forces_lj": (_i, [_p, _p, _p]),
@@ -196,6 +201,7 @@ SIGNATURES = {
     "hm_batch_run": (_i, [_p, _u64]),
     "hm_batch_prime": (_i, [_p, _u64]),
     "hm_batch_set_chunk": (_i, [_p, _u]),
+    "hm_batch_set_sort_period": (_i, [_p, _u]),
     "hm_batch_tune": (_i, [_p, _p, _u]),
     "hm_batch_set_pair_launch": (_i, [_p, _u, _u]),
     "hm_batch_config": (_i, [_p, _p]),

@@ -61,10 +61,12 @@ class ReplicaBatch:
     buffer=0.4, dt=0.005, tau=0.0)
 
     positions: list of [N_r, 3] arrays; L, kT, seeds: one per replica. velocities: list of
-    [N_r, 3] arrays or None (then call ``thermalize``)."""
+    [N_r, 3] arrays or None (then call ``thermalize``). Every sort_period steps (0: never) each
+    replica's particles are re-laid in cell order in device memory; ``state`` and ``forces``
+    always return them in the order given here."""
 
     def __init__(self, positions, L, kT, seeds, velocities=None, epsilon=1.0, sigma=1.0,
-                 r_cut=2.5, buffer=0.4, dt=0.005, tau=0.0):
+                 r_cut=2.5, buffer=0.4, dt=0.005, tau=0.0, sort_period=100):
         self.nrep = len(positions)
         self.n = np.array([len(p) for p in positions], np.uint32)
         self.L = np.ascontiguousarray(L, np.float64).reshape(self.nrep)
@@ -84,6 +86,7 @@ class ReplicaBatch:
         if not self._h:
             raise _lib.HoomdAmdError(-1, "hm_batch_create: " + _lib.lib().hm_last_error().decode())
         self.timestep = 0
+        _lib.check(_lib.lib().hm_batch_set_sort_period(self._h, int(sort_period)), "sort period")
 
     def __del__(self):
         if getattr(self, "_h", None):

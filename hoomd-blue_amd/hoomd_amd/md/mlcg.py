@@ -20,8 +20,12 @@ Workflow (multiscale coarse graining by force matching):
    beads of each chain, 5:1 for the Kremer-Grest melt).
 2. ``FrameSet.record`` maps frames of the fine-grained run on the GPU: site centres of mass and
    the summed conservative forces.
-3. ``boltzmann_harmonic`` fits the CG bonds from the mapped bond-length distribution.
-4. ``SiteEnergyMLP.train`` matches the remaining force (mapped force minus CG bond force).
+3. ``boltzmann_harmonic`` fits the CG bonds from the mapped bond-length distribution, and
+   ``rdf_prior_table`` a short-range repulsive pair prior from the mapped site RDF (a Table
+   pair force): the excluded volume of the blobs, which the frames sample too rarely at short
+   distances for the network to learn.
+4. ``SiteEnergyMLP.train`` matches the remaining force (mapped force minus CG bond and prior
+   forces, ``FrameSet.subtract_harmonic`` / ``FrameSet.subtract_pair``).
 """
 
 import ctypes as C
@@ -224,6 +228,7 @@ class FrameSet:
         self.d_force = _lib.DeviceArray((self.nframes * self.nsites * 3,), np.float32, fill=0)
 
     def _off(self, k):
+        k = int(k)
         if not 0 <= k < self.nframes:
             raise IndexError(k)
         return 4 * 3 * self.nsites * k
@@ -264,6 +269,15 @@ class FrameSet:
             self.nframes, self.nsites, self.d_pos.ptr, self.d_force.ptr, self.d_force.ptr,
             d_row.ptr, d_partner.ptr, float(k), float(r0), _lib.box(L), None), "bond forces")
         _lib.check(_lib.lib().hm_synchronize(), "bond forces")
+
+    def subtract_pair(self, sim, force):
+        """Replace every frame's forces by the forces minus those of the pair force ``force``
+        (LJ or Table, attached to ``sim``, whose sites are these frames' sites) at the frame's
+        positions. The simulation's positions are left at the last frame."""
+        sim._attach()
+        for k in range(self.nframes):
+            _lib.check(_lib.lib().hm_sys_frame_subtract_force(
+                sim._system, force._force_id, self.ptr_pos(k), self.ptr_force(k)), "prior force")
 
     def set(self, k, positions=None, forces=None):
         for arr, ptr in ((positions, self.ptr_pos(k)), (forces, self.ptr_force(k))):
@@ -329,6 +343,38 @@ def rdf(pos, L, r_max, nbins=60, exclude_bonded=None):
     rho = N / L ** 3
     vol = 4.0 / 3.0 * np.pi * (edges[1:] ** 3 - edges[:-1] ** 3)
     return 0.5 * (edges[1:] + edges[:-1]), shell / (N * rho * vol)
+
+
+def rdf_prior_table(r, g, kT=1.0, g_join=0.9, width=400, min_count_g=0.02):
+    """Repulsive prior from the short-range Boltzmann inversion of a site RDF (the first step of
+    iterative Boltzmann inversion): U(r) = -kT ln g(r) up to the first bin where g reaches
+    g_join (its r is the returned cut-off), shifted to zero there. Below the first bin with
+    g >= min_count_g the samples are too few, so U continues along a straight line with the
+    slope of the first well-sampled bins (a finite repulsive wall). Returns (Table params,
+    r_cut)."""
+    r = np.asarray(r, np.float64)
+    g = np.asarray(g, np.float64)
+    above = np.nonzero(g >= g_join)[0]
+    if len(above) == 0:
+        raise ValueError("rdf_prior_table: g(r) never reaches g_join")
+    ic = int(above[0])
+    rc = float(r[ic])
+    ok = np.nonzero(g[:ic + 1] >= min_count_g)[0]
+    i0 = int(ok[0])
+    U = np.full(len(r), np.nan)
+    U[i0:ic + 1] = -kT * np.log(g[i0:ic + 1])
+    U -= U[ic]
+    i1 = min(i0 + 3, ic)
+    slope = (U[i1] - U[i0]) / (r[i1] - r[i0]) if i1 > i0 else -10.0 * kT
+    slope = min(slope, -kT)  # a wall: U falls with r
+    U[:i0] = U[i0] + slope * (r[:i0] - r[i0])
+    # the wall continues down to r = 0
+    r_ext = np.concatenate([[0.0], r[:ic + 1]])
+    U_ext = np.concatenate([[U[i0] + slope * (0.0 - r[i0])], U[:ic + 1]])
+    rr = np.linspace(0.0, rc, width, endpoint=False)
+    Ut = np.interp(rr, r_ext, U_ext)
+    Ft = -np.gradient(Ut, rr)
+    return dict(r_min=0.0, U=Ut, F=Ft), rc
 
 
 def soft_cosine_table(A, r_cut, width=200):

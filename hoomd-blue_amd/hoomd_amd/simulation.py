@@ -33,6 +33,10 @@ class Operations:
     @integrator.setter
     def integrator(self, value):
         self._integrator = value
+        # forces know their simulation from here on (methods such as a learned force's
+        # descriptors() attach the simulation themselves when first called)
+        for f in getattr(value, "forces", None) or []:
+            f._simulation = self._simulation
         self._simulation._invalidate()
 
     def add(self, op):

@@ -463,6 +463,15 @@ int hm_sys_pair_config(const hm_system* s, unsigned int out[5]);
 /* Autotuner samples of the last sweep: rows of (tpp, size, batch, median ms), at most max_rows
  * written; returns the number of rows of the sweep. */
 int hm_sys_tune_log(const hm_system* s, float* out, unsigned int max_rows);
+/* List-build launch chosen by the autotuner: out[0] = workgroup shape (0 default, 1 tiny,
+ * 2 small, 3 mid, 4 big), out[1] = persistent grid (0 default), out[2] = 1 when the choice came
+ * from the process's cache of earlier sweeps (same pair kind, forces, types, exclusions, size,
+ * density, list range and cell grid). log3 receives rows (shape, grid, median ms) of the last
+ * sweep; returns its number of rows. The autotuner sweeps at the first run and once more after
+ * 500 steps (on the state the run has reached), unless the cache answered. */
+int hm_sys_nlist_launch(const hm_system* s, int out[3], float* log3, unsigned int max_rows);
+/* Forget every launch shape the autotuner has cached in this process. */
+void hm_tune_cache_clear(void);
 /* Steps per captured hipGraph chunk (0 disables graphs; default 100). */
 int hm_sys_set_chunk(hm_system* s, unsigned int steps, int use_graph);
 /* Advance nsteps timesteps. Returns HM_OK or a negative error (state is left at the last good
@@ -492,6 +501,12 @@ int hm_sys_thermo(hm_system* s, double* out15);
  * last run (NeighborList num_builds), out[7] = shortest period between distance-check builds in
  * the last run, 100 when none (getSmallestRebuild), out[8] = dangerous builds in the last run. */
 int hm_sys_stats(hm_system* s, uint64_t* out9);
+/* Replace the cell capacity (particles per cell, <= 256) and the row capacity (neighbours per
+ * particle, rounded up to 32) and rebuild the lists before the forces of the next step. A
+ * capacity below the need makes that build overflow: the run restores the chunk's first state,
+ * grows the capacity and replays the chunk (the reference regrows and repeats the build,
+ * NeighborList.cc:411-427, CellList.cc:183-196). */
+int hm_sys_set_list_capacity(hm_system* s, unsigned int cmax, unsigned int nmax);
 /* Thermalize velocities of all particles (ParticleGroup::thermalizeParticleMomenta). */
 int hm_sys_thermalize(hm_system* s, double kT);
 /* Launch stream used by the engine (a hipStream_t). */
@@ -533,6 +548,12 @@ int hm_sys_mlp_train(hm_system* s, int force_id, const hm_mlp_train_args* args);
 int hm_sys_mlp_loss_grad(hm_system* s, int force_id, const float* frame_pos,
                          const float* frame_force, double* loss, float* grad);
 
+/* Force-matching targets minus a known pair force (a prior): the system's positions become
+ * frame_pos (device [N][3] FP32 by tag), the lists are rebuilt, pair force force_id (LJ or
+ * Table) is evaluated and subtracted from frame_force (device [N][3] FP32 by tag). */
+int hm_sys_frame_subtract_force(hm_system* s, int force_id, const float* frame_pos,
+                                float* frame_force);
+
 /* Coarse-grained site map: site I holds the particles with tags tags[row[I]] ..
  * tags[row[I+1]-1] (host arrays, row[nsites+1]). */
 int hm_sys_set_site_map(hm_system* s, unsigned int nsites, const unsigned int* row,
@@ -541,6 +562,11 @@ int hm_sys_set_site_map(hm_system* s, unsigned int nsites, const unsigned int* r
  * d_force[3*nsites] (may be NULL) = summed forces of all force computes (device FP32). */
 int hm_sys_map_frame(hm_system* s, float* d_pos, float* d_force);
 
+/* Benchmark helper (no counterpart in the reference): average durations in ms of a complete
+ * list rebuild at the current positions, iters times, HIP events on the engine stream:
+ * avg_ms2[0] = cell binning and ordering, avg_ms2[1] = neighbour-list build. The next step
+ * rebuilds again through the regular machinery. */
+int hm_sys_time_build(hm_system* s, unsigned int iters, double* avg_ms2);
 /* Benchmark helper (no counterpart in the reference): average durations in ms of the learned
  * site energy's three kernels (densities, site MLP on bf16 MFMA, forces) of force force_id over
  * iters launches each, HIP events on the engine stream. */
@@ -590,6 +616,9 @@ int hm_batch_run(hm_batch* b, uint64_t nsteps);
 int hm_batch_prime(hm_batch* b, uint64_t nsteps);
 /* Steps per captured chunk (default 100). */
 int hm_batch_set_chunk(hm_batch* b, unsigned int steps);
+/* Re-lay each replica's particles in cell order every period steps (default 100; 0 never),
+ * as the reference's particle sorter does with a Hilbert order (SFCPackTuner.cc:119-285). */
+int hm_batch_set_sort_period(hm_batch* b, unsigned int period);
 /* Autotune the pair launch (tpp x block, median of 5) on the current lists; cached per
  * (particles, density) bucket for the process. log3 receives rows (tpp, block, ms); returns
  * the number of rows of the sweep (0 when the cache answered). */

@@ -72,6 +72,8 @@ def lib():
             "os_sort_order": (None, [_p, _sz, _p, _u, _p, _p]),
             "os_create": (_p, [_sz, _d, _d, _d, _p, _p, _p, _d, _d, _d, _i, _d, _d, C.c_uint16]),
             "os_destroy": (None, [_p]),
+            "oracle_set_threads": (None, [_i]),
+            "oracle_threads": (_i, []),
             "os_set_thermostat": (None, [_p, _i, _d, _d]),
             "os_set_sorter": (None, [_p, _u, _u]),
             "os_set_bonds": (None, [_p, _sz, _p, _d, _d, _d, _d, _d, _i]),
@@ -420,6 +422,15 @@ def sort_order(box, pos, grid, traversal=None):
 
 
 # ------------------------------------------------------------------ full system
+def set_threads(n):
+    """Host threads of the step loop (1: the serial order the tests compare against)."""
+    lib().oracle_set_threads(int(n))
+
+
+def threads():
+    return lib().oracle_threads()
+
+
 class System:
     """The restated CPU step loop (half list + 3rd law), by tag."""
 

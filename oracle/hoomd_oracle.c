@@ -10,6 +10,28 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Host threads of the step loop's list build, pair forces and integrator (1 by default: the
+ * tests compare against the serial order of every sum). The reference parallelises its CPU
+ * path over MPI ranks with spatial domains (hoomd/Communicator.h:90-125); the benchmark's CPU
+ * baseline runs this restatement on P threads instead. With P > 1 the step loop keeps full
+ * lists (each pair evaluated from both ends, as the reference's GPU path does) so that rows are
+ * independent and the threads share no writes; each thread takes contiguous ranges of the
+ * Hilbert-sorted particles. Results differ from the serial run only in summation order. */
+static int g_threads = 1;
+
+void oracle_set_threads(int n)
+{
+    g_threads = n > 1 ? n : 1;
+}
+
+int oracle_threads(void)
+{
+    return g_threads;
+}
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -526,6 +548,7 @@ unsigned int on_build_binned(const ob_box* b, size_t N, const double* pos3, cons
         cmax = ovf;
     }
     unsigned int max_seen = 0;
+#pragma omp parallel for schedule(dynamic, 512) reduction(max : max_seen) if (g_threads > 1) num_threads(g_threads)
     for (size_t i = 0; i < N; i++)
     {
         unsigned int cnt = 0;
@@ -608,6 +631,8 @@ int on_distance_check(const ob_box* b, size_t N, const double* pos3, const doubl
     double rmax = old_rmin + r_buff;
     double delta_max = (rmax * 1.0 - old_rmin) / 2.0;
     double maxsq = delta_max > 0 ? delta_max * delta_max : 0;
+    int moved = 0;
+#pragma omp parallel for reduction(| : moved) if (g_threads > 1) num_threads(g_threads)
     for (size_t i = 0; i < N; i++)
     {
         double dx[3];
@@ -615,9 +640,9 @@ int on_distance_check(const ob_box* b, size_t N, const double* pos3, const doubl
             dx[d] = pos3[3 * i + d] - 1.0 * last3[3 * i + d];
         ob_min_image(b, dx);
         if (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2] >= maxsq)
-            return 1;
+            moved = 1;
     }
-    return 0;
+    return moved;
 }
 
 /* =========================================================================== forces */
@@ -629,6 +654,71 @@ typedef int (*pair_eval_fn)(double rsq, const void* ctx, double* force_divr, dou
  * as a callback: per particle, force sum, half pair energy and half virial (PotentialPair.h:
  * 739-772); with a half list (third_law) the partner gets the opposite force and the other
  * halves. */
+/* Particle i's row: its force, half energies and half virials, and with a half list
+ * (third_law) the partners' opposite forces and other halves, added into force4 / virial6. */
+static inline void pair_row(const ob_box* b, size_t N, size_t i, const double* pos3,
+                            const unsigned int* nlist, const unsigned int* n_neigh,
+                            unsigned int nmax, int third_law, int compute_virial,
+                            pair_eval_fn eval, const void* ctx, double* force4, double* virial6)
+{
+    const double* pi = &pos3[3 * i];
+    double fi[3] = {0, 0, 0}, pei = 0.0;
+    double vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+    for (unsigned int k = 0; k < n_neigh[i]; k++)
+    {
+        unsigned int j = nlist[i * nmax + k];
+        double dx[3] = {pi[0] - pos3[3 * j], pi[1] - pos3[3 * j + 1], pi[2] - pos3[3 * j + 2]};
+        ob_min_image(b, dx);
+        double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        double force_divr = 0.0, pair_eng = 0.0;
+        if (!eval(rsq, ctx, &force_divr, &pair_eng))
+            continue;
+        double f2 = force_divr * 0.5;
+        fi[0] += dx[0] * force_divr;
+        fi[1] += dx[1] * force_divr;
+        fi[2] += dx[2] * force_divr;
+        pei += pair_eng * 0.5;
+        if (compute_virial)
+        {
+            vxx += f2 * dx[0] * dx[0];
+            vxy += f2 * dx[0] * dx[1];
+            vxz += f2 * dx[0] * dx[2];
+            vyy += f2 * dx[1] * dx[1];
+            vyz += f2 * dx[1] * dx[2];
+            vzz += f2 * dx[2] * dx[2];
+        }
+        if (third_law)
+        {
+            force4[4 * j + 0] -= dx[0] * force_divr;
+            force4[4 * j + 1] -= dx[1] * force_divr;
+            force4[4 * j + 2] -= dx[2] * force_divr;
+            force4[4 * j + 3] += pair_eng * 0.5;
+            if (compute_virial)
+            {
+                virial6[0 * N + j] += f2 * dx[0] * dx[0];
+                virial6[1 * N + j] += f2 * dx[0] * dx[1];
+                virial6[2 * N + j] += f2 * dx[0] * dx[2];
+                virial6[3 * N + j] += f2 * dx[1] * dx[1];
+                virial6[4 * N + j] += f2 * dx[1] * dx[2];
+                virial6[5 * N + j] += f2 * dx[2] * dx[2];
+            }
+        }
+    }
+    force4[4 * i + 0] += fi[0];
+    force4[4 * i + 1] += fi[1];
+    force4[4 * i + 2] += fi[2];
+    force4[4 * i + 3] += pei;
+    if (compute_virial)
+    {
+        virial6[0 * N + i] += vxx;
+        virial6[1 * N + i] += vxy;
+        virial6[2 * N + i] += vxz;
+        virial6[3 * N + i] += vyy;
+        virial6[4 * N + i] += vyz;
+        virial6[5 * N + i] += vzz;
+    }
+}
+
 static void pair_accumulate(const ob_box* b, size_t N, const double* pos3,
                             const unsigned int* nlist, const unsigned int* n_neigh,
                             unsigned int nmax, int third_law, int compute_virial,
@@ -637,65 +727,11 @@ static void pair_accumulate(const ob_box* b, size_t N, const double* pos3,
     memset(force4, 0, sizeof(double) * 4 * N);
     if (virial6)
         memset(virial6, 0, sizeof(double) * 6 * N);
+    /* rows of a full list are independent: threads take contiguous ranges of them */
+#pragma omp parallel for schedule(dynamic, 512) if (g_threads > 1 && !third_law) num_threads(g_threads)
     for (size_t i = 0; i < N; i++)
-    {
-        const double* pi = &pos3[3 * i];
-        double fi[3] = {0, 0, 0}, pei = 0.0;
-        double vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
-        for (unsigned int k = 0; k < n_neigh[i]; k++)
-        {
-            unsigned int j = nlist[i * nmax + k];
-            double dx[3] = {pi[0] - pos3[3 * j], pi[1] - pos3[3 * j + 1], pi[2] - pos3[3 * j + 2]};
-            ob_min_image(b, dx);
-            double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-            double force_divr = 0.0, pair_eng = 0.0;
-            if (!eval(rsq, ctx, &force_divr, &pair_eng))
-                continue;
-            double f2 = force_divr * 0.5;
-            fi[0] += dx[0] * force_divr;
-            fi[1] += dx[1] * force_divr;
-            fi[2] += dx[2] * force_divr;
-            pei += pair_eng * 0.5;
-            if (compute_virial)
-            {
-                vxx += f2 * dx[0] * dx[0];
-                vxy += f2 * dx[0] * dx[1];
-                vxz += f2 * dx[0] * dx[2];
-                vyy += f2 * dx[1] * dx[1];
-                vyz += f2 * dx[1] * dx[2];
-                vzz += f2 * dx[2] * dx[2];
-            }
-            if (third_law)
-            {
-                force4[4 * j + 0] -= dx[0] * force_divr;
-                force4[4 * j + 1] -= dx[1] * force_divr;
-                force4[4 * j + 2] -= dx[2] * force_divr;
-                force4[4 * j + 3] += pair_eng * 0.5;
-                if (compute_virial)
-                {
-                    virial6[0 * N + j] += f2 * dx[0] * dx[0];
-                    virial6[1 * N + j] += f2 * dx[0] * dx[1];
-                    virial6[2 * N + j] += f2 * dx[0] * dx[2];
-                    virial6[3 * N + j] += f2 * dx[1] * dx[1];
-                    virial6[4 * N + j] += f2 * dx[1] * dx[2];
-                    virial6[5 * N + j] += f2 * dx[2] * dx[2];
-                }
-            }
-        }
-        force4[4 * i + 0] += fi[0];
-        force4[4 * i + 1] += fi[1];
-        force4[4 * i + 2] += fi[2];
-        force4[4 * i + 3] += pei;
-        if (compute_virial)
-        {
-            virial6[0 * N + i] += vxx;
-            virial6[1 * N + i] += vxy;
-            virial6[2 * N + i] += vxz;
-            virial6[3 * N + i] += vyy;
-            virial6[4 * N + i] += vyz;
-            virial6[5 * N + i] += vzz;
-        }
-    }
+        pair_row(b, N, i, pos3, nlist, n_neigh, nmax, third_law, compute_virial, eval, ctx,
+                 force4, virial6);
 }
 
 typedef struct
@@ -817,6 +853,7 @@ void oi_nvt_step_one(const ob_box* b, size_t N, double* pos3, double* vel3, cons
                      int* image3, double dt, double alpha)
 {
     /* TwoStepConstantVolume::integrateStepOne (hoomd/md/TwoStepConstantVolume.cc:7-80) */
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads)
     for (size_t j = 0; j < N; j++)
     {
         for (int d = 0; d < 3; d++)
@@ -828,6 +865,7 @@ void oi_nvt_step_one(const ob_box* b, size_t N, double* pos3, double* vel3, cons
             vel3[3 * j + d] = v;
         }
     }
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads)
     for (size_t j = 0; j < N; j++)
         ob_wrap(b, &pos3[3 * j], &image3[3 * j]);
 }
@@ -836,6 +874,7 @@ void oi_nvt_step_two(size_t N, double* vel3, double* accel3, const double* mass,
                      const double* net_force4, double dt, double alpha)
 {
     /* TwoStepConstantVolume::integrateStepTwo (TwoStepConstantVolume.cc:213-260) */
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads)
     for (size_t j = 0; j < N; j++)
     {
         double minv = 1.0 / mass[j];
@@ -882,6 +921,7 @@ double oi_kinetic_energy(size_t N, const double* vel3, const double* mass)
 {
     /* ComputeThermo::computeProperties, no pressure tensor (hoomd/md/ComputeThermo.cc:133-149) */
     double ke = 0.0;
+#pragma omp parallel for reduction(+ : ke) if (g_threads > 1) num_threads(g_threads)
     for (size_t j = 0; j < N; j++)
         ke += mass[j]
               * (vel3[3 * j] * vel3[3 * j] + vel3[3 * j + 1] * vel3[3 * j + 1]
@@ -1283,7 +1323,8 @@ static void os_nlist_compute(os_system* s, uint64_t timestep)
     os_build_exclusions(s);
     for (;;)
     {
-        unsigned int need = on_build_binned(&s->box, s->N, s->pos, NULL, 1, &s->rcut, s->r_buff, 0,
+        unsigned int need = on_build_binned(&s->box, s->N, s->pos, NULL, 1, &s->rcut, s->r_buff,
+                                            g_threads > 1,
                                             s->n_excl, s->excl, s->max_excl, s->nmax, s->nlist,
                                             s->n_neigh);
         if (need <= s->nmax)
@@ -1301,8 +1342,8 @@ static int os_compute_net_force(os_system* s, uint64_t timestep)
 {
     size_t N = s->N;
     os_nlist_compute(s, timestep);
-    op_pair_lj(&s->box, N, s->pos, s->nlist, s->n_neigh, s->nmax, 1, s->sigma, s->epsilon, s->rcut,
-               0.0, s->shift_mode, 1, s->pair_f, s->pair_v);
+    op_pair_lj(&s->box, N, s->pos, s->nlist, s->n_neigh, s->nmax, g_threads > 1 ? 0 : 1, s->sigma,
+               s->epsilon, s->rcut, 0.0, s->shift_mode, 1, s->pair_f, s->pair_v);
     memcpy(s->net_f, s->pair_f, sizeof(double) * 4 * N);
     if (s->nb)
     {

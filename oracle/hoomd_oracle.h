@@ -188,6 +188,11 @@ void os_get_forces(const os_system* s, double* pair_force4, double* pair_virial6
 uint64_t os_timestep(const os_system* s);
 uint64_t os_n_builds(const os_system* s);
 /* NeighborList rebuild_check_delay / check_dist (NeighborList.cc:1313-1418); defaults 1, 1 */
+/* Host threads of the step loop (default 1; the tests keep the serial summation order). With
+ * n > 1 the list build, pair forces (per-thread third-law buffers) and integrator loops run on
+ * n OpenMP threads: the benchmark's CPU baseline at P = host cores. */
+void oracle_set_threads(int n);
+int oracle_threads(void);
 void os_set_nlist_policy(os_system* s, unsigned int rebuild_check_delay, int check_dist);
 uint64_t os_n_dangerous(const os_system* s);
 unsigned int os_shortest_rebuild(const os_system* s);

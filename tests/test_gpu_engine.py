@@ -334,8 +334,10 @@ def test_pair_kernel_kinds_after_rebuilds(hoomd, kind, n):
 
 def test_autotuner_sweeps_and_keeps_results(hoomd):
     """The autotuner times every launch configuration at the first run and keeps the fastest;
-    the trajectory does not depend on the configuration chosen."""
+    the trajectory does not depend on the configuration chosen. A second system of the same
+    kind takes the choice from the process's cache without a sweep."""
     from hoomd_amd import _lib
+    _lib.lib().hm_tune_cache_clear()
     pos, vel, L = lj_state(8)
     out = []
     for tuned in (True, False):
@@ -361,6 +363,12 @@ def test_autotuner_sweeps_and_keeps_results(hoomd):
         out.append(sim.state.get_snapshot().particles.position.copy())
     # different launch shapes change only the FP32 summation order of the forces
     assert np.abs(out[0] - out[1]).max() < 1e-6
+    # the list build was swept too, and a second system of the same kind is answered by the cache
+    nl = np.zeros(3, np.int32)
+    sim, lj = make_sim(hoomd, pos, vel, L)
+    sim.run(0)
+    rows = _lib.lib().hm_sys_nlist_launch(sim._system, nl.ctypes.data, None, 0)
+    assert rows == 0 and nl[2] == 1 and sim.operations.is_tuning_complete
 
 
 @pytest.mark.parametrize("delay,dist,kT", [(3, False, 1.2), (0, False, 1.2), (4, True, 40.0),

@@ -220,7 +220,7 @@ def test_lj_reference_layout_tilted_box(H):
     ii, jj, dd = ii[order], jj[order], dd[order]
     nn = np.bincount(ii, minlength=N).astype(np.uint32)
     nl = np.zeros((N, int(nn.max())), np.uint32)
-    start = np.concatenate([[0], np.cumsum(nn)[:-1]])
+    start = np.concatenate([[0], np.cumsum(nn.astype(np.int64))[:-1]])
     for k in range(N):
         nl[k, :nn[k]] = jj[start[k]:start[k] + nn[k]]
     types = np.zeros(N, np.int64)
