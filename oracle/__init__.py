@@ -87,6 +87,8 @@ def lib():
             "os_shortest_rebuild": (C.c_uint, [_p]),
             "os_kinetic_energy": (_d, [_p]),
             "os_potential_energy": (_d, [_p]),
+            "nvp_run": (C.c_longlong, [_i, _sz, _d, _p, _p, _d, _d, _d, C.c_uint64, C.c_uint64,
+                                       _i, _p, _p, _p, _p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -506,3 +508,27 @@ class System:
 
     def potential_energy(self):
         return lib().os_potential_energy(self.h)
+
+
+NVE_VARIANTS = {"fp64": 0, "fp64_state_fp32_pairs": 1, "fp32": 2}
+
+
+def nve_precision(variant, pos, vel, L, steps, every, r_cut=2.5, r_buff=0.4, dt=0.005,
+                  threads=1):
+    """NVE LJ run of the precision study (nve_precision.c) in one variant of NVE_VARIANTS.
+    Returns (kinetic, shifted potential) energy samples at steps 0, every, ..., the final
+    positions and velocities, and the number of list builds."""
+    pos = _f64(pos)
+    vel = _f64(vel)
+    n = len(pos)
+    ns = steps // every + 1
+    ke = np.zeros(ns)
+    pe = np.zeros(ns)
+    po = np.zeros((n, 3))
+    vo = np.zeros((n, 3))
+    builds = lib().nvp_run(NVE_VARIANTS[variant], n, float(L), _ptr(pos), _ptr(vel), r_cut,
+                           r_buff, dt, steps, every, threads, _ptr(ke), _ptr(pe), _ptr(po),
+                           _ptr(vo))
+    if builds < 0:
+        raise ValueError("nvp_run: bad arguments or allocation failure")
+    return ke, pe, po, vo, int(builds)

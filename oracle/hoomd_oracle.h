@@ -23,11 +23,16 @@
  *                                 time reversibility
  *   orng_philox ................ pinned: Random123 published known-answer vectors
  *   orng_seed / counter / u01 .. pinned: random_numbers_test.cc known answers
- *   orng_normal / gamma ........ PARITY UNPINNED as streams: only their moments are checked
- *                                 (the reference's RandomNumbers.h does not build here and its
- *                                 tests hold no normal/gamma vectors); oi_bussi_alpha and
- *                                 oi_thermalize inherit this
+ *   orng_normal / gamma ........ PARITY UNPINNED as streams: only their distributions are
+ *                                 checked (moments, and Kolmogorov-Smirnov tests of the
+ *                                 thermalize normals, Bussi's K' against Gamma(g/2, kT) and the
+ *                                 Langevin uniforms); the reference's RandomNumbers.h needs the
+ *                                 absent Random123 headers and its tests hold no normal/gamma
+ *                                 vectors; oi_bussi_alpha and oi_thermalize inherit this
  *   os_hilbert_* ............... pinned: permutation/adjacency invariants of the Hilbert curve
+ *   nvp_run (nve_precision.c) .. pinned: its FP64 variant reproduces os_run's NVE trajectory
+ *                                 (tests/test_oracle_nve_precision.py); the FP32 variants
+ *                                 differ from it in the declared types only
  */
 #ifndef HOOMD_ORACLE_H
 #define HOOMD_ORACLE_H

@@ -57,6 +57,40 @@ def test_normal_and_gamma_moments():
     assert abs(var / (g / 2 * kT**2) - 1.0) < 0.1  # Gamma(g/2, kT) variance
 
 
+def test_stream_distributions_ks():
+    """Kolmogorov-Smirnov checks of the streams whose exact values cannot be pinned here (the
+    reference's RandomNumbers.h needs the Random123 headers, an un-vendored submodule, and its
+    tests hold no normal or gamma vectors). A distribution test cannot catch a change that
+    keeps the distribution (sin for cos in Box-Muller, a swapped word pair); it does catch a
+    wrong scale, shape, a lost draw or a biased branch.
+
+    * thermalize: each axis N(0, sqrt(kT/m)) over 3 x 40,000 particles, m = 1 and m = 2.5;
+      axes uncorrelated;
+    * Bussi at tau = 0: K' = alpha^2 K is Gamma(g/2, scale kT) for g = 2 (the shape-1/2 gamma
+      path with its boost), 5, 30 and 2,997 degrees of freedom, 20,000 timesteps each;
+    * Langevin noise: uniform on [-1, 1] per axis."""
+    from scipy import stats
+    N = 40000
+    for m, kT in ((1.0, 1.2), (2.5, 0.7)):
+        v = O.thermalize(np.full(N, m), np.arange(N), kT, 3, 21)
+        sd = math.sqrt(kT / m)
+        for d in range(3):
+            assert stats.kstest(v[:, d], "norm", args=(0.0, sd)).pvalue > 1e-3
+        c = np.corrcoef(v.T)
+        assert np.abs(c[np.triu_indices(3, 1)]).max() < 4 / math.sqrt(N)
+    for g, kT in ((2.0, 1.0), (5.0, 1.3), (30.0, 0.8), (2997.0, 1.2)):
+        K = 0.5 * g * kT * 1.7  # any current kinetic energy: K' does not depend on it at tau 0
+        ks = np.array([O.bussi_alpha(K, g, 0.005, 0.0, kT, t, 7) ** 2 * K for t in range(20000)])
+        assert stats.kstest(ks, "gamma", args=(g / 2, 0.0, kT)).pvalue > 1e-3, g
+    vel = np.zeros((N, 3))
+    f4 = np.zeros((N, 4))
+    gamma, dt = 1.0, 0.01
+    _, a = O.langevin_step_two(vel, np.ones(N), np.arange(N), f4, gamma, 1.0, dt, 4, 9)
+    r = a / math.sqrt(6 * gamma * 1.0 / dt)  # the uniform draws, F = r sqrt(6 gamma kT / dt)
+    for d in range(3):
+        assert stats.kstest(r[:, d], "uniform", args=(-1.0, 2.0)).pvalue > 1e-3
+
+
 # ------------------------------------------------------------------ evaluators
 def test_lj_golden(golden):
     """forces_and_energies.json 'LJ' (Mathematica), rtol 5e-6 as test_potential.py:994."""
