@@ -159,6 +159,28 @@ def host_cpu():
     return model, max(1, min(16, n))
 
 
+def parity_block(sim, lj, L):
+    """Per-quantity error of the engine's current forces, energies and virials against the
+    oracle's FP64 re-evaluation on the same state (full binned list, no third law), the
+    liquid the timed steps left. Test infrastructure: the oracle is the checker here."""
+    import oracle as O
+    from gpu_helpers import rel_rms
+    t0 = time.perf_counter()
+    O.set_threads(host_cpu()[1])
+    pos = np.asarray(sim.state.get_snapshot().particles.position, np.float64)
+    f, e, v = lj.forces, lj.energies, lj.virials
+    nl, nn = O.nlist_binned(O.Box(L), pos, 2.5, 0.4, full=True)
+    rf, rv = O.pair_lj(O.Box(L), pos, nl, nn, third_law=False)
+    out = {"state": "after the timed steps", "reference": "oracle FP64 re-evaluation",
+           "force_rel_rms": rel_rms(f, rf[:, :3]), "energy_rel_rms": rel_rms(e, rf[:, 3]),
+           "virial_rel_rms": rel_rms(np.asarray(v).T, rv), "bar": 1e-5,
+           "pairs_within_r_list": int(nn.sum() // 2)}
+    log(f"[bench] parity vs FP64: force {out['force_rel_rms']:.3e}, energy "
+        f"{out['energy_rel_rms']:.3e}, virial {out['virial_rel_rms']:.3e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    return out
+
+
 def cpu_baseline(n_cells, rho, seed, steps):
     """Oracle (FP64 C restatement of the reference CPU path: Bussi, binned lists) on the host's
     cores (full lists on P threads; the serial form keeps the half list with the third law), on
@@ -267,59 +289,32 @@ def howto_cpu(snap, seed, seconds):
     return steps / dt, dt, steps
 
 
-def kg_melt(nchain=20000, length=50, rho=0.85):
-    """Kremer-Grest melt without overlaps: beads on a simple-cubic lattice at density rho, chains
-    of `length` consecutive sites along z rows (rows of n sites hold n / length chains), so
-    every bond starts at the lattice spacing (1.056 at rho 0.85) and no two beads overlap."""
-    N = nchain * length
-    n = int(round(N ** (1 / 3)))
-    if n ** 3 != N or n % length:
-        raise ValueError("nchain * length must be a cube whose edge is a multiple of length")
-    a = rho ** (-1 / 3)
-    L = n * a
-    g = np.arange(n)
-    x, y, z = np.meshgrid(g, g, g, indexing="ij")  # x varies slowest: index = (x n + y) n + z
-    # chains run along z (consecutive indices), length beads each
-    pos = (np.stack([x, y, z], -1).reshape(-1, 3) + 0.5) * a - L / 2
-    idx = np.arange(N).reshape(-1, length)
-    bonds = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1).astype(np.uint32)
-    return pos, bonds, L
-
-
-def kg(steps, warmup, seed, cpu_steps):
-    """Kremer-Grest melt production step: 20 000 chains x 50 beads at rho 0.85, FENE (k 30,
-    r0 1.5) + WCA bonds, WCA pairs (r_cut 2^(1/6), mode shift, bond exclusions, buffer 0.4),
-    Langevin kT 1, gamma 1, dt 0.01. Prints one JSON line."""
+def kg(steps, warmup, seed, cpu_steps, pushoff=10000):
+    """Kremer-Grest melt production step: 20 000 chains x 50 beads at rho 0.85, random-walk
+    chains (bond 0.97) pushed apart by a soft cosine ramped over `pushoff` steps, then FENE
+    (k 30, r0 1.5) + WCA bonds, WCA pairs (r_cut 2^(1/6), mode shift, bond exclusions, buffer
+    0.4), Langevin kT 1, gamma 1, dt 0.01. Prints one JSON line."""
     import hoomd_amd as hoomd
     from hoomd_amd import _lib
-    pos, bonds, L = kg_melt()
-    N = len(pos)
-    snap = hoomd.Snapshot()
-    snap.configuration.box = [L, L, L, 0, 0, 0]
-    snap.particles.N = N
-    snap.particles.types = ["A"]
-    snap.particles.position[:] = pos
-    snap.bonds.N = len(bonds)
-    snap.bonds.types = ["A-A"]
-    snap.bonds.group[:] = bonds
-    sim = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
-    sim.create_state_from_snapshot(snap)
-    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
-    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=("bond",))
-    wca = hoomd.md.pair.LJ(nlist=nl, default_r_cut=2 ** (1 / 6), mode="shift")
-    wca.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0)
-    fene = hoomd.md.bond.FENEWCA()
-    fene.params["A-A"] = dict(k=30.0, r0=1.5, epsilon=1.0, sigma=1.0, delta=0.0)
-    lang = hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.0, default_gamma=1.0)
-    sim.operations.integrator = hoomd.md.Integrator(dt=0.01, forces=[wca, fene], methods=[lang])
+    from kg_workload import pushed_off_melt
+    t0 = time.perf_counter()
+    sim, L = pushed_off_melt(hoomd, 20000, 50, pushoff_steps=pushoff, seed=seed, log=log)
+    _lib.check(_lib.lib().hm_synchronize())
+    log(f"[bench] push-off {pushoff} steps in {time.perf_counter() - t0:.1f} s")
+    snap0 = sim.state.get_snapshot()
+    N = snap0.particles.N
+    bonds = np.array(snap0.bonds.group, np.uint32)
+    nl = sim.operations.integrator.forces[0].nlist
     log(f"[bench] KG melt N={N}, {len(bonds)} bonds: warmup {warmup} steps")
     sim.run(warmup)
     _lib.check(_lib.lib().hm_synchronize())
     _lib.check(_lib.lib().hm_sys_prime(sim._system, steps), "prime graphs")
+    builds0 = nl.num_builds
     t0 = time.perf_counter()
     sim.run(steps)
     _lib.check(_lib.lib().hm_synchronize())
     elapsed = time.perf_counter() - t0
+    builds = nl.num_builds - builds0
     tps = steps / elapsed
     cpu = None
     if cpu_steps:
@@ -343,11 +338,12 @@ def kg(steps, warmup, seed, cpu_steps):
         "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
         "particle_steps_per_s": tps * N,
         "dtype": "f32 pair/bond arithmetic, f64 positions/integration",
-        "data": "synthetic (lattice chains, thermalized)",
+        "data": f"synthetic (random-walk chains, bond 0.97, soft-cosine push-off {pushoff} "
+                "steps, thermalized)",
         "config": {"workload": f"KG melt {N // 50} x 50 beads, rho 0.85, FENE k 30 r0 1.5 + WCA, "
                                "WCA pairs shift, bond exclusions, buffer 0.4, Langevin kT 1 "
                                "gamma 1, dt 0.01", "N": N, "bonds": int(len(bonds)),
-                   "nlist_builds_in_timed_steps": nl.num_builds},
+                   "nlist_builds_in_timed_steps": builds},
         "cpu_baseline": cpu}), flush=True)
     return 0
 
@@ -433,7 +429,8 @@ def replicas(nrep, steps, warmup, seed, cpu_seconds):
 MLP_SITE_FLOP = 2 * (16 * 64 + 64 * 64) + 2 * 64 + 2 * (64 * 64 + 64 * 16)  # forward + dE/dG
 
 
-def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pushoff):
+def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pushoff, lr=2e-3,
+         frames_per_step=1, ibi_iters=12, cg_dt=0.02):
     """Learned CG of the Kremer-Grest melt, end to end on the GPU, against the plain run:
 
     1. fine-grained melt: n_chains x 50 random-walk chains (bond 0.97, rho 0.85) pushed off with
@@ -445,9 +442,12 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
        force - CG bond force (device);
     4. the MLP site energy (16 Gaussian densities, 16 -> 64 -> 64 -> 1 SiLU, bf16 MFMA) trained
        by force matching with the project's Adam kernels, train_steps steps over the frames;
-    5. the CG system (sites of frame 0, CG bonds, the MLP; Langevin kT 1, dt 0.02, site mass 5,
+    5. the CG system (sites of frame 0, CG bonds, the MLP; Langevin kT 1, dt cg_dt, site mass 5,
        gamma 5) timed, then sampled for RDF, bond-length distribution and pressure against the
-       mapped frames / the plain run.
+       mapped frames / the plain run;
+    6. structure refinement: ibi_iters iterative Boltzmann inversion steps on the pair prior
+       (a Table out to the MLP cut-off) with the network in place, then the same measurements;
+       the comparison model without the network gets the same refinement from the same prior.
 
     Prints one JSON line: value = fine-grained-equivalent simulated time per wall-second of the
     CG run over that of the plain run (tau_FG advanced per second)."""
@@ -514,11 +514,12 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=seed)
     harm = hoomd.md.bond.Harmonic()
     harm.params["cg"] = dict(k=kb, r0=r0)
-    prior = hoomd.md.pair.Table(nlist=nl, default_r_cut=rc_prior)
+    prior_params = MC.regrid_table(prior_params, rc_prior, mlp.r_cut, width=600)
+    prior = hoomd.md.pair.Table(nlist=nl, default_r_cut=mlp.r_cut)
     prior.params[("S", "S")] = prior_params
     lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
     lang.gamma["S"] = float(per)
-    cg.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp, harm, prior],
+    cg.operations.integrator = hoomd.md.Integrator(dt=cg_dt, forces=[mlp, harm, prior],
                                                    methods=[lang])
     cg.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
     fs.subtract_pair(cg, prior)
@@ -530,9 +531,9 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     mlp.params = zero
     base = float(np.mean([mlp.loss_grad(fs, int(k))[0] for k in held[:10]]))
     mlp.params = MC.init_params(seed)
-    order = np.random.default_rng(seed).integers(0, ntrain, size=train_steps)
+    order = np.random.default_rng(seed).integers(0, ntrain, size=train_steps * frames_per_step)
     t1 = time.perf_counter()
-    loss = mlp.train(fs, train_steps, lr=2e-3, order=order)
+    loss = mlp.train(fs, train_steps, lr=lr, order=order, frames_per_step=frames_per_step)
     _lib.check(_lib.lib().hm_synchronize())
     t_train = time.perf_counter() - t1
     trained = float(np.mean([mlp.loss_grad(fs, int(k))[0] for k in held[:10]]))
@@ -550,7 +551,24 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     tps_cg = steps / (time.perf_counter() - t0)
     log(f"[mlcg] CG run: {tps_cg:.1f} steps/s at {smap.nsites} sites")
     # structure of the CG run, with the network and without it (bonds + prior)
-    def structure(nsamp=20, every=250):
+    def ibi(params, iters, relax=1000, frames=2, every=200):
+        errs = []
+        for _ in range(iters):
+            cg.run(relax)
+            gs = []
+            for _ in range(frames):
+                cg.run(every)
+                pc = cg.state.get_snapshot().particles.position
+                gs.append(MC.rdf(pc, L, rdf_edges_max, nb, exclude_bonded=cgb)[1])
+            g_it = np.mean(gs, axis=0)
+            errs.append(float(np.abs(g_it - g_map).sum() / g_map.sum()))
+            params = MC.ibi_update(params, mlp.r_cut, r_bins, g_it, g_map, kT=1.0)
+            prior.params[("S", "S")] = params
+        return params, errs
+
+    kT_cg = []
+
+    def structure(nsamp=10, every=250):
         gs, ps, rs = [], [], []
         for k in range(nsamp):
             cg.run(every)
@@ -558,13 +576,22 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
             gs.append(MC.rdf(pc, L, rdf_edges_max, nb, exclude_bonded=cgb)[1])
             rs.append(MC.bond_lengths(pc, L, cgb))
             ps.append(thermo.pressure)
+            kT_cg.append(thermo.kinetic_temperature)
         return np.mean(gs, axis=0), np.mean(ps), np.concatenate(rs)
 
     g_cg, p_cg_mean, r_cg = structure()
+    t_ibi = time.perf_counter()
+    _, ibi_errs = ibi(prior_params, ibi_iters)
+    g_ibi, p_ibi, r_ibi = structure()
+    log(f"[mlcg] IBI {ibi_iters} steps with the network ({time.perf_counter() - t_ibi:.1f} s): "
+        f"RDF L1 {' '.join(f'{e:.3f}' for e in ibi_errs)}")
     trained_params = mlp.params
     mlp.params = np.zeros(MC.NPARAM, np.float32)
+    prior.params[("S", "S")] = prior_params
     cg.run(2000)
     g_none, p_none, r_none = structure()
+    _, ibi_errs_none = ibi(prior_params, ibi_iters)
+    g_none_ibi, p_none_ibi, r_none_ibi = structure()
     mlp.params = trained_params
     dr = rdf_edges_max / nb
     rdf_l1 = float(np.abs(g_cg - g_map).sum() / g_map.sum())
@@ -574,13 +601,20 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     hm_, _ = np.histogram(r_map, hb_e)
     hc_, _ = np.histogram(r_cg, hb_e)
     bond_tv = float(0.5 * np.abs(hm_ / hm_.sum() - hc_ / hc_.sum()).sum())
-    hn_, _ = np.histogram(r_none, hb_e)
-    bond_tv_none = float(0.5 * np.abs(hm_ / hm_.sum() - hn_ / hn_.sum()).sum())
+    def tv(rs):
+        h, _ = np.histogram(np.clip(rs, hb_e[0], hb_e[-1]), hb_e)
+        return float(0.5 * np.abs(hm_ / hm_.sum() - h / h.sum()).sum())
+
+    def l1(g):
+        return float(np.abs(g - g_map).sum() / g_map.sum())
+
+    bond_tv_none = tv(r_none)
+    p_fg_mean = float(np.mean(p_fg))
     ms3 = (Cc.c_double * 3)()
     _lib.check(_lib.lib().hm_sys_time_mlp(cg._system, mlp._force_id, 20, ms3), "time mlp")
     site_flops = MLP_SITE_FLOP * smap.nsites
     mfma = site_flops / (ms3[1] * 1e-3) / 2.5e15
-    speedup = (tps_cg * 0.02) / (tps_fg * 0.01)
+    speedup = (tps_cg * cg_dt) / (tps_fg * 0.01)
     log(f"[mlcg] speed-up {speedup:.2f}x, RDF L1 error {rdf_l1:.4f}, bond TV {bond_tv:.4f}")
     print(json.dumps({
         "metric": "fine-grained-equivalent simulated time per wall-second, MLP-CG / plain KG",
@@ -588,13 +622,15 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
         "higher_is_better": True,
         "plain": {"timesteps_per_s": tps_fg, "particles": N, "dt": 0.01,
                   "tau_per_s": tps_fg * 0.01, "pressure": float(np.mean(p_fg))},
-        "cg": {"timesteps_per_s": tps_cg, "sites": smap.nsites, "dt": 0.02,
-               "tau_per_s": tps_cg * 0.02, "pressure": float(p_cg_mean),
+        "cg": {"timesteps_per_s": tps_cg, "sites": smap.nsites, "dt": cg_dt,
+               "tau_per_s": tps_cg * cg_dt, "pressure": float(p_cg_mean),
+               "kinetic_temperature": float(np.mean(kT_cg)),
                "kernel_ms": {"densities": ms3[0], "site_mlp": ms3[1], "forces": ms3[2]},
                "mfma": {"site_flop": MLP_SITE_FLOP, "achieved_tflops": site_flops /
                         (ms3[1] * 1e-3) / 1e12, "peak_tflops_bf16": 2500.0,
                         "utilisation": mfma}},
         "training": {"frames": nframes, "frame_every": frame_every, "steps": train_steps,
+                     "lr": lr, "frames_per_step": frames_per_step,
                      "seconds": t_train, "loss_first50": float(loss[:50].mean()),
                      "loss_last50": float(loss[-50:].mean()),
                      "heldout_residual_bonds_only": base, "heldout_residual_trained": trained},
@@ -603,10 +639,20 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
                    "pressure_rel": float(abs(p_cg_mean - np.mean(p_fg)) / abs(np.mean(p_fg)))},
         "without_network": {"rdf_rel_l1": rdf_l1_none, "bond_length_tv": bond_tv_none,
                             "pressure": float(p_none)},
+        "ibi": {"iterations": ibi_iters, "relax_steps": 1000, "frames_per_iteration": 2,
+                "with_network": {"rdf_rel_l1": l1(g_ibi), "bond_length_tv": tv(r_ibi),
+                                 "pressure_rel": float(abs(p_ibi - p_fg_mean) / abs(p_fg_mean)),
+                                 "rdf_l1_per_iteration": ibi_errs},
+                "without_network": {"rdf_rel_l1": l1(g_none_ibi),
+                                    "bond_length_tv": tv(r_none_ibi),
+                                    "pressure_rel": float(abs(p_none_ibi - p_fg_mean)
+                                                          / abs(p_fg_mean)),
+                                    "rdf_l1_per_iteration": ibi_errs_none}},
         "prior": {"r_cut": rc_prior},
         "rdf": {"r": [float((i + 0.5) * dr) for i in range(nb)],
                 "mapped": g_map.tolist(), "cg": g_cg.tolist(), "cg_without_network":
-                g_none.tolist()},
+                g_none.tolist(), "cg_ibi": g_ibi.tolist(),
+                "cg_without_network_ibi": g_none_ibi.tolist()},
         "dtype": "bf16 MFMA (fp32 accumulate) MLP, f32 densities/forces, f64 integration",
         "data": "synthetic (seeded random-walk melt, pushed off)",
         "config": {"workload": f"KG melt {n_chains} x {chain_len} -> 5:1 CG "
@@ -624,6 +670,8 @@ def main():
     ap.add_argument("--n", type=int, default=1048576, help="particles (4 * n_cells^3)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the FP64 re-evaluation of the final state (LJ workload)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--pair-only", type=int, default=0,
                     help="profiling mode: after warmup launch only the pair kernel this many "
@@ -648,6 +696,10 @@ def main():
     ap.add_argument("--frame-every", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--pushoff", type=int, default=10000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--frames-per-step", type=int, default=1)
+    ap.add_argument("--ibi-iters", type=int, default=12)
+    ap.add_argument("--cg-dt", type=float, default=0.02)
     ap.add_argument("--replicas", type=int, default=0, metavar="R",
                     help="run R batched uneven LJ replicas instead (256 for the batched workload)")
     a = ap.parse_args()
@@ -656,7 +708,7 @@ def main():
             print("bench.py runs on exactly one GPU", file=sys.stderr)
             return 2
         return mlcg(a.mlcg, a.steps, a.warmup, a.seed, a.frames, a.frame_every, a.train_steps,
-                    a.pushoff)
+                    a.pushoff, a.lr, a.frames_per_step, a.ibi_iters, a.cg_dt)
     if a.replicas:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
@@ -666,7 +718,7 @@ def main():
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
             return 2
-        return kg(a.steps, a.warmup, a.seed, 0 if a.no_cpu else a.cpu_steps)
+        return kg(a.steps, a.warmup, a.seed, 0 if a.no_cpu else a.cpu_steps, a.pushoff)
     if a.howto:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
@@ -754,6 +806,7 @@ def main():
         log(f"[bench] list-build tuner shape={int(r[0])} grid={int(r[1])} {r[2] * 1e3:.1f} us")
     launch["nlist_shape"], launch["nlist_grid"] = int(nl_out[0]), int(nl_out[1])
 
+    parity = None if a.no_parity else parity_block(sim, lj, L)
     cpu = None
     if not a.no_cpu:
         model, P = host_cpu()
@@ -792,6 +845,7 @@ def main():
                      "avg_launch_us": ms * 1e3, "algorithmic_bytes": B,
                      "nlist_entries": int(entries)},
         "roofline_nlist_pair": nlp,
+        "parity": parity,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

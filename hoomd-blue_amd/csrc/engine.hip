@@ -100,7 +100,13 @@ struct ForceSlot
     float* mE = nullptr;
     float4 *mT = nullptr, *mErr = nullptr, *mC = nullptr;
     float *mLoss = nullptr, *mGpart = nullptr, *mM1 = nullptr, *mM2 = nullptr, *mGrad = nullptr;
+    float* mGacc = nullptr;  // minibatch gradient sum
     double* mLossOut = nullptr;
+    // optional tabulated pair term of the force pass: U then F, mtab_w points on
+    // [mtab_rmin, r_cut)
+    float* mtab = nullptr;
+    unsigned int mtab_w = 0;
+    double mtab_rmin = 0.0;
     unsigned int adam_t = 0;
 };
 bool is_pair(const ForceSlot& f)
@@ -284,6 +290,8 @@ struct hm_system
             dfree(f.mC);
             dfree(f.mLoss);
             dfree(f.mGpart);
+            dfree(f.mGacc);
+            dfree(f.mtab);
             dfree(f.mM1);
             dfree(f.mM2);
             dfree(f.mGrad);
@@ -781,6 +789,9 @@ struct hm_system
         a.loss_partial = f.mLoss;
         a.gpart = f.mGpart;
         a.loss_scale = (float)(2.0 / (3.0 * (double)N));
+        a.ptab = f.mtab;
+        a.ptab_rmin = (float)f.mtab_rmin;
+        a.ptab_width = f.mtab ? f.mtab_w : 0;
         return a;
     }
 
@@ -2386,6 +2397,47 @@ int hm_sys_mlp_set_params(hm_system* s, int force_id, const float* params)
     });
 }
 
+int hm_sys_mlp_set_pair_table(hm_system* s, int force_id, unsigned int width, double r_min,
+                              const double* U, const double* F)
+{
+    ForceSlot* f = mlp_slot(s, force_id);
+    if (!f || (width && (!U || !F || !(r_min >= 0.0) || !(r_min < f->mm.r_cut))))
+        return fail(HM_ERR_ARGS, "hm_sys_mlp_set_pair_table: invalid arguments (0 <= r_min < "
+                                 "r_cut, U and F of width points)");
+    return guarded([&]() {
+        hchk(hipStreamSynchronize(s->stream), "sync");
+        if (width == 0)
+        {
+            dfree(f->mtab);
+            f->mtab = nullptr;
+            f->mtab_w = 0;
+            s->clear_graphs();
+            s->prepared = false;
+            return HM_OK;
+        }
+        std::vector<float> h(2 * (size_t)width);
+        for (unsigned int k = 0; k < width; k++)
+        {
+            h[k] = (float)U[k];
+            h[width + k] = (float)F[k];
+        }
+        // same shape: overwrite in place, so captured step graphs stay valid
+        if (!f->mtab || f->mtab_w != width || f->mtab_rmin != r_min)
+        {
+            dfree(f->mtab);
+            f->mtab = nullptr;
+            f->mtab = dalloc<float>(2 * (size_t)width);
+            f->mtab_w = width;
+            f->mtab_rmin = r_min;
+            s->clear_graphs();
+        }
+        hchk(hipMemcpy(f->mtab, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice),
+             "upload");
+        s->prepared = false;
+        return HM_OK;
+    });
+}
+
 int hm_sys_mlp_get_params(hm_system* s, int force_id, float* params)
 {
     ForceSlot* f = mlp_slot(s, force_id);
@@ -2499,34 +2551,55 @@ int hm_sys_mlp_train(hm_system* s, int force_id, const hm_mlp_train_args* t)
         || !(t->lr > 0.0) || !(t->beta1 >= 0.0 && t->beta1 < 1.0)
         || !(t->beta2 >= 0.0 && t->beta2 < 1.0) || !(t->eps > 0.0))
         return fail(HM_ERR_ARGS, "hm_sys_mlp_train: invalid arguments");
+    const unsigned int B = t->frames_per_step ? t->frames_per_step : 1;
     if (t->order)
-        for (unsigned int k = 0; k < t->nsteps; k++)
+        for (size_t k = 0; k < (size_t)t->nsteps * B; k++)
             if (t->order[k] >= t->nframes)
                 return fail(HM_ERR_ARGS, "hm_sys_mlp_train: frame order out of range");
     return guarded([&]() {
         mlp_alloc_training(s, *f);
         dfree(f->mLossOut);
-        f->mLossOut = dalloc<double>(t->nsteps ? t->nsteps : 1);
+        f->mLossOut = dalloc<double>((size_t)t->nsteps * B + 1);
+        if (B > 1 && !f->mGacc)
+            f->mGacc = dalloc<float>(MLP_NPARAM);
         const size_t stride = (size_t)3 * s->N;
         for (unsigned int k = 0; k < t->nsteps; k++)
         {
-            const unsigned int fr = t->order ? t->order[k] : k % t->nframes;
-            mlp_frame_pass(s, *f, t->frame_pos + stride * fr, t->frame_force + stride * fr,
-                           f->mLossOut + k);
-            hchk(launch_mlp_adam(f->mparams, f->mM1, f->mM2, f->mGpart, f->mbf16, (float)t->lr,
-                                 (float)t->beta1, (float)t->beta2, (float)t->eps, ++f->adam_t,
-                                 s->stream),
-                 "mlp adam");
+            for (unsigned int b = 0; b < B; b++)
+            {
+                const size_t q = (size_t)k * B + b;
+                const unsigned int fr = t->order ? t->order[q] : (unsigned int)(q % t->nframes);
+                mlp_frame_pass(s, *f, t->frame_pos + stride * fr, t->frame_force + stride * fr,
+                               f->mLossOut + q);
+                if (B > 1)
+                    hchk(launch_mlp_grad_accum(f->mGpart, f->mGacc, b == 0, s->stream),
+                         "mlp gradient sum");
+            }
+            if (B > 1)
+                hchk(launch_mlp_adam_acc(f->mparams, f->mM1, f->mM2, f->mGacc, 1.0f / (float)B,
+                                         f->mbf16, (float)t->lr, (float)t->beta1,
+                                         (float)t->beta2, (float)t->eps, ++f->adam_t, s->stream),
+                     "mlp adam");
+            else
+                hchk(launch_mlp_adam(f->mparams, f->mM1, f->mM2, f->mGpart, f->mbf16,
+                                     (float)t->lr, (float)t->beta1, (float)t->beta2,
+                                     (float)t->eps, ++f->adam_t, s->stream),
+                     "mlp adam");
         }
         hchk(hipStreamSynchronize(s->stream), "sync");
         if (t->loss && t->nsteps)
         {
-            std::vector<double> h(t->nsteps);
-            hchk(hipMemcpy(h.data(), f->mLossOut, sizeof(double) * t->nsteps,
+            std::vector<double> h((size_t)t->nsteps * B);
+            hchk(hipMemcpy(h.data(), f->mLossOut, sizeof(double) * h.size(),
                            hipMemcpyDeviceToHost),
                  "download");
             for (unsigned int k = 0; k < t->nsteps; k++)
-                t->loss[k] = h[k] / (3.0 * (double)s->N);
+            {
+                double sum = 0.0;
+                for (unsigned int b = 0; b < B; b++)
+                    sum += h[(size_t)k * B + b];
+                t->loss[k] = sum / B / (3.0 * (double)s->N);
+            }
         }
         s->prepared = false;
         return HM_OK;

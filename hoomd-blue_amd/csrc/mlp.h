@@ -6,6 +6,7 @@
 //   site    E_i = MLP(x_i), x_i = (G(i) - gmean) gscale  16 -> 64 -> 64 -> 1 SiLU on bf16 MFMA
 //           A(i) = dE_i/dG(i)                           the analytic backward of the MLP
 //   pass 2  F_i = -sum_j sum_k (A_k(i) + A_k(j)) phi_k'(r_ij) r_ij / |r_ij|
+//           (+ an optional tabulated pair term of the same pairs, e.g. a structural prior)
 // Training adds the adjoint pass C(i) = -gscale sum_j phi'(r_ij) (e_i - e_j) . r_ij / |r_ij|
 // (e = dLoss/dF) and the per-site parameter gradient of C(i) . dE_i/dx, whose matrix products
 // also run on bf16 MFMA.
@@ -61,6 +62,11 @@ struct MlpArgs
     float* Esite;                  // [N] site energies
     float4* force;                 // [N] (fx, fy, fz, site energy)
     float* virial;                 // [6N] rows of pitch N, or NULL
+    // optional tabulated pair term on [ptab_rmin, rcut) evaluated in the force pass like
+    // EvaluatorPairTable (U then F, ptab_width points each); ptab_width 0: none
+    const float* ptab;
+    float ptab_rmin;
+    unsigned int ptab_width;
     // training
     const float4* target;          // [N] target forces (xyz)
     float4* err;                   // [N] dLoss/dF
@@ -86,6 +92,11 @@ hipError_t launch_mlp_loss(const float* loss_partial, unsigned int n, double* ou
 hipError_t launch_mlp_adam(float* master, float* m1, float* m2, const float* gpart,
                            uint16_t* wb, float lr, float beta1, float beta2, float eps,
                            unsigned int t, hipStream_t s);
+// minibatches: gacc (+)= the sum of the gpart rows; Adam step t on gacc * scale
+hipError_t launch_mlp_grad_accum(const float* gpart, float* gacc, bool first, hipStream_t s);
+hipError_t launch_mlp_adam_acc(float* master, float* m1, float* m2, const float* gacc,
+                               float scale, uint16_t* wb, float lr, float beta1, float beta2,
+                               float eps, unsigned int t, hipStream_t s);
 // gradient only: sum of the gpart rows into grad (FP32)
 hipError_t launch_mlp_grad_sum(const float* gpart, float* grad, hipStream_t s);
 // bf16 copies wb from master

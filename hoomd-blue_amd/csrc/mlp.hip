@@ -10,6 +10,7 @@
 // so a layer's result, rounded to bf16, is already the next layer's B operand: no shuffles
 // between layers. Weights (and their transposes, for the backward) are staged in LDS as bf16.
 #include "mlp.h"
+#include "evaluators.h"
 
 #include <cmath>
 
@@ -314,8 +315,10 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
     const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
     const bool valid = i < a.N;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float fx = 0.f, fy = 0.f, fz = 0.f, ep = 0.f;
     float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const EvaluatorPairTable<float>::param_type tp{a.ptab, a.ptab + a.ptab_width, a.ptab_rmin,
+                                                   a.ptab_width};
     if (valid)
     {
         float ai[MLP_NK];
@@ -360,7 +363,17 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
                 const float gk = __expf(-dd * dd * a.d.inv2w2);
                 sdot += (ai[k] + aj[k]) * gk * (dfc - fc * dd * a.d.invw2);
             }
-            const float fdr = -sdot / r;
+            float fdr = -sdot / r;
+            if (a.ptab_width)
+            {
+                float tf = 0.f, te = 0.f;
+                if (EvaluatorPairTable<float>(rsq, a.d.rcutsq, tp).evalForceAndEnergy(tf, te,
+                                                                                      false))
+                {
+                    fdr += tf;
+                    ep += 0.5f * te;
+                }
+            }
             fx += fdr * dx;
             fy += fdr * dy;
             fz += fdr * dz;
@@ -379,6 +392,8 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
     fx = group_sum<PAIR_TPP>(fx);
     fy = group_sum<PAIR_TPP>(fy);
     fz = group_sum<PAIR_TPP>(fz);
+    if (a.ptab_width)
+        ep = group_sum<PAIR_TPP>(ep);
     if (a.virial)
 #pragma unroll
         for (int m = 0; m < 6; m++)
@@ -386,7 +401,7 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
     float sq = 0.f;
     if (valid && sub == 0)
     {
-        a.force[i] = make_float4(fx, fy, fz, a.Esite[i]);
+        a.force[i] = make_float4(fx, fy, fz, a.Esite[i] + ep);
         if (a.virial)
             for (int m = 0; m < 6; m++)
                 a.virial[(size_t)m * a.N + i] = v[m];
@@ -735,6 +750,39 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(float* master, float* m1,
     write_bf16_copy(wb, p, v);
 }
 
+// minibatch: gacc += the sum of one frame's gradient rows (gacc cleared first when first)
+__global__ __launch_bounds__(256) void mlp_grad_accum_kernel(const float* gpart, float* gacc,
+                                                             int first)
+{
+    const unsigned int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (unsigned int)MLP_NPARAM)
+        return;
+    float g = first ? 0.f : gacc[p];
+    for (unsigned int w = 0; w < MLP_GRAD_BLOCKS; w++)
+        g += gpart[(size_t)w * MLP_NPARAM + p];
+    gacc[p] = g;
+}
+
+// Adam on a summed gradient, scaled (1 / frames of the minibatch)
+__global__ __launch_bounds__(256) void mlp_adam_acc_kernel(float* master, float* m1, float* m2,
+                                                           const float* gacc, float scale,
+                                                           uint16_t* wb, float lr, float beta1,
+                                                           float beta2, float eps, float c1,
+                                                           float c2)
+{
+    const unsigned int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (unsigned int)MLP_NPARAM)
+        return;
+    const float gsum = gacc[p] * scale;
+    const float mm = beta1 * m1[p] + (1.f - beta1) * gsum;
+    const float vv = beta2 * m2[p] + (1.f - beta2) * gsum * gsum;
+    m1[p] = mm;
+    m2[p] = vv;
+    const float v = master[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    master[p] = v;
+    write_bf16_copy(wb, p, v);
+}
+
 __global__ __launch_bounds__(256) void mlp_grad_sum_kernel(const float* gpart, float* grad)
 {
     const unsigned int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -932,6 +980,24 @@ hipError_t launch_mlp_adam(float* master, float* m1, float* m2, const float* gpa
     const float c2 = (float)(1.0 - std::pow((double)beta2, (double)t));
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, master,
                        m1, m2, gpart, wb, lr, beta1, beta2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad_accum(const float* gpart, float* gacc, bool first, hipStream_t s)
+{
+    hipLaunchKernelGGL(mlp_grad_accum_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s,
+                       gpart, gacc, first ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam_acc(float* master, float* m1, float* m2, const float* gacc,
+                               float scale, uint16_t* wb, float lr, float beta1, float beta2,
+                               float eps, unsigned int t, hipStream_t s)
+{
+    const float c1 = (float)(1.0 - std::pow((double)beta1, (double)t));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, (double)t));
+    hipLaunchKernelGGL(mlp_adam_acc_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s,
+                       master, m1, m2, gacc, scale, wb, lr, beta1, beta2, eps, c1, c2);
     return hipGetLastError();
 }
 

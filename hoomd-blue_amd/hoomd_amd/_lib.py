@@ -95,7 +95,7 @@ class HmMlpArgs(C.Structure):
 class HmMlpTrainArgs(C.Structure):
     _fields_ = [("nframes", _u), ("frame_pos", _p), ("frame_force", _p), ("nsteps", _u),
                 ("order", _p), ("lr", _d), ("beta1", _d), ("beta2", _d), ("eps", _d),
-                ("loss", _p)]
+                ("loss", _p), ("frames_per_step", _u)]
 
 
 class HmBondArgs(C.Structure):
@@ -179,6 +179,7 @@ SIGNATURES = {
     "hm_sys_mlp_set_params": (_i, [_p, _i, _p]),
     "hm_sys_mlp_get_params": (_i, [_p, _i, _p]),
     "hm_sys_mlp_set_model": (_i, [_p, _i, C.POINTER(HmMlpModel)]),
+    "hm_sys_mlp_set_pair_table": (_i, [_p, _i, _u, _d, _p, _p]),
     "hm_sys_mlp_descriptors": (_i, [_p, _i, _p, _p]),
     "hm_sys_mlp_train": (_i, [_p, _i, C.POINTER(HmMlpTrainArgs)]),
     "hm_sys_mlp_loss_grad": (_i, [_p, _i, _p, _p, C.POINTER(_d), _p]),

@@ -74,6 +74,7 @@ class SiteEnergyMLP(Force):
             raise ValueError(f"params must hold {NPARAM} values")
         self._gmean = np.zeros(NK)
         self._gscale = np.ones(NK)
+        self._table = None
 
     def _model(self):
         m = _lib.HmMlpModel()
@@ -96,6 +97,41 @@ class SiteEnergyMLP(Force):
         self._force_id = fid
         self._simulation = sim
         self.nlist._simulation = sim
+        if self._table is not None:
+            self._push_table()
+
+    def _push_table(self):
+        t = self._table
+        if t is None:
+            r = _lib.lib().hm_sys_mlp_set_pair_table(self._simulation._system, self._force_id, 0,
+                                                     0.0, None, None)
+        else:
+            r = _lib.lib().hm_sys_mlp_set_pair_table(
+                self._simulation._system, self._force_id, len(t["U"]), float(t["r_min"]),
+                t["U"].ctypes.data, t["F"].ctypes.data)
+        _lib.check(r, "mlcg.SiteEnergyMLP.pair_table")
+
+    @property
+    def pair_table(self):
+        """Optional tabulated pair term (dict r_min, U, F on [r_min, r_cut), the form of
+        md.pair.Table params) evaluated in the site energy's own force pass over the same pairs:
+        a structural prior at no extra pass. None: no pair term."""
+        if self._table is None:
+            return None
+        return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self._table.items()}
+
+    @pair_table.setter
+    def pair_table(self, value):
+        if value is None:
+            self._table = None
+        else:
+            U = np.ascontiguousarray(value["U"], np.float64).ravel().copy()
+            F = np.ascontiguousarray(value["F"], np.float64).ravel().copy()
+            if U.size == 0 or U.size != F.size:
+                raise ValueError("pair_table: U and F need the same, nonzero length")
+            self._table = dict(r_min=float(value.get("r_min", 0.0)), U=U, F=F)
+        if self._attached():
+            self._push_table()
 
     # ------------------------------------------------------------------ parameters
     @property
@@ -147,11 +183,12 @@ class SiteEnergyMLP(Force):
 
     # ------------------------------------------------------------------ training
     def train(self, frames, steps, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, order=None,
-              target=None):
+              target=None, frames_per_step=1):
         """Force matching on device frames: Adam steps over frames (in ``order``, default
-        cycling), each on loss = sum_i |F_i - T_i|^2 / (3N). ``target`` is a FrameSet of
-        target forces when they differ from ``frames`` (e.g. mapped minus bond forces).
-        Returns the loss of every step, taken before its update."""
+        cycling), each on loss = sum_i |F_i - T_i|^2 / (3N), the gradient averaged over
+        ``frames_per_step`` frames (``order`` then lists steps * frames_per_step frames).
+        ``target`` is a FrameSet of target forces when they differ from ``frames`` (e.g. mapped
+        minus bond forces). Returns the loss of every step, taken before its update."""
         sim = self._simulation
         sim._attach()
         t = target if target is not None else frames
@@ -160,11 +197,12 @@ class SiteEnergyMLP(Force):
         a.frame_pos = frames.d_pos.ptr
         a.frame_force = t.d_force.ptr
         a.nsteps = int(steps)
+        a.frames_per_step = max(1, int(frames_per_step))
         ordr = None
         if order is not None:
             ordr = np.ascontiguousarray(order, np.uint32)
-            if ordr.size < steps:
-                raise ValueError("order needs one frame per step")
+            if ordr.size < steps * a.frames_per_step:
+                raise ValueError("order needs frames_per_step frames per step")
             a.order = ordr.ctypes.data
         a.lr, a.beta1, a.beta2, a.eps = float(lr), float(betas[0]), float(betas[1]), float(eps)
         loss = np.zeros(max(int(steps), 1))
@@ -333,9 +371,10 @@ def rdf(pos, L, r_max, nbins=60, exclude_bonded=None):
     x[x >= L] = 0.0
     tree = cKDTree(x, boxsize=L)
     edges = np.linspace(0.0, r_max, nbins + 1)
-    cum = tree.count_neighbors(tree, edges).astype(np.float64)
-    cum -= N  # self pairs at distance 0
-    shell = np.diff(cum)
+    pairs = tree.query_pairs(r_max, output_type="ndarray")
+    d = _wrap(x[pairs[:, 0]] - x[pairs[:, 1]], L)
+    h, _ = np.histogram(np.sqrt((d * d).sum(axis=1)), bins=edges)
+    shell = 2.0 * h.astype(np.float64)  # ordered pairs
     if exclude_bonded is not None and len(exclude_bonded):
         rb = bond_lengths(pos, L, np.asarray(exclude_bonded))
         hb, _ = np.histogram(rb, bins=edges)
@@ -375,6 +414,39 @@ def rdf_prior_table(r, g, kT=1.0, g_join=0.9, width=400, min_count_g=0.02):
     Ut = np.interp(rr, r_ext, U_ext)
     Ft = -np.gradient(Ut, rr)
     return dict(r_min=0.0, U=Ut, F=Ft), rc
+
+
+def regrid_table(params, r_cut, r_cut_new, width=None):
+    """Table params (r_min 0 on [0, r_cut)) carried onto [0, r_cut_new) with U = 0 beyond the
+    old cut-off: the same forces, on the grid an iterative refinement can reshape out to
+    r_cut_new."""
+    U = np.asarray(params["U"], np.float64)
+    w = len(U) if width is None else int(width)
+    r_old = np.linspace(0.0, r_cut, len(U), endpoint=False)
+    r = np.linspace(0.0, r_cut_new, w, endpoint=False)
+    Un = np.interp(r, np.concatenate([r_old, [r_cut]]), np.concatenate([U, [0.0]]), right=0.0)
+    return dict(r_min=0.0, U=Un, F=-np.gradient(Un, r))
+
+
+def ibi_update(params, r_cut, r, g_cur, g_target, kT=1.0, damping=0.3, g_min=0.02, taper=0.3):
+    """One iterative Boltzmann inversion step on a Table (Reith, Puetz and Mueller-Plathe,
+    J. Comput. Chem. 24, 1624 (2003)): U <- U + damping kT ln(g_cur / g_target), the
+    correction taken on the RDF bins r where both RDFs are at least g_min, smoothed over three
+    bins, held at its first sampled value below them, and brought to zero at r_cut by a cosine
+    taper over the last `taper` of the range (so U(r_cut) stays 0). F = -dU/dr."""
+    U = np.asarray(params["U"], np.float64)
+    rt = np.linspace(0.0, r_cut, len(U), endpoint=False)
+    r = np.asarray(r, np.float64)
+    ok = (np.asarray(g_cur) >= g_min) & (np.asarray(g_target) >= g_min) & (r < r_cut)
+    if ok.sum() < 3:
+        raise ValueError("ibi_update: too few sampled bins")
+    d = damping * kT * np.log(np.asarray(g_cur)[ok] / np.asarray(g_target)[ok])
+    d = np.convolve(np.pad(d, 1, mode="edge"), np.ones(3) / 3.0, mode="valid")
+    corr = np.interp(rt, r[ok], d)  # held constant outside the sampled bins
+    t = np.clip((r_cut - rt) / taper, 0.0, 1.0)
+    corr *= 0.5 * (1.0 - np.cos(np.pi * t))
+    Un = U + corr
+    return dict(r_min=0.0, U=Un, F=-np.gradient(Un, rt))
 
 
 def soft_cosine_table(A, r_cut, width=200):

@@ -525,13 +525,20 @@ int hm_sys_mlp_set_params(hm_system* s, int force_id, const float* params);
 int hm_sys_mlp_get_params(hm_system* s, int force_id, float* params);
 /* Replace the normalisation and basis constants (r_cut cannot change). */
 int hm_sys_mlp_set_model(hm_system* s, int force_id, const hm_mlp_cg_model* model);
+/* A tabulated pair term evaluated in the site energy's force pass over the same pairs, as
+ * md.pair.Table would (EvaluatorPairTable.h:152-196: U and F = -dU/dr on width points r_min +
+ * k (r_cut - r_min) / width, linear interpolation), so a structural prior costs no pass of its
+ * own. width 0 removes it. Same width and r_min: overwritten in place (captured step graphs
+ * stay valid). Forces, energies, virials and the training loss include it. */
+int hm_sys_mlp_set_pair_table(hm_system* s, int force_id, unsigned int width, double r_min,
+                              const double* U, const double* F);
 /* Densities G by tag [16N] (host) of the current state, or, when frame_pos is not NULL, of
  * the positions frame_pos (device [N][3] FP32 by tag), which then replace the state's. */
 int hm_sys_mlp_descriptors(hm_system* s, int force_id, const float* frame_pos, float* G);
 
 /* Force-matching training on device-resident frames: step k loads frame order[k] (or
  * k mod nframes), builds the list, evaluates the forces, the loss sum |F - T|^2 / (3N) and
- * its gradient, and takes one Adam step. */
+ * its gradient, and takes one Adam step (on the mean gradient of frames_per_step frames). */
 typedef struct hm_mlp_train_args
 {
     unsigned int nframes;
@@ -541,6 +548,8 @@ typedef struct hm_mlp_train_args
     const unsigned int* order;     /* host [nsteps] frame of each step, or NULL */
     double lr, beta1, beta2, eps;
     double* loss;                  /* host [nsteps] loss of each step before its update, or NULL */
+    unsigned int frames_per_step;  /* frames whose gradients one Adam step averages (0 or 1:
+                                    * one); order then holds nsteps * frames_per_step frames */
 } hm_mlp_train_args;
 int hm_sys_mlp_train(hm_system* s, int force_id, const hm_mlp_train_args* args);
 /* Loss and gradient (host [HM_MLP_NPARAM], may be NULL) of one frame (device pointers as in

@@ -150,6 +150,84 @@ def test_training_teacher_student(hoomd):
     # the bf16-rounding oracle (~1e-4 of F) weigh ~2e-3 of the loss at most
     assert abs(l_gpu - l_or) <= 2e-3 * l_or
     assert rel_rms(g_gpu, g_or) < 3e-3
+    # a minibatch step: its loss is the mean of its frames' losses, and its Adam step (the
+    # first after a parameter reset: -lr g / (|g| + eps) per parameter) uses their mean gradient
+    p0 = mlp.params
+    ls, gs = zip(*[mlp.loss_grad(fs, k) for k in (0, 3, 5)])
+    mlp.params = p0  # resets the Adam moments
+    lb = mlp.train(fs, 1, lr=1e-3, eps=1e-8, order=[0, 3, 5], frames_per_step=3)
+    assert abs(lb[0] - np.mean(ls)) <= 1e-6 * np.mean(ls)
+    g = np.mean(gs, axis=0)
+    step = mlp.params - p0
+    big = np.abs(g) > 1e-4 * np.abs(g).max()
+    expect = -1e-3 * g / (np.abs(g) + 1e-8)
+    assert np.abs(step[big] - expect[big]).max() < 2e-6
+
+
+def _soft_table(r_min=0.5, r_cut=3.0, width=300):
+    r = r_min + np.arange(width) * (r_cut - r_min) / width
+    U = 2.0 * (1.0 - r / r_cut) ** 2 - 0.3 * np.exp(-((r - 1.6) / 0.3) ** 2)
+    F = 4.0 / r_cut * (1.0 - r / r_cut) - 0.3 * 2.0 * (r - 1.6) / 0.09 * np.exp(
+        -((r - 1.6) / 0.3) ** 2)
+    return dict(r_min=r_min, U=U, F=F)
+
+
+def test_pair_table_in_force_pass(hoomd):
+    """A tabulated pair term inside the site energy's force pass (SiteEnergyMLP.pair_table) gives
+    the forces, energies and virials of the same site energy plus a separate md.pair.Table with
+    the same table (1e-5 relative RMS: the same FP32 evaluation, summed in another order). An
+    in-place update of the table (same width) takes effect without new graphs, and the training
+    loss includes the term: with a zero network and targets equal to the table's forces, the
+    loss vanishes."""
+    from hoomd_amd.md import mlcg
+    print("pair table in the force pass: N=2500", flush=True)
+    pos, L = cg_config(2500, seed=17)
+    p = M.init_params(5).astype(np.float32)
+    tab = _soft_table()
+    sim_a, mlp_a, _ = cg_sim(hoomd, pos, L, params=p)
+    mlp_a.pair_table = tab
+    sim_b, mlp_b, _ = cg_sim(hoomd, pos, L, params=p)
+    table_b = hoomd.md.pair.Table(nlist=mlp_b.nlist, default_r_cut=3.0)
+    table_b.params[("S", "S")] = tab
+    lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
+    sim_b.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp_b, table_b],
+                                                      methods=[lang])
+    sim_a.run(0)
+    sim_b.run(0)
+    fb = mlp_b.forces + table_b.forces
+    eb = mlp_b.energies + table_b.energies
+    vb = mlp_b.virials + table_b.virials
+    ef, ee = rel_rms(mlp_a.forces, fb), rel_rms(mlp_a.energies, eb)
+    ev = rel_rms(mlp_a.virials, vb)
+    print(f"fused vs separate: force {ef:.2e}, energy {ee:.2e}, virial {ev:.2e}", flush=True)
+    assert ef < 1e-5 and ee < 1e-5 and ev < 1e-5
+    # in place, after step graphs were captured: doubling U and F adds the table's forces
+    # once more (the forces without the table taken at the same positions)
+    sim_a.run(10)
+    f1 = mlp_a.forces.copy()
+    mlp_a.pair_table = dict(r_min=tab["r_min"], U=2 * tab["U"], F=2 * tab["F"])
+    sim_a.run(0)
+    f2 = mlp_a.forces.copy()
+    mlp_a.pair_table = None
+    sim_a.run(0)
+    f0 = mlp_a.forces.copy()
+    assert rel_rms(f2 - f1, f1 - f0) < 1e-5
+    sim_a.run(10)
+    assert np.all(np.isfinite(mlp_a.forces))
+    # training loss includes the term
+    mlp_a.params = np.zeros(mlcg.NPARAM, np.float32)
+    mlp_a.pair_table = tab
+    sim_a.run(0)
+    x = sim_a.state.get_snapshot().particles.position
+    ft = mlp_a.forces  # the table alone: the zero network has no force
+    fs = mlcg.FrameSet(1, len(pos))
+    fs.set(0, positions=x, forces=ft)
+    loss, _ = mlp_a.loss_grad(fs, 0)
+    mlp_a.pair_table = None
+    loss_none, _ = mlp_a.loss_grad(fs, 0)
+    print(f"loss with the term {loss:.3e}, without {loss_none:.3e}", flush=True)
+    # (the frame holds FP32 positions: the target differs by that rounding only)
+    assert loss < 1e-6 * loss_none and loss_none > 0
 
 
 def test_kg_to_cg_network_gains(hoomd):
