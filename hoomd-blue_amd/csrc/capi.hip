@@ -1,6 +1,7 @@
 // capi.hip -- extern "C" kernel drivers declared in include/hoomd_amd.h.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "capi_util.h"
@@ -26,6 +27,40 @@ int check(hipError_t e, const char* what)
     return HM_ERR_LAUNCH;
 }
 } // namespace hm
+
+namespace
+{
+// Partial sums of hm_thermo_noalloc: one row of THERMO_SUMS doubles per partial-sum workgroup
+// (THERMO_BLOCKS rows, 60 KiB), a copy per device, held until the process ends. Host code uses
+// only the per-device addresses taken with hipGetSymbolAddress below, never this name.
+constexpr size_t THERMO_SCRATCH = (size_t)THERMO_BLOCKS * THERMO_SUMS;
+static __device__ __align__(256) double scratch[THERMO_SCRATCH];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Address of the current device's scratch array, looked up once per device (with that device
+// current, as every call here runs).
+int scratch_for_current_device(double** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return fail(HM_ERR_LAUNCH, "hm_thermo_noalloc: no usable current device");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev])
+    {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return check(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                         "hm_thermo_noalloc: scratch address");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return HM_OK;
+}
+} // namespace
 
 extern "C"
 {
@@ -419,6 +454,23 @@ int hm_thermo(unsigned int N, const double* vel, const float* const* forces,
                   "hm_thermo");
     (void)hipFreeAsync(partial, s);
     return r;
+}
+
+int hm_thermo_noalloc(unsigned int N, const double* vel, const float* const* forces,
+                      const float* const* virials, unsigned int nforces, double* out,
+                      hm_stream_t stream)
+{
+    if (!vel || !out || nforces > 4 || (nforces && !forces))
+        return fail(HM_ERR_ARGS, "hm_thermo_noalloc: invalid arguments");
+    // the partial-sum grid is THERMO_BLOCKS workgroups whatever N is, and every workgroup writes
+    // its whole row, so the scratch needs no clearing and no N outgrows it
+    double* partial = nullptr;
+    int r = scratch_for_current_device(&partial);
+    if (r != HM_OK)
+        return r;
+    return check(launch_thermo(N, (const double4*)vel, (const float4* const*)forces, virials,
+                               nforces, partial, out, (hipStream_t)stream),
+                 "hm_thermo_noalloc");
 }
 
 double hm_bussi_alpha_host(double K, double ndof, double dt, double tau, double kT,

@@ -140,6 +140,7 @@ SIGNATURES = {
     "hm_nvt_step_two": (_i, [_u, _p, _p, _p, _u, _d, _d, _p]),
     "hm_langevin_step_two": (_i, [_u, _p, _p, _p, _p, _p, _u, _p, _u, _d, _d, _u64, _u16, _p]),
     "hm_thermo": (_i, [_u, _p, _p, _p, _u, _p, _p]),
+    "hm_thermo_noalloc": (_i, [_u, _p, _p, _p, _u, _p, _p]),
     "hm_bussi_alpha_host": (_d, [_d, _d, _d, _d, _d, _u64, _u16, C.c_uint32]),
     "hm_thermalize": (_i, [_u, _p, _p, _d, _u64, _u16, _p]),
     "hm_sfc_order": (_i, [_u, _p, HmBox, _u, _p, _p]),

@@ -299,6 +299,15 @@ int hm_langevin_step_two(unsigned int N, double* vel, double* accel, const doubl
  * hold 0..4 arrays. Allocates its partial-sum scratch on the stream (hipMallocAsync). */
 int hm_thermo(unsigned int N, const double* vel, const float* const* forces,
               const float* const* virials, unsigned int nforces, double* out, hm_stream_t stream);
+/* hm_thermo without any allocation (no counterpart in the reference; for callers that cannot
+ * allocate: code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as hm_thermo. The partial sums live in a library array of
+ * THERMO_BLOCKS x 15 doubles (60 KiB) per device, held until the process ends; the partial-sum
+ * grid is that fixed size for every N, so no N exceeds it. Every call on one device shares that
+ * device's array: calls that overlap on one device (different streams or threads) race. */
+int hm_thermo_noalloc(unsigned int N, const double* vel, const float* const* forces,
+                      const float* const* virials, unsigned int nforces, double* out,
+                      hm_stream_t stream);
 /* Bussi rescale factor for the translational DOF from the kinetic energy K, drawn on the host
  * thread exactly as BussiThermostat::compute_rescale_factor (hoomd/md/Thermostat.h:357-446). */
 double hm_bussi_alpha_host(double K, double ndof, double dt, double tau, double kT,
