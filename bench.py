@@ -319,6 +319,8 @@ def kg(steps, warmup, seed, cpu_steps, pushoff=10000):
     cpu = None
     if cpu_steps:
         import oracle as O
+        model, P = host_cpu()
+        O.set_threads(P)
         snap = sim.state.get_snapshot()
         s = O.System(snap.particles.position, snap.particles.velocity, L, r_cut=2 ** (1 / 6),
                      mode="shift", buffer=0.4, dt=0.01, seed=seed)
@@ -329,9 +331,10 @@ def kg(steps, warmup, seed, cpu_steps, pushoff=10000):
         t1 = time.perf_counter()
         s.run(cpu_steps)
         cdt = time.perf_counter() - t1
-        cpu = {"value": cpu_steps / cdt, "unit": "timesteps/s", "cores": 1, "kind": "port",
+        cpu = {"value": cpu_steps / cdt, "unit": "timesteps/s", "cores": P, "kind": "port",
+               "cpu_model": model,
                "sample": f"N={N} KG melt state after the timed steps, {cpu_steps} steps, "
-                         f"{cdt:.1f} s on one host core"}
+                         f"{cdt:.1f} s on {P} host threads"}
     print(json.dumps({
         "metric": "timesteps/s (Kremer-Grest melt, Langevin)", "value": tps,
         "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -376,8 +379,13 @@ def replicas(nrep, steps, warmup, seed, cpu_seconds):
     st1 = b.stats()
     tps = steps / elapsed
     rs = b.replica_stats()
-    entries = float(rs[:, 3].sum())
-    ms = b.time_pair(20)
+    # the pair launch slows as particles drift from their cell order between sorts (every 100
+    # steps): sample it at 10 points over one sort period, entries likewise
+    ms, entries = 0.0, 0.0
+    for _ in range(10):
+        b.run(10)
+        ms += b.time_pair(10) / 10
+        entries += float(b.replica_stats()[:, 3].sum()) / 10
     Bp = 4.0 * entries + N * (4 + 32 + 16)
     achieved = Bp / (ms * 1e-3) / 1e9
     T = 2.0 * rs[:, 0] / (3.0 * n - 3.0)
@@ -385,26 +393,31 @@ def replicas(nrep, steps, warmup, seed, cpu_seconds):
     cpu = None
     if cpu_seconds > 0:
         import oracle as O
-        # a bounded sample: the first replicas in seed order, each run until the sample time is
-        # spent, particle-steps/s over the sample
-        done_ps, spent, used = 0.0, 0.0, 0
-        for r in range(nrep):
-            if spent >= cpu_seconds:
-                break
+        from concurrent.futures import ThreadPoolExecutor
+        # a bounded sample: the first 3 P replicas in seed order, one replica per host thread at
+        # a time (independent replicas spread over the cores, as separate CPU runs would be),
+        # each run 20 steps after its first; particle-steps/s over the wall time of the sample
+        model, P = host_cpu()
+        O.set_threads(1)
+        sample = np.arange(min(nrep, 3 * P))
+        systems = []
+        for r in sample[np.argsort(-n[sample], kind="stable")]:  # largest first: less idle tail
             v = O.thermalize(np.ones(n[r]), np.arange(n[r]), kT[r], 0, int(seeds[r]))
             s = O.System(pos[r], v, L[r], dt=0.005, seed=int(seeds[r]))
             s.set_bussi(float(kT[r]), 0.0)
             s.run(1)
-            k = 20
-            t1 = time.perf_counter()
-            s.run(k)
-            dt_ = time.perf_counter() - t1
-            done_ps += k * n[r]
-            spent += dt_
-            used += 1
-        cpu = {"value": done_ps / spent, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"the oracle's FP64 step loop on the first {used} replicas, 20 steps "
-                         f"each after the first, {spent:.1f} s on one host core"}
+            systems.append(s)
+        k = 20
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=P) as ex:
+            list(ex.map(lambda s: s.run(k), systems))
+        spent = time.perf_counter() - t1
+        done_ps = float(k * n[sample].sum())
+        cpu = {"value": done_ps / spent, "unit": "particle-steps/s", "cores": P, "kind": "port",
+               "cpu_model": model,
+               "sample": f"the oracle's FP64 step loop on the first {len(sample)} replicas "
+                         f"(N_total {int(n[sample].sum())}), 20 steps each after the first, one "
+                         f"replica per thread on {P} host threads, {spent:.1f} s"}
     print(json.dumps({
         "metric": "particle-steps/s (batched uneven LJ replicas)", "value": tps * N,
         "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -430,7 +443,7 @@ MLP_SITE_FLOP = 2 * (16 * 64 + 64 * 64) + 2 * 64 + 2 * (64 * 64 + 64 * 16)  # fo
 
 
 def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pushoff, lr=2e-3,
-         frames_per_step=1, ibi_iters=12, cg_dt=0.02):
+         frames_per_step=1, ibi_iters=12, cg_dt=0.04, pressure_match=False):
     """Learned CG of the Kremer-Grest melt, end to end on the GPU, against the plain run:
 
     1. fine-grained melt: n_chains x 50 random-walk chains (bond 0.97, rho 0.85) pushed off with
@@ -514,15 +527,14 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=seed)
     harm = hoomd.md.bond.Harmonic()
     harm.params["cg"] = dict(k=kb, r0=r0)
+    # the pair prior, out to the MLP cut-off, evaluated inside the site energy's force pass
     prior_params = MC.regrid_table(prior_params, rc_prior, mlp.r_cut, width=600)
-    prior = hoomd.md.pair.Table(nlist=nl, default_r_cut=mlp.r_cut)
-    prior.params[("S", "S")] = prior_params
+    mlp.pair_table = prior_params
     lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
     lang.gamma["S"] = float(per)
-    cg.operations.integrator = hoomd.md.Integrator(dt=cg_dt, forces=[mlp, harm, prior],
+    cg.operations.integrator = hoomd.md.Integrator(dt=cg_dt, forces=[mlp, harm],
                                                    methods=[lang])
     cg.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
-    fs.subtract_pair(cg, prior)
     G = np.concatenate([mlp.descriptors(fs, int(k)) for k in sample[:5]])
     mlp.fit_normalization(G)
     ntrain = max(1, int(nframes * 0.9))
@@ -551,20 +563,37 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     tps_cg = steps / (time.perf_counter() - t0)
     log(f"[mlcg] CG run: {tps_cg:.1f} steps/s at {smap.nsites} sites")
     # structure of the CG run, with the network and without it (bonds + prior)
+    p_fg_mean = float(np.mean(p_fg))
+
     def ibi(params, iters, relax=1000, frames=2, every=200):
+        """Iterative Boltzmann inversion of the pair prior against the mapped RDF (with
+        pressure_match, plus the linear pressure correction toward the plain run's pressure),
+        and of the harmonic CG bond
+        against the mapped bond-length mean and variance (r0 moves by the difference of the
+        means, k scales by the square root of the variance ratio)."""
         errs = []
+        k_b, r_b = kb, r0
+        m_map, v_map = float(r_map.mean()), float(r_map.var())
         for _ in range(iters):
             cg.run(relax)
-            gs = []
+            gs, rs, ps = [], [], []
             for _ in range(frames):
                 cg.run(every)
                 pc = cg.state.get_snapshot().particles.position
                 gs.append(MC.rdf(pc, L, rdf_edges_max, nb, exclude_bonded=cgb)[1])
+                rs.append(MC.bond_lengths(pc, L, cgb))
+                ps.append(thermo.pressure)
             g_it = np.mean(gs, axis=0)
+            rb = np.concatenate(rs)
             errs.append(float(np.abs(g_it - g_map).sum() / g_map.sum()))
             params = MC.ibi_update(params, mlp.r_cut, r_bins, g_it, g_map, kT=1.0)
-            prior.params[("S", "S")] = params
-        return params, errs
+            if pressure_match:
+                params = MC.pressure_ramp(params, mlp.r_cut, float(np.mean(ps)) - p_fg_mean)
+            mlp.pair_table = params
+            r_b += m_map - float(rb.mean())
+            k_b *= float(np.sqrt(rb.var() / v_map))
+            harm.params["cg"] = dict(k=k_b, r0=r_b)
+        return params, errs, (k_b, r_b)
 
     kT_cg = []
 
@@ -581,16 +610,24 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
 
     g_cg, p_cg_mean, r_cg = structure()
     t_ibi = time.perf_counter()
-    _, ibi_errs = ibi(prior_params, ibi_iters)
+    ibi_table, ibi_errs, ibi_bond = ibi(prior_params, ibi_iters)
     g_ibi, p_ibi, r_ibi = structure()
     log(f"[mlcg] IBI {ibi_iters} steps with the network ({time.perf_counter() - t_ibi:.1f} s): "
-        f"RDF L1 {' '.join(f'{e:.3f}' for e in ibi_errs)}")
+        f"RDF L1 {' '.join(f'{e:.3f}' for e in ibi_errs)}; bond k {ibi_bond[0]:.3f} r0 "
+        f"{ibi_bond[1]:.3f}")
+    # the refined model's rate (the same kernels; the table is updated in place)
+    _lib.check(_lib.lib().hm_synchronize())
+    t0 = time.perf_counter()
+    cg.run(steps)
+    _lib.check(_lib.lib().hm_synchronize())
+    tps_cg_ibi = steps / (time.perf_counter() - t0)
     trained_params = mlp.params
     mlp.params = np.zeros(MC.NPARAM, np.float32)
-    prior.params[("S", "S")] = prior_params
+    mlp.pair_table = prior_params
+    harm.params["cg"] = dict(k=kb, r0=r0)
     cg.run(2000)
     g_none, p_none, r_none = structure()
-    _, ibi_errs_none = ibi(prior_params, ibi_iters)
+    _, ibi_errs_none, ibi_bond_none = ibi(prior_params, ibi_iters)
     g_none_ibi, p_none_ibi, r_none_ibi = structure()
     mlp.params = trained_params
     dr = rdf_edges_max / nb
@@ -609,21 +646,24 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
         return float(np.abs(g - g_map).sum() / g_map.sum())
 
     bond_tv_none = tv(r_none)
-    p_fg_mean = float(np.mean(p_fg))
     ms3 = (Cc.c_double * 3)()
     _lib.check(_lib.lib().hm_sys_time_mlp(cg._system, mlp._force_id, 20, ms3), "time mlp")
     site_flops = MLP_SITE_FLOP * smap.nsites
     mfma = site_flops / (ms3[1] * 1e-3) / 2.5e15
-    speedup = (tps_cg * cg_dt) / (tps_fg * 0.01)
-    log(f"[mlcg] speed-up {speedup:.2f}x, RDF L1 error {rdf_l1:.4f}, bond TV {bond_tv:.4f}")
+    speedup = (tps_cg_ibi * cg_dt) / (tps_fg * 0.01)
+    log(f"[mlcg] final model: speed-up {speedup:.2f}x, RDF L1 error {l1(g_ibi):.4f}, bond TV "
+        f"{tv(r_ibi):.4f}, pressure {p_ibi:.3f} (plain run {p_fg_mean:.3f}); without the "
+        f"network: RDF {l1(g_none_ibi):.4f}, bond TV {tv(r_none_ibi):.4f}")
     print(json.dumps({
         "metric": "fine-grained-equivalent simulated time per wall-second, MLP-CG / plain KG",
         "value": speedup, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "higher_is_better": True,
         "plain": {"timesteps_per_s": tps_fg, "particles": N, "dt": 0.01,
                   "tau_per_s": tps_fg * 0.01, "pressure": float(np.mean(p_fg))},
-        "cg": {"timesteps_per_s": tps_cg, "sites": smap.nsites, "dt": cg_dt,
-               "tau_per_s": tps_cg * cg_dt, "pressure": float(p_cg_mean),
+        "cg": {"timesteps_per_s": tps_cg_ibi, "sites": smap.nsites, "dt": cg_dt,
+               "tau_per_s": tps_cg_ibi * cg_dt, "pressure": float(p_ibi),
+               "model": "trained site energy + refined pair prior (in the force pass) + "
+                        "refined harmonic bonds",
                "kinetic_temperature": float(np.mean(kT_cg)),
                "kernel_ms": {"densities": ms3[0], "site_mlp": ms3[1], "forces": ms3[2]},
                "mfma": {"site_flop": MLP_SITE_FLOP, "achieved_tflops": site_flops /
@@ -635,19 +675,27 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
                      "loss_last50": float(loss[-50:].mean()),
                      "heldout_residual_bonds_only": base, "heldout_residual_trained": trained},
         "cg_bonds": {"k": kb, "r0": r0},
-        "errors": {"rdf_rel_l1": rdf_l1, "rdf_rel_l2": rdf_l2, "bond_length_tv": bond_tv,
-                   "pressure_rel": float(abs(p_cg_mean - np.mean(p_fg)) / abs(np.mean(p_fg)))},
-        "without_network": {"rdf_rel_l1": rdf_l1_none, "bond_length_tv": bond_tv_none,
-                            "pressure": float(p_none)},
-        "ibi": {"iterations": ibi_iters, "relax_steps": 1000, "frames_per_iteration": 2,
-                "with_network": {"rdf_rel_l1": l1(g_ibi), "bond_length_tv": tv(r_ibi),
-                                 "pressure_rel": float(abs(p_ibi - p_fg_mean) / abs(p_fg_mean)),
-                                 "rdf_l1_per_iteration": ibi_errs},
-                "without_network": {"rdf_rel_l1": l1(g_none_ibi),
-                                    "bond_length_tv": tv(r_none_ibi),
-                                    "pressure_rel": float(abs(p_none_ibi - p_fg_mean)
-                                                          / abs(p_fg_mean)),
-                                    "rdf_l1_per_iteration": ibi_errs_none}},
+        "errors": {"rdf_rel_l1": l1(g_ibi),
+                   "rdf_rel_l2": float(np.sqrt(((g_ibi - g_map) ** 2).sum()
+                                               / (g_map ** 2).sum())),
+                   "bond_length_tv": tv(r_ibi),
+                   "pressure_rel": float(abs(p_ibi - p_fg_mean) / abs(p_fg_mean))},
+        "without_network": {"rdf_rel_l1": l1(g_none_ibi), "bond_length_tv": tv(r_none_ibi),
+                            "pressure_rel": float(abs(p_none_ibi - p_fg_mean)
+                                                  / abs(p_fg_mean)),
+                            "bond_k_r0": list(ibi_bond_none),
+                            "refinement_rdf_l1_per_iteration": ibi_errs_none},
+        "refinement": {"iterations": ibi_iters, "relax_steps": 1000, "frames_per_iteration": 2,
+                       "pressure_match": bool(pressure_match),
+                       "bond_k_r0": list(ibi_bond), "rdf_l1_per_iteration": ibi_errs},
+        "force_matched_only": {
+            "timesteps_per_s": tps_cg, "speedup": (tps_cg * cg_dt) / (tps_fg * 0.01),
+            "with_network": {"rdf_rel_l1": rdf_l1, "rdf_rel_l2": rdf_l2,
+                             "bond_length_tv": bond_tv,
+                             "pressure_rel": float(abs(p_cg_mean - p_fg_mean) / abs(p_fg_mean))},
+            "without_network": {"rdf_rel_l1": rdf_l1_none, "bond_length_tv": bond_tv_none,
+                                "pressure_rel": float(abs(p_none - p_fg_mean)
+                                                      / abs(p_fg_mean))}},
         "prior": {"r_cut": rc_prior},
         "rdf": {"r": [float((i + 0.5) * dr) for i in range(nb)],
                 "mapped": g_map.tolist(), "cg": g_cg.tolist(), "cg_without_network":
@@ -699,7 +747,11 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--frames-per-step", type=int, default=1)
     ap.add_argument("--ibi-iters", type=int, default=12)
-    ap.add_argument("--cg-dt", type=float, default=0.02)
+    ap.add_argument("--cg-dt", type=float, default=0.04,
+                    help="timestep of the CG run (0.02 and 0.04 give the same structure and "
+                         "temperature)")
+    ap.add_argument("--pressure-match", action="store_true",
+                    help="add the linear pressure correction to each refinement step")
     ap.add_argument("--replicas", type=int, default=0, metavar="R",
                     help="run R batched uneven LJ replicas instead (256 for the batched workload)")
     a = ap.parse_args()
@@ -708,7 +760,8 @@ def main():
             print("bench.py runs on exactly one GPU", file=sys.stderr)
             return 2
         return mlcg(a.mlcg, a.steps, a.warmup, a.seed, a.frames, a.frame_every, a.train_steps,
-                    a.pushoff, a.lr, a.frames_per_step, a.ibi_iters, a.cg_dt)
+                    a.pushoff, a.lr, a.frames_per_step, a.ibi_iters, a.cg_dt,
+                    a.pressure_match)
     if a.replicas:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)

@@ -314,17 +314,48 @@ __global__ __launch_bounds__(256) void bcell_rank_kernel(const unsigned int* __r
 }
 
 // -------------------------------------------------------------------------- list build
-// One wavefront per home cell of a rebuilding replica. The candidates (the particles of the
-// unique adjacent cells, in cell order: offsets {-1, 0, 1} per axis where the axis has at least
-// 3 cells, {0, 1} for 2, {0} for 1, as CellList::initializeCellAdj keeps each adjacent cell
-// once) are staged in LDS, NL_CAP at a time; then for each home particle the 64 lanes test 64
-// candidates at once (FP64 minimum image, kept when j != i and r^2 <= r_list^2) and a ballot
-// places the hits contiguously in the particle's row, in candidate order
-// (NeighborListGPUBinned.cu:52-298 with the lanes over candidates instead of over particles).
-constexpr unsigned int NL_CAP = 512;
+// One workgroup of NL_WAVES wavefronts per home cell of a rebuilding replica. The candidates
+// (the particles of the unique adjacent cells, in cell order: offsets {-1, 0, 1} per axis where
+// the axis has at least 3 cells, {0, 1} for 2, {0} for 1, as CellList::initializeCellAdj keeps
+// each adjacent cell once) are staged in LDS, NL_CAP at a time; then each wavefront takes every
+// NL_WAVES-th home particle, its 64 lanes test 64 candidates at once, and a ballot places the hits
+// contiguously in the particle's row, in candidate order (NeighborListGPUBinned.cu:52-298 with
+// the lanes over candidates instead of over particles).
+//
+// With at least 4 cells per axis, candidates and home particles are staged as FP32 offsets from
+// the home-cell centre (each wrapped to its nearest image once, in FP64): every offset is at most
+// 1.5 w per axis (w = L / dim >= r_list), strictly below L / 2 = 2 w or more, and a pair within
+// r_list has one image shorter than L / 2, so the FP32 difference of offsets is the minimum
+// image. Its r^2 is off the FP64 value by at
+// most 8 sqrt(3) u w r_list + 3 u r_list^2 (u = 2^-24: offsets rounded to FP32, one subtraction
+// per axis, three squares and two sums near r^2 = r_list^2); pairs within 32 u (w r_list +
+// r_list^2) of the cut are re-decided from the FP64 positions with the FP64 minimum image, so the
+// rows equal those of the FP64 test. Boxes with fewer than 4 cells per axis take the FP64 test
+// for every pair (with 3, an offset of exactly 1.5 w = L / 2 has two nearest images).
+constexpr unsigned int NL_CAP = 1024;
 constexpr unsigned int NL_HOME_MAX = 256;
+constexpr unsigned int NL_WAVES = 4;
 
-__global__ __launch_bounds__(64) void bnlist_kernel(
+__device__ __forceinline__ bool within_fp64(const double4& xi, const double4& xj, const Rep& R,
+                                            double rlistsq)
+{
+#pragma clang fp contract(off)
+    double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    dz -= R.L * rint(dz * R.Linv);
+    dy -= R.L * rint(dy * R.Linv);
+    dx -= R.L * rint(dx * R.Linv);
+    return dx * dx + dy * dy + dz * dz <= rlistsq;
+}
+
+__device__ __forceinline__ float centre_offset(double x, double centre, const Rep& R)
+{
+#pragma clang fp contract(off)
+    double d = x - centre;
+    d -= R.L * rint(d * R.Linv);
+    return (float)d;
+}
+
+__global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
     const Rep* __restrict__ reps, const unsigned int* __restrict__ cell_rep,
     const RepState* __restrict__ rst, const double4* __restrict__ pos, unsigned int cmax,
     const unsigned int* __restrict__ cell_size, const unsigned int* __restrict__ cell_idx,
@@ -332,8 +363,7 @@ __global__ __launch_bounds__(64) void bnlist_kernel(
     unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond)
 {
 #pragma clang fp contract(off)
-    __shared__ double sx[NL_CAP], sy[NL_CAP], sz[NL_CAP];
-    __shared__ unsigned int sj[NL_CAP];
+    __shared__ float4 sc[NL_CAP];  // offsets from the home-cell centre, candidate index in w
     __shared__ unsigned int acell[27], aoff[28];
     __shared__ unsigned int hcnt[NL_HOME_MAX];
     const unsigned int c = blockIdx.x;
@@ -341,62 +371,79 @@ __global__ __launch_bounds__(64) void bnlist_kernel(
     if (!rst[r].rebuild)
         return;
     const Rep R = reps[r];
-    const unsigned int lane = threadIdx.x;
+    const unsigned int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned int lc = c - R.cell_off;
     const int d = (int)R.dim;
     const int cx = (int)(lc % R.dim), cy = (int)((lc / R.dim) % R.dim), cz = (int)(lc / (R.dim * R.dim));
     const int lo = d >= 3 ? -1 : 0;
     const int span = d >= 3 ? 3 : (d == 2 ? 2 : 1);
     const unsigned int nadj = (unsigned int)(span * span * span);
-    // adjacent cells in (z, y, x) offset order and their sizes, prefix sums by a wave scan
-    unsigned int asz = 0;
-    if (lane < nadj)
+    if (wave == 0)
     {
-        const int ox = lo + (int)(lane % span), oy = lo + (int)((lane / span) % span),
-                  oz = lo + (int)(lane / (span * span));
-        const int x = (cx + ox + d) % d, y = (cy + oy + d) % d, z = (cz + oz + d) % d;
-        const unsigned int ac = R.cell_off + ((unsigned int)z * R.dim + y) * R.dim + x;
-        acell[lane] = ac;
-        asz = min(cell_size[ac], cmax);
+        // adjacent cells in (z, y, x) offset order and their sizes, prefix sums by a wave scan
+        unsigned int asz = 0;
+        if (lane < nadj)
+        {
+            const int ox = lo + (int)(lane % span), oy = lo + (int)((lane / span) % span),
+                      oz = lo + (int)(lane / (span * span));
+            const int x = (cx + ox + d) % d, y = (cy + oy + d) % d, z = (cz + oz + d) % d;
+            const unsigned int ac = R.cell_off + ((unsigned int)z * R.dim + y) * R.dim + x;
+            acell[lane] = ac;
+            asz = min(cell_size[ac], cmax);
+        }
+        unsigned int incl = asz;
+        for (unsigned int o = 1; o < 32; o <<= 1)
+        {
+            const unsigned int t = __shfl_up(incl, o, 64);
+            if (lane >= o)
+                incl += t;
+        }
+        if (lane < nadj)
+            aoff[lane + 1] = incl;
+        if (lane == 0)
+            aoff[0] = 0;
     }
-    unsigned int incl = asz;
-    for (unsigned int o = 1; o < 32; o <<= 1)
-    {
-        const unsigned int t = __shfl_up(incl, o, 64);
-        if (lane >= o)
-            incl += t;
-    }
-    if (lane < nadj)
-        aoff[lane + 1] = incl;
-    if (lane == 0)
-        aoff[0] = 0;
-    const unsigned int total = __shfl(incl, (int)nadj - 1, 64);
     const unsigned int nhome = min(min(cell_size[c], cmax), NL_HOME_MAX);
     const unsigned int* home = cell_idx + (size_t)c * cmax;
-    for (unsigned int h = lane; h < nhome; h += 64)
+    for (unsigned int h = tid; h < nhome; h += blockDim.x)
         hcnt[h] = 0;
     __syncthreads();
+    const unsigned int total = aoff[nadj];
+    const bool rel = d >= 4;
+    const double w = R.L / (double)d;
+    const double ccx = -R.hi + ((double)cx + 0.5) * w, ccy = -R.hi + ((double)cy + 0.5) * w,
+                 ccz = -R.hi + ((double)cz + 0.5) * w;
+    const double rl = sqrt(rlistsq);
+    const float rlsq_f = (float)rlistsq;
+    const float guard = (float)(32.0 * 0x1p-24 * (w * rl + rlistsq));
     for (unsigned int base = 0; base < total; base += NL_CAP)
     {
         const unsigned int npass = min(NL_CAP, total - base);
-        for (unsigned int k = lane; k < npass; k += 64)
+        for (unsigned int k = tid; k < npass; k += blockDim.x)
         {
             const unsigned int g = base + k;
             unsigned int seg = 0;
             while (aoff[seg + 1] <= g)
                 seg++;
             const unsigned int j = cell_idx[(size_t)acell[seg] * cmax + (g - aoff[seg])];
-            const double4 xj = pos[j];
-            sj[k] = j;
-            sx[k] = xj.x;
-            sy[k] = xj.y;
-            sz[k] = xj.z;
+            float4 q = make_float4(0.f, 0.f, 0.f, __uint_as_float(j));
+            if (rel)
+            {
+                const double4 xj = pos[j];
+                q.x = centre_offset(xj.x, ccx, R);
+                q.y = centre_offset(xj.y, ccy, R);
+                q.z = centre_offset(xj.z, ccz, R);
+            }
+            sc[k] = q;
         }
         __syncthreads();
-        for (unsigned int h = 0; h < nhome; h++)
+        for (unsigned int h = wave; h < nhome; h += NL_WAVES)
         {
             const unsigned int i = home[h];
             const double4 xi = pos[i];
+            const float hx = rel ? centre_offset(xi.x, ccx, R) : 0.f;
+            const float hy = rel ? centre_offset(xi.y, ccy, R) : 0.f;
+            const float hz = rel ? centre_offset(xi.z, ccz, R) : 0.f;
             unsigned int cnt = hcnt[h];
             unsigned int* row = nlist + (size_t)i * nmax;
             for (unsigned int k0 = 0; k0 < npass; k0 += 64)
@@ -406,12 +453,19 @@ __global__ __launch_bounds__(64) void bnlist_kernel(
                 unsigned int j = 0;
                 if (k < npass)
                 {
-                    j = sj[k];
-                    double dx = xi.x - sx[k], dy = xi.y - sy[k], dz = xi.z - sz[k];
-                    dz -= R.L * rint(dz * R.Linv);
-                    dy -= R.L * rint(dy * R.Linv);
-                    dx -= R.L * rint(dx * R.Linv);
-                    hit = j != i && dx * dx + dy * dy + dz * dz <= rlistsq;
+                    const float4 q = sc[k];
+                    j = __float_as_uint(q.w);
+                    if (rel)
+                    {
+                        const float dx = hx - q.x, dy = hy - q.y, dz = hz - q.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        hit = d2 <= rlsq_f;
+                        if (fabsf(d2 - rlsq_f) <= guard)
+                            hit = within_fp64(xi, pos[j], R, rlistsq);
+                    }
+                    else
+                        hit = within_fp64(xi, pos[j], R, rlistsq);
+                    hit = hit && j != i;
                 }
                 const uint64_t m = wave_ballot(hit);
                 const unsigned int at = cnt + lanes_below(m);
@@ -424,7 +478,7 @@ __global__ __launch_bounds__(64) void bnlist_kernel(
         }
         __syncthreads();
     }
-    for (unsigned int h = lane; h < nhome; h += 64)
+    for (unsigned int h = tid; h < nhome; h += blockDim.x)
     {
         const unsigned int cnt = hcnt[h];
         n_neigh[home[h]] = min(cnt, nmax);
@@ -742,7 +796,8 @@ struct hm_batch
         const size_t nslot = (size_t)ncell * cmax;
         hipLaunchKernelGGL(bcell_rank_kernel, dim3((unsigned int)((nslot + 255) / 256)), dim3(256),
                            0, stream, cell_rep, rst, ncell, cmax, cell_size, cell_raw, cell_idx);
-        hipLaunchKernelGGL(bnlist_kernel, dim3(ncell), dim3(64), 0, stream, reps, cell_rep, rst,
+        hipLaunchKernelGGL(bnlist_kernel, dim3(ncell), dim3(64 * NL_WAVES), 0, stream, reps,
+                           cell_rep, rst,
                            pos, cmax, cell_size, cell_idx, rlistsq, nmax, nlist, n_neigh, cond);
         bchk(hipGetLastError(), "batch rebuild");
     }

@@ -100,6 +100,11 @@ class SiteEnergyMLP(Force):
         if self._table is not None:
             self._push_table()
 
+    def _pull(self):
+        """Keep the trained parameters when the simulation rebuilds its system."""
+        if self._attached():
+            self.params  # the getter downloads them into the host copy
+
     def _push_table(self):
         t = self._table
         if t is None:
@@ -447,6 +452,18 @@ def ibi_update(params, r_cut, r, g_cur, g_target, kT=1.0, damping=0.3, g_min=0.0
     corr *= 0.5 * (1.0 - np.cos(np.pi * t))
     Un = U + corr
     return dict(r_min=0.0, U=Un, F=-np.gradient(Un, rt))
+
+
+def pressure_ramp(params, r_cut, dp, kT=1.0, scale=0.1, dp_scale=1.0):
+    """The linear pressure correction used beside iterative Boltzmann inversion (Reith, Puetz
+    and Mueller-Plathe 2003): U <- U + A (1 - r / r_cut), A = -scale kT clip(dp / dp_scale, -1,
+    1), dp = P_model - P_target. A model below its target pressure gets a repulsive ramp; the
+    ramp is zero at r_cut. F = -dU/dr."""
+    U = np.asarray(params["U"], np.float64)
+    rt = np.linspace(float(params.get("r_min", 0.0)), r_cut, len(U), endpoint=False)
+    A = -scale * kT * float(np.clip(dp / dp_scale, -1.0, 1.0))
+    Un = U + A * (1.0 - rt / r_cut)
+    return dict(r_min=float(params.get("r_min", 0.0)), U=Un, F=-np.gradient(Un, rt))
 
 
 def soft_cosine_table(A, r_cut, width=200):

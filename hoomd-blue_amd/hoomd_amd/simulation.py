@@ -171,6 +171,12 @@ class Simulation:
     def _destroy_system(self):
         if self._system is not None:
             self._state._sync_from_device()
+            # forces whose device state the host copy does not follow (trained parameters)
+            for f in getattr(self, "_attached_forces", ()):
+                pull = getattr(f, "_pull", None)
+                if pull is not None and getattr(f, "_simulation", None) is self:
+                    pull()
+            self._attached_forces = ()
             self._timestep = int(_lib.lib().hm_sys_timestep(self._system))
             _lib.lib().hm_sys_destroy(self._system)
             self._system = None
@@ -233,6 +239,7 @@ class Simulation:
         L = _lib.lib()
         h = self._system
         integ._attach(self, h)
+        self._attached_forces = tuple(integ.forces)
         sorter = [t for t in self.operations.tuners if type(t).__name__ == "ParticleSorter"]
         if sorter:
             s = sorter[-1]

@@ -230,6 +230,32 @@ def test_pair_table_in_force_pass(hoomd):
     assert loss < 1e-6 * loss_none and loss_none > 0
 
 
+def test_trained_params_survive_system_rebuild(hoomd):
+    """A parameter change of another force (here the CG bonds) makes the simulation rebuild its
+    device system; the site energy keeps the parameters training left on the device."""
+    from scipy.spatial import cKDTree
+    from hoomd_amd.md import mlcg
+    pos, L = cg_config(1500, seed=23)
+    x = np.mod(pos + L / 2, L)
+    _, nn = cKDTree(x, boxsize=L).query(x, k=2)
+    pairs = np.unique(np.sort(np.stack([np.arange(len(pos)), nn[:, 1]], 1), axis=1), axis=0)
+    sim, mlp, harm = cg_sim(hoomd, pos, L, bonds=pairs.astype(np.uint32),
+                            params=M.init_params(3).astype(np.float32), exclusions=("bond",))
+    sim.run(0)
+    fs = mlcg.FrameSet(1, len(pos))
+    fs.set(0, positions=pos, forces=np.zeros((len(pos), 3)))
+    from hoomd_amd import _lib
+    before = mlp.params
+    mlp.train(fs, 3, lr=1e-2)
+    # read the device copy directly: the property would refresh the host copy itself
+    trained = np.zeros(mlcg.NPARAM, np.float32)
+    _lib.check(_lib.lib().hm_sys_mlp_get_params(sim._system, mlp._force_id, trained.ctypes.data))
+    assert not np.array_equal(before, trained)
+    harm.params["cg"] = dict(k=6.0, r0=1.6)
+    sim.run(1)
+    assert np.array_equal(mlp.params, trained)
+
+
 def test_kg_to_cg_network_gains(hoomd):
     """Fine-grained Kremer-Grest melt (200 chains x 50 beads, random walks pushed off, WCA +
     FENE, Langevin) mapped 5:1 on the GPU. The CG model without the network: harmonic CG bonds
