@@ -84,7 +84,17 @@ struct Rep
     unsigned int tile_off, ntile;
     unsigned int seed;          // 16-bit user seed
     unsigned int pad;
+    double w;                   // cell width L / dim
 };
+
+// Replicas with at least REL_MIN_DIM cells per axis keep FP32 positions relative to the centre
+// of each particle's cell at the last build (rel, refreshed every step) and store row entries
+// as the neighbour index plus the code (ox+1) + 3 (oy+1) + 9 (oz+1) of the neighbour cell's
+// offset from the particle's cell, so the pair vector is rel_i - rel_j - o w in FP32.
+constexpr unsigned int REL_MIN_DIM = 4;
+constexpr unsigned int BCODE_SHIFT = 26;
+constexpr unsigned int BJ_MASK = (1u << BCODE_SHIFT) - 1u;
+constexpr unsigned int BCODE_SELF = 13u; // offset (0, 0, 0)
 
 // per-replica device state
 struct RepState
@@ -113,7 +123,22 @@ struct PairConst
     float lj1, lj2;
     double rcutsq;
     float rcutsq_f;
+    float guard_w, guard_0; // FP32 cutoff guard band: guard_w * w + guard_0
 };
+
+// FP32 offset of x from the centre of the cell with packed coordinates cc (x | y << 10 | z << 20),
+// wrapped to the nearest image in FP64 first
+__device__ __forceinline__ float4 centre_rel(const double4& x, unsigned int cc, const Rep& R)
+{
+#pragma clang fp contract(off)
+    const double c[3] = {-R.hi + ((double)(cc & 1023u) + 0.5) * R.w,
+                         -R.hi + ((double)((cc >> 10) & 1023u) + 0.5) * R.w,
+                         -R.hi + ((double)(cc >> 20) + 0.5) * R.w};
+    double d[3] = {x.x - c[0], x.y - c[1], x.z - c[2]};
+    for (int k = 0; k < 3; k++)
+        d[k] -= R.L * rint(d[k] * R.Linv);
+    return make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
+}
 
 __device__ __forceinline__ double block_reduce(double v, double* sh)
 {
@@ -148,7 +173,8 @@ __global__ __launch_bounds__(BTILE) void bstep_one_kernel(
     RepState* __restrict__ rst, const BatchStep* __restrict__ bst,
     const double* __restrict__ ke_partial, double4* __restrict__ pos, double4* __restrict__ vel,
     const d3* __restrict__ accel, i3* __restrict__ image, const d3* __restrict__ last,
-    BCond* __restrict__ cond, double dt, double tau, int bussi, double maxsq)
+    BCond* __restrict__ cond, double dt, double tau, int bussi, double maxsq,
+    const unsigned int* __restrict__ cellc, float4* __restrict__ rel)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[4];
@@ -207,6 +233,8 @@ __global__ __launch_bounds__(BTILE) void bstep_one_kernel(
         pos[p] = x;
         vel[p] = v;
         image[p] = im;
+        if (R.dim >= REL_MIN_DIM)
+            rel[p] = centre_rel(x, cellc[p], R);
         if (x.x != x.x || x.y != x.y || x.z != x.z)
             atomicMax(&cond->nan_idx, p + 1);
         const d3 l = last[p];
@@ -260,7 +288,8 @@ __global__ __launch_bounds__(BTILE) void bcell_bin_kernel(
     const Rep* __restrict__ reps, const unsigned int* __restrict__ tile_rep,
     const RepState* __restrict__ rst, const double4* __restrict__ pos, d3* __restrict__ last,
     unsigned int cmax, unsigned int* __restrict__ cell_size, unsigned int* __restrict__ cell_raw,
-    unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond)
+    unsigned int* __restrict__ n_neigh, BCond* __restrict__ cond, unsigned int* __restrict__ cellc,
+    float4* __restrict__ rel)
 {
     const unsigned int t = blockIdx.x;
     const unsigned int r = tile_rep[t];
@@ -275,9 +304,12 @@ __global__ __launch_bounds__(BTILE) void bcell_bin_kernel(
     const double4 x = pos[p];
     last[p] = d3{x.x, x.y, x.z};
     n_neigh[p] = 0;
-    const unsigned int c = R.cell_off
-                           + (cell_coord(x.z, R) * R.dim + cell_coord(x.y, R)) * R.dim
-                           + cell_coord(x.x, R);
+    const unsigned int ccx = cell_coord(x.x, R), ccy = cell_coord(x.y, R), ccz = cell_coord(x.z, R);
+    const unsigned int c = R.cell_off + (ccz * R.dim + ccy) * R.dim + ccx;
+    const unsigned int cc = ccx | (ccy << 10) | (ccz << 20);
+    cellc[p] = cc;
+    if (R.dim >= REL_MIN_DIM)
+        rel[p] = centre_rel(x, cc, R);
     const unsigned int slot = atomicAdd(&cell_size[c], 1u);
     if (slot < cmax)
         cell_raw[(size_t)c * cmax + slot] = p;
@@ -409,7 +441,7 @@ __global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
         hcnt[h] = 0;
     __syncthreads();
     const unsigned int total = aoff[nadj];
-    const bool rel = d >= 4;
+    const bool rel = d >= (int)REL_MIN_DIM;
     const double w = R.L / (double)d;
     const double ccx = -R.hi + ((double)cx + 0.5) * w, ccy = -R.hi + ((double)cy + 0.5) * w,
                  ccz = -R.hi + ((double)cz + 0.5) * w;
@@ -426,7 +458,9 @@ __global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
             while (aoff[seg + 1] <= g)
                 seg++;
             const unsigned int j = cell_idx[(size_t)acell[seg] * cmax + (g - aoff[seg])];
-            float4 q = make_float4(0.f, 0.f, 0.f, __uint_as_float(j));
+            // with offsets, the stored entry carries the cell-offset code: seg enumerates the
+            // 27 offsets as (ox + 1) + 3 (oy + 1) + 9 (oz + 1)
+            float4 q = make_float4(0.f, 0.f, 0.f, __uint_as_float(rel ? j | (seg << BCODE_SHIFT) : j));
             if (rel)
             {
                 const double4 xj = pos[j];
@@ -450,11 +484,12 @@ __global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
             {
                 const unsigned int k = k0 + lane;
                 bool hit = false;
-                unsigned int j = 0;
+                unsigned int e = 0;
                 if (k < npass)
                 {
                     const float4 q = sc[k];
-                    j = __float_as_uint(q.w);
+                    e = __float_as_uint(q.w);
+                    const unsigned int j = e & BJ_MASK;
                     if (rel)
                     {
                         const float dx = hx - q.x, dy = hy - q.y, dz = hz - q.z;
@@ -470,7 +505,7 @@ __global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
                 const uint64_t m = wave_ballot(hit);
                 const unsigned int at = cnt + lanes_below(m);
                 if (hit && at < nmax)
-                    row[at] = j;
+                    row[at] = e;
                 cnt += (unsigned int)__popcll(m);
             }
             if (lane == 0)
@@ -488,20 +523,88 @@ __global__ __launch_bounds__(64 * NL_WAVES) void bnlist_kernel(
 }
 
 // -------------------------------------------------------------------------- pair forces
-// tpp lanes per particle split its row; FP64 separation and exact cutoff test, FP32 evaluator
-// arithmetic and FP32 per-particle sums (force, pair energy halved as PotentialPair.h:739-772).
+// tpp lanes per particle split its row; FP32 evaluator arithmetic and FP32 per-particle sums
+// (force, pair energy halved as PotentialPair.h:739-772). Replicas with offsets: the FP32 pair
+// vector rel_i - rel_j - o w (16-byte gathers) and an FP32 cutoff test whose guard band is
+// re-decided from the FP64 positions; others: FP64 separation and exact cutoff test.
+__device__ __forceinline__ void lj_accumulate(const PairConst& pc, float rsq, float dx, float dy,
+                                              float dz, float& fx, float& fy, float& fz,
+                                              float& e)
+{
+#pragma clang fp contract(off)
+    const float r2inv = recip(rsq);
+    const float r6inv = r2inv * r2inv * r2inv;
+    const float fdr = r2inv * r6inv * (12.f * pc.lj1 * r6inv - 6.f * pc.lj2);
+    e += r6inv * (pc.lj1 * r6inv - pc.lj2);
+    fx += dx * fdr;
+    fy += dy * fdr;
+    fz += dz * fdr;
+}
+
 template<unsigned int TPP>
 __global__ __launch_bounds__(256) void bpair_kernel(
     const Rep* __restrict__ reps, const unsigned int* __restrict__ part_rep, unsigned int N,
     const double4* __restrict__ pos, const unsigned int* __restrict__ nlist, unsigned int nmax,
-    const unsigned int* __restrict__ n_neigh, PairConst pc, float4* __restrict__ force)
+    const unsigned int* __restrict__ n_neigh, PairConst pc, float4* __restrict__ force,
+    const float4* __restrict__ rel)
 {
 #pragma clang fp contract(off)
     const unsigned int g = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int p = g / TPP;
     const unsigned int sub = g % TPP;
     float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
-    if (p < N)
+    constexpr unsigned int UNR = 4;
+    if (p < N && reps[part_rep[p]].dim >= REL_MIN_DIM)
+    {
+        const Rep R = reps[part_rep[p]];
+        const float4 ri = rel[p];
+        const float w = (float)R.w;
+        const float guard = pc.guard_w * w + pc.guard_0;
+        const unsigned int nn = min(n_neigh[p], nmax);
+        const unsigned int* row = nlist + (size_t)p * nmax;
+        for (unsigned int k0 = sub; k0 < nn; k0 += TPP * UNR)
+        {
+            unsigned int en[UNR];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
+            {
+                const unsigned int k = k0 + u * TPP;
+                // the particle itself, at zero offset, stands in for missing entries
+                en[u] = k < nn ? row[k] : (p | (BCODE_SELF << BCODE_SHIFT));
+            }
+            float4 rj[UNR];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
+                rj[u] = rel[en[u] & BJ_MASK];
+#pragma unroll
+            for (unsigned int u = 0; u < UNR; u++)
+            {
+                const unsigned int j = en[u] & BJ_MASK, code = en[u] >> BCODE_SHIFT;
+                const float ox = (float)((int)(code % 3u) - 1);
+                const float oy = (float)((int)((code / 3u) % 3u) - 1);
+                const float oz = (float)((int)(code / 9u) - 1);
+                const float dx = (ri.x - rj[u].x) - ox * w;
+                const float dy = (ri.y - rj[u].y) - oy * w;
+                const float dz = (ri.z - rj[u].z) - oz * w;
+                const float rsq = dx * dx + dy * dy + dz * dz;
+                const float t = rsq - pc.rcutsq_f;
+                bool in = t < 0.f;
+                if (fabsf(t) <= guard)
+                {
+                    // exact FP64 decision (the FP64 form of this kernel)
+                    const double4 xi = pos[p], xj = pos[j];
+                    double ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    ddz -= R.L * rint(ddz * R.Linv);
+                    ddy -= R.L * rint(ddy * R.Linv);
+                    ddx -= R.L * rint(ddx * R.Linv);
+                    in = ddx * ddx + ddy * ddy + ddz * ddz < pc.rcutsq;
+                }
+                if (in && j != p)
+                    lj_accumulate(pc, rsq, dx, dy, dz, fx, fy, fz, e);
+            }
+        }
+    }
+    else if (p < N)
     {
         const Rep R = reps[part_rep[p]];
         const double4 xi = pos[p];
@@ -509,7 +612,6 @@ __global__ __launch_bounds__(256) void bpair_kernel(
         const unsigned int* row = nlist + (size_t)p * nmax;
         // UNR neighbours per lane per pass: every index, then every position load is issued
         // before the first is used, so several gathers are in flight per lane
-        constexpr unsigned int UNR = 4;
         for (unsigned int k0 = sub; k0 < nn; k0 += TPP * UNR)
         {
             unsigned int j[UNR];
@@ -533,15 +635,7 @@ __global__ __launch_bounds__(256) void bpair_kernel(
                 const double rsq = dx * dx + dy * dy + dz * dz;
                 // the particle itself stands in for missing entries: rsq = 0 is excluded
                 if (rsq < pc.rcutsq && j[u] != p)
-                {
-                    const float r2inv = recip((float)rsq);
-                    const float r6inv = r2inv * r2inv * r2inv;
-                    const float fdr = r2inv * r6inv * (12.f * pc.lj1 * r6inv - 6.f * pc.lj2);
-                    e += r6inv * (pc.lj1 * r6inv - pc.lj2);
-                    fx += (float)dx * fdr;
-                    fy += (float)dy * fdr;
-                    fz += (float)dz * fdr;
-                }
+                    lj_accumulate(pc, (float)rsq, (float)dx, (float)dy, (float)dz, fx, fy, fz, e);
             }
         }
     }
@@ -693,6 +787,9 @@ struct hm_batch
     // cell-order sort: replica-local tag of each particle (its index at creation, minus the
     // replica's first index), the new order and a staging buffer for the permutation
     unsigned int* tag = nullptr;
+    // offsets from the particle's cell centre at the last build, and that cell (packed)
+    float4* rel = nullptr;
+    unsigned int* cellc = nullptr;
     unsigned int* cell_start = nullptr;
     unsigned int* order = nullptr;
     void* stage = nullptr;
@@ -717,7 +814,7 @@ struct hm_batch
                          (void**)&tile_rep, (void**)&cell_rep, (void**)&part_rep, (void**)&pos,
                          (void**)&vel, (void**)&accel, (void**)&image, (void**)&last,
                          (void**)&force, (void**)&ke_partial, (void**)&cell_size,
-                         (void**)&cell_raw, (void**)&cell_idx, (void**)&nlist, (void**)&n_neigh, (void**)&tag, (void**)&cell_start, (void**)&order, (void**)&stage, (void**)&s_pos,
+                         (void**)&cell_raw, (void**)&cell_idx, (void**)&nlist, (void**)&n_neigh, (void**)&tag, (void**)&rel, (void**)&cellc, (void**)&cell_start, (void**)&order, (void**)&stage, (void**)&s_pos,
                          (void**)&s_vel, (void**)&s_accel, (void**)&s_last, (void**)&s_image,
                          (void**)&s_ke, (void**)&s_tag, (void**)&s_rst, (void**)&s_bst})
             if (*p)
@@ -765,23 +862,23 @@ struct hm_batch
         {
         case 1:
             hipLaunchKernelGGL(bpair_kernel<1>, grid, block, 0, stream, reps, part_rep, N, pos,
-                               nlist, nmax, n_neigh, pc, force);
+                               nlist, nmax, n_neigh, pc, force, rel);
             break;
         case 2:
             hipLaunchKernelGGL(bpair_kernel<2>, grid, block, 0, stream, reps, part_rep, N, pos,
-                               nlist, nmax, n_neigh, pc, force);
+                               nlist, nmax, n_neigh, pc, force, rel);
             break;
         case 4:
             hipLaunchKernelGGL(bpair_kernel<4>, grid, block, 0, stream, reps, part_rep, N, pos,
-                               nlist, nmax, n_neigh, pc, force);
+                               nlist, nmax, n_neigh, pc, force, rel);
             break;
         case 8:
             hipLaunchKernelGGL(bpair_kernel<8>, grid, block, 0, stream, reps, part_rep, N, pos,
-                               nlist, nmax, n_neigh, pc, force);
+                               nlist, nmax, n_neigh, pc, force, rel);
             break;
         default:
             hipLaunchKernelGGL(bpair_kernel<16>, grid, block, 0, stream, reps, part_rep, N, pos,
-                               nlist, nmax, n_neigh, pc, force);
+                               nlist, nmax, n_neigh, pc, force, rel);
             break;
         }
         bchk(hipGetLastError(), "batch pair");
@@ -792,7 +889,7 @@ struct hm_batch
         hipLaunchKernelGGL(bcell_clear_kernel, dim3((ncell + 255) / 256), dim3(256), 0, stream,
                            reps, cell_rep, rst, ncell, cell_size);
         hipLaunchKernelGGL(bcell_bin_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
-                           rst, pos, last, cmax, cell_size, cell_raw, n_neigh, cond);
+                           rst, pos, last, cmax, cell_size, cell_raw, n_neigh, cond, cellc, rel);
         const size_t nslot = (size_t)ncell * cmax;
         hipLaunchKernelGGL(bcell_rank_kernel, dim3((unsigned int)((nslot + 255) / 256)), dim3(256),
                            0, stream, cell_rep, rst, ncell, cmax, cell_size, cell_raw, cell_idx);
@@ -806,7 +903,7 @@ struct hm_batch
     {
         hipLaunchKernelGGL(bstep_one_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps, tile_rep,
                            rst, bst, ke_partial, pos, vel, accel, image, last, cond, dt, tau, 1,
-                           maxsq);
+                           maxsq, cellc, rel);
         bchk(hipGetLastError(), "batch step one");
         launch_rebuild();
         launch_pair();
@@ -938,7 +1035,7 @@ struct hm_batch
                                stream, reps, cell_rep, rst, ncell, cell_size);
             hipLaunchKernelGGL(bcell_bin_kernel, dim3(ntile), dim3(BTILE), 0, stream, reps,
                                tile_rep, rst, pos, last, cmax, cell_size, cell_raw, n_neigh,
-                               cond);
+                               cond, cellc, rel);
             const size_t nslot = (size_t)ncell * cmax;
             hipLaunchKernelGGL(bcell_rank_kernel, dim3((unsigned int)((nslot + 255) / 256)),
                                dim3(256), 0, stream, cell_rep, rst, ncell, cmax, cell_size,
@@ -1207,6 +1304,14 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         b->pc.lj2 = (float)(4.0 * epsilon * s6);
         b->pc.rcutsq = r_cut * r_cut;
         b->pc.rcutsq_f = (float)(r_cut * r_cut);
+        {
+            // FP32 pair-vector error per component, u (3 w + 2 r_buff + r_cut): the two offsets
+            // (each within w / 2 + r_buff / 2 of its cell centre), their difference, o w and
+            // the final subtraction; then r^2 within 2 sqrt(3) r_cut e + 3 u r_cut^2, taken 4x
+            const double u = 1.0 / 16777216.0, rc = r_cut, s3 = std::sqrt(3.0);
+            b->pc.guard_w = (float)(4.0 * u * 2.0 * s3 * rc * 3.0);
+            b->pc.guard_0 = (float)(4.0 * u * (2.0 * s3 * rc * (2.0 * r_buff + rc) + 3.0 * rc * rc));
+        }
         b->hrep.resize(nrep);
         unsigned long long off = 0, coff = 0, toff = 0;
         double rho_max = 0.0, cell_fill_max = 0.0;
@@ -1228,6 +1333,7 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
             // CellList::computeDimensions (hoomd/CellList.cc:83-131): floor(L / r_list) cells
             R.dim = std::max(1u, (unsigned int)std::floor(L[q] / rlist));
             R.cell_inv = R.dim / L[q];
+            R.w = L[q] / R.dim;
             R.off = (unsigned int)off;
             R.n = n[q];
             R.cell_off = (unsigned int)coff;
@@ -1242,7 +1348,8 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
             const double w = L[q] / R.dim;
             cell_fill_max = std::max(cell_fill_max, n[q] / (L[q] * L[q] * L[q]) * w * w * w);
             rho_max = std::max(rho_max, n[q] / (L[q] * L[q] * L[q]));
-            if (off >= (1ull << 31) || coff >= (1ull << 31))
+            // row entries hold a particle index in BCODE_SHIFT bits; cell coordinates in 10
+            if (off >= (1ull << BCODE_SHIFT) || coff >= (1ull << 31) || R.dim >= 1024u)
                 throw BError{HM_ERR_ARGS, "hm_batch_create: too many particles or cells"};
         }
         b->N = (unsigned int)off;
@@ -1273,6 +1380,10 @@ hm_batch* hm_batch_create(unsigned int nrep, const unsigned int* n, const double
         b->cell_size = balloc<unsigned int>(b->ncell);
         b->n_neigh = balloc<unsigned int>(N);
         b->tag = balloc<unsigned int>(N);
+        b->rel = balloc<float4>(N);
+        b->cellc = balloc<unsigned int>(N);
+        bchk(hipMemset(b->rel, 0, sizeof(float4) * N), "clear offsets");
+        bchk(hipMemset(b->cellc, 0, sizeof(unsigned int) * N), "clear cells");
         b->alloc_lists();
         std::vector<unsigned int> trep(b->ntile), crep(b->ncell), prep_(N), htag(N);
         std::vector<double4> hp(N), hv(N);
