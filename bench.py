@@ -309,12 +309,11 @@ def kg(steps, warmup, seed, cpu_steps, pushoff=10000):
     sim.run(warmup)
     _lib.check(_lib.lib().hm_synchronize())
     _lib.check(_lib.lib().hm_sys_prime(sim._system, steps), "prime graphs")
-    builds0 = nl.num_builds
     t0 = time.perf_counter()
     sim.run(steps)
     _lib.check(_lib.lib().hm_synchronize())
     elapsed = time.perf_counter() - t0
-    builds = nl.num_builds - builds0
+    builds = nl.num_builds  # builds of the last run
     tps = steps / elapsed
     cpu = None
     if cpu_steps:
@@ -710,6 +709,81 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     return 0
 
 
+def cg_step(n_chains, steps, warmup, seed):
+    """Profiling mode of the learned-CG step alone: n_chains x 10 CG sites as random walks of
+    step 1.7 at site density 0.17 (the 5:1-mapped melt's), the site energy with random
+    parameters (init_params(seed) x 0.05) and a repulsive pair table in its force pass, harmonic
+    bonds k 2.3 r0 1.74, Langevin kT 1 gamma 5, site mass 5, dt 0.02; relaxed for warmup steps,
+    then steps timed. Prints one JSON line (timesteps/s and the three site-energy kernels)."""
+    import ctypes as Cc
+
+    import hoomd_amd as hoomd
+    from hoomd_amd import _lib
+    from hoomd_amd.md import mlcg as MC
+    per = 10
+    N = n_chains * per
+    L = (N / 0.17) ** (1 / 3)
+    rng = np.random.default_rng(seed)
+    start = rng.uniform(-L / 2, L / 2, (n_chains, 1, 3))
+    stp = rng.normal(size=(n_chains, per, 3))
+    stp /= np.linalg.norm(stp, axis=2, keepdims=True)
+    pos = ((start + 1.7 * np.cumsum(stp, axis=1)).reshape(-1, 3) + L / 2) % L - L / 2
+    idx = np.arange(N).reshape(n_chains, per)
+    bonds = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1).astype(np.uint32)
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = N
+    snap.particles.types = ["S"]
+    snap.particles.position[:] = pos
+    snap.particles.mass[:] = 5.0
+    snap.bonds.N = len(bonds)
+    snap.bonds.types = ["cg"]
+    snap.bonds.group[:] = bonds
+    cg = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed)
+    cg.create_state_from_snapshot(snap)
+    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=("bond",))
+    mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=seed)
+    r = (np.arange(80) + 0.5) * 0.05
+    prior, rc = MC.rdf_prior_table(r, np.clip((r - 0.3) / 1.5, 0.0, 1.0))
+    mlp.pair_table = MC.regrid_table(prior, rc, mlp.r_cut, width=600)
+    harm = hoomd.md.bond.Harmonic()
+    harm.params["cg"] = dict(k=2.3, r0=1.74)
+    lang = hoomd.md.methods.Langevin(filter=hoomd.filter.All(), kT=1.0)
+    lang.gamma["S"] = 5.0
+    cg.operations.integrator = hoomd.md.Integrator(dt=0.02, forces=[mlp, harm], methods=[lang])
+    cg.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
+    mlp.params = MC.init_params(seed) * 0.05
+    log(f"[cg-step] {N} sites, L {L:.3f}: relax {warmup} steps")
+    cg.run(warmup)
+    _lib.check(_lib.lib().hm_sys_prime(cg._system, steps), "prime")
+    _lib.check(_lib.lib().hm_synchronize())
+    t0 = time.perf_counter()
+    cg.run(steps)
+    _lib.check(_lib.lib().hm_synchronize())
+    el = time.perf_counter() - t0
+    builds = nl.num_builds  # builds of the last run
+    ms3 = (Cc.c_double * 3)()
+    _lib.check(_lib.lib().hm_sys_time_mlp(cg._system, mlp._force_id, 20, ms3), "time mlp")
+    site_flops = MLP_SITE_FLOP * N
+    log(f"[cg-step] {steps / el:.1f} steps/s; densities {ms3[0] * 1e3:.1f} us, site MLP "
+        f"{ms3[1] * 1e3:.1f} us, forces {ms3[2] * 1e3:.1f} us")
+    print(json.dumps({
+        "metric": "timesteps/s (learned-CG step, synthetic state)", "value": steps / el,
+        "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps, "higher_is_better": True,
+        "kernel_ms": {"densities": ms3[0], "site_mlp": ms3[1], "forces": ms3[2]},
+        "mfma": {"site_flop": MLP_SITE_FLOP, "achieved_tflops": site_flops / (ms3[1] * 1e-3)
+                 / 1e12, "peak_tflops_bf16": 2500.0,
+                 "utilisation": site_flops / (ms3[1] * 1e-3) / 2.5e15},
+        "dtype": "bf16 MFMA (fp32 accumulate) MLP, f32 densities/forces, f64 integration",
+        "data": "synthetic (random-walk CG chains, random network parameters)",
+        "config": {"workload": f"learned-CG step, {N} sites ({n_chains} x {per}), MLP "
+                               "16-64-64-1 + pair table in the force pass, harmonic bonds, "
+                               "Langevin dt 0.02", "list_builds_in_timed_steps": builds}}),
+          flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -752,6 +826,9 @@ def main():
                          "gives the same structure and temperature)")
     ap.add_argument("--pressure-match", action="store_true",
                     help="add the linear pressure correction to each refinement step")
+    ap.add_argument("--cg-step", type=int, default=0, metavar="CHAINS",
+                    help="profiling mode: the learned-CG step alone on CHAINS x 10 synthetic "
+                         "sites (20000 for the 200 000-site system)")
     ap.add_argument("--replicas", type=int, default=0, metavar="R",
                     help="run R batched uneven LJ replicas instead (256 for the batched workload)")
     a = ap.parse_args()
@@ -762,6 +839,11 @@ def main():
         return mlcg(a.mlcg, a.steps, a.warmup, a.seed, a.frames, a.frame_every, a.train_steps,
                     a.pushoff, a.lr, a.frames_per_step, a.ibi_iters, a.cg_dt,
                     a.pressure_match)
+    if a.cg_step:
+        if a.gpus != 1:
+            print("bench.py runs on exactly one GPU", file=sys.stderr)
+            return 2
+        return cg_step(a.cg_step, a.steps, a.warmup, a.seed)
     if a.replicas:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)

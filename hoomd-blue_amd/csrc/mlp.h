@@ -44,6 +44,10 @@ struct MlpDescr
 {
     float rcut, rcutsq, inv_rcut, mu0, dmu, inv2w2, invw2;
     float gmean[MLP_NK], gscale[MLP_NK];
+    // the Gaussians by recurrence from the middle centre (three exponentials per pair) when
+    // every g_MID stays far above the FP32 underflow over [0, rcut); else sixteen exponentials
+    int rebase;
+    float inv_dmu, b, q;   // 1 / dmu, inv2w2 dmu^2, exp(-2 b)
 };
 
 struct MlpArgs

@@ -10,7 +10,6 @@
 // so a layer's result, rounded to bf16, is already the next layer's B operand: no shuffles
 // between layers. Weights (and their transposes, for the backward) are staged in LDS as bf16.
 #include "mlp.h"
-#include "evaluators.h"
 
 #include <cmath>
 
@@ -23,6 +22,7 @@ typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 constexpr int PAIR_TPP = 4;  // lanes per site in the pair passes
+constexpr int GMID = MLP_NK / 2;  // base centre of the Gaussian recurrence
 
 __device__ __forceinline__ unsigned short f2bf(float f)
 {
@@ -60,23 +60,18 @@ __device__ __forceinline__ bf4 ldsA(const unsigned short* W, int K, int m, int k
 
 __device__ __forceinline__ float sigm(float h)
 {
-    return 1.0f / (1.0f + __expf(-h));
+    // hardware reciprocal (1 ulp): the result feeds bf16 operands and FP32 sums
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-h));
 }
-__device__ __forceinline__ float silu(float h)
+// silu'(h) and silu''(h) from s = sigm(h)
+__device__ __forceinline__ float silu_d_s(float h, float s)
 {
-    return h * sigm(h);
-}
-__device__ __forceinline__ float silu_d(float h)
-{
-    const float s = sigm(h);
     return s * (1.0f + h * (1.0f - s));
 }
-__device__ __forceinline__ float silu_dd(float h)
+__device__ __forceinline__ float silu_dd_s(float h, float s)
 {
-    const float s = sigm(h);
     return s * (1.0f - s) * (2.0f + h * (1.0f - 2.0f * s));
 }
-
 __device__ __forceinline__ f4v zero4()
 {
     f4v z = {0.f, 0.f, 0.f, 0.f};
@@ -107,6 +102,42 @@ __device__ __forceinline__ void cutoff(const MlpDescr& d, float r, float& fc, fl
     sincospif(r * d.inv_rcut, &sn, &cs);
     fc = 0.5f * (cs + 1.0f);
     dfc = -0.5f * 3.14159265358979f * d.inv_rcut * sn;
+}
+
+// g[k] = exp(-(r - mu_k)^2 inv2w2), mu_k = mu0 + k dmu. With rebase: t = (r - mu0) / dmu and
+// g_k = exp(-b (t - k)^2); from the middle centre GMID the ratio g_{k+1} / g_k = exp(2 b (t - k) - b)
+// and g_{k-1} / g_k = exp(-2 b (t - k) - b) each change by the factor exp(-2 b) per step, so three
+// exponentials and thirty products give all sixteen (about 1e-6 relative error after 8 steps).
+__device__ __forceinline__ void gaussians(const MlpDescr& d, float r, float (&g)[MLP_NK])
+{
+    if (d.rebase)
+    {
+        const float s = (r - d.mu0) * d.inv_dmu - (float)GMID;
+        g[GMID] = __expf(-d.b * s * s);
+        float up = __expf(2.f * d.b * s - d.b);    // g_{GMID+1} / g_GMID
+        float dn = __expf(-2.f * d.b * s - d.b);   // g_{GMID-1} / g_GMID
+#pragma unroll
+        for (int k = GMID; k + 1 < MLP_NK; k++)
+        {
+            g[k + 1] = g[k] * up;
+            up *= d.q;
+        }
+#pragma unroll
+        for (int k = GMID; k > 0; k--)
+        {
+            g[k - 1] = g[k] * dn;
+            dn *= d.q;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < MLP_NK; k++)
+        {
+            const float dd = r - (d.mu0 + (float)k * d.dmu);
+            g[k] = __expf(-dd * dd * d.inv2w2);
+        }
+    }
 }
 
 template<int T> __device__ __forceinline__ float group_sum(float v)
@@ -143,12 +174,11 @@ __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
             const float r = sqrtf(rsq);
             float fc, dfc;
             cutoff(a.d, r, fc, dfc);
+            float gk[MLP_NK];
+            gaussians(a.d, r, gk);
 #pragma unroll
             for (int k = 0; k < MLP_NK; k++)
-            {
-                const float dd = r - (a.d.mu0 + (float)k * a.d.dmu);
-                acc[k] += __expf(-dd * dd * a.d.inv2w2) * fc;
-            }
+                acc[k] += gk[k] * fc;
         }
     }
 #pragma unroll
@@ -201,6 +231,7 @@ struct SiteTile
     bf4 xb;           // normalised descriptor (B operand, features 4g..4g+3)
     float x[4];
     f4v h1[4], h2[4];
+    f4v s1[4];        // sigm(h1), kept for the backward
     bf4 a1b[4];
     bf4 d2b[4];       // w3 * silu'(h2)
     f4v u[4];         // W2^T d2
@@ -230,7 +261,8 @@ __device__ __forceinline__ void site_eval(const MlpLds& S, const MlpArgs& a, con
         for (int r = 0; r < 4; r++)
         {
             T.h1[m][r] += S.b1[16 * m + 4 * g + r];
-            act[r] = silu(T.h1[m][r]);
+            T.s1[m][r] = sigm(T.h1[m][r]);
+            act[r] = T.h1[m][r] * T.s1[m][r];
         }
         T.a1b[m] = pack4(act);
     }
@@ -248,8 +280,9 @@ __device__ __forceinline__ void site_eval(const MlpLds& S, const MlpArgs& a, con
         {
             const int o = 16 * m + 4 * g + r;
             acc[r] += S.b2[o];
-            e += S.w3[o] * silu(acc[r]);
-            d2[r] = S.w3[o] * silu_d(acc[r]);
+            const float sg = sigm(acc[r]);
+            e += S.w3[o] * (acc[r] * sg);
+            d2[r] = S.w3[o] * silu_d_s(acc[r], sg);
         }
         T.h2[m] = acc;
         T.d2b[m] = pack4(d2);
@@ -268,7 +301,7 @@ __device__ __forceinline__ void site_eval(const MlpLds& S, const MlpArgs& a, con
         f4v d1;
 #pragma unroll
         for (int r = 0; r < 4; r++)
-            d1[r] = acc[r] * silu_d(T.h1[m][r]);
+            d1[r] = acc[r] * silu_d_s(T.h1[m][r], T.s1[m][r]);
         T.d1b[m] = pack4(d1);
     }
     f4v dx = zero4();
@@ -310,15 +343,30 @@ __global__ __launch_bounds__(256) void mlp_site_kernel(MlpArgs a)
 }
 
 // ------------------------------------------------------------------------------ pass 2
+// Tables up to this many points are staged in LDS for the force pass
+constexpr unsigned int MLP_TAB_LDS = 2048;
+
 template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
 {
+    __shared__ float stab[2 * MLP_TAB_LDS];
     const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
     const bool valid = i < a.N;
     float fx = 0.f, fy = 0.f, fz = 0.f, ep = 0.f;
     float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const EvaluatorPairTable<float>::param_type tp{a.ptab, a.ptab + a.ptab_width, a.ptab_rmin,
-                                                   a.ptab_width};
+    // the pair table (EvaluatorPairTable.h:152-196 interpolation, the division by the spacing
+    // taken as a product with its reciprocal), from LDS when it fits
+    const unsigned int tw = a.ptab_width;
+    const float* tU = a.ptab;
+    if (tw && tw <= MLP_TAB_LDS)
+    {
+        for (unsigned int t = threadIdx.x; t < 2 * tw; t += blockDim.x)
+            stab[t] = a.ptab[t];
+        __syncthreads();
+        tU = stab;
+    }
+    const float* tF = tU + tw;
+    const float t_inv = tw ? (float)tw / (a.d.rcut - a.ptab_rmin) : 0.f;
     if (valid)
     {
         float ai[MLP_NK];
@@ -334,58 +382,81 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
         const double4 pi = a.pos[i];
         const unsigned int n = min(a.n_neigh[i], a.nmax);
         const unsigned int* row = a.nlist + (size_t)i * a.nmax;
-        for (unsigned int q = sub; q < n; q += PAIR_TPP)
+        // two neighbours per iteration: both position and both dE/dG gathers are issued before
+        // the arithmetic of the first
+        constexpr int U = 2;
+        for (unsigned int q0 = sub; q0 < n; q0 += PAIR_TPP * U)
         {
-            const unsigned int j = row[q];
-            float dx, dy, dz;
-            pair_vec(a.box, pi, a.pos[j], dx, dy, dz);
-            const float rsq = dx * dx + dy * dy + dz * dz;
-            if (!(rsq < a.d.rcutsq))
-                continue;
-            const float r = sqrtf(rsq);
-            float fc, dfc;
-            cutoff(a.d, r, fc, dfc);
-            float aj[MLP_NK];
+            unsigned int jj[U];
+            bool live[U];
 #pragma unroll
-            for (int w = 0; w < 4; w++)
+            for (int u = 0; u < U; u++)
             {
-                const float4 t = a.A[(size_t)j * 4 + w];
-                aj[4 * w] = t.x;
-                aj[4 * w + 1] = t.y;
-                aj[4 * w + 2] = t.z;
-                aj[4 * w + 3] = t.w;
+                const unsigned int q = q0 + u * PAIR_TPP;
+                live[u] = q < n;
+                jj[u] = live[u] ? row[q] : i;
             }
-            float sdot = 0.f;
+            double4 pj[U];
+            float4 aj4[U][4];
 #pragma unroll
-            for (int k = 0; k < MLP_NK; k++)
+            for (int u = 0; u < U; u++)
             {
-                const float dd = r - (a.d.mu0 + (float)k * a.d.dmu);
-                const float gk = __expf(-dd * dd * a.d.inv2w2);
-                sdot += (ai[k] + aj[k]) * gk * (dfc - fc * dd * a.d.invw2);
+                pj[u] = a.pos[jj[u]];
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    aj4[u][w] = a.A[(size_t)jj[u] * 4 + w];
             }
-            float fdr = -sdot / r;
-            if (a.ptab_width)
+#pragma unroll
+            for (int u = 0; u < U; u++)
             {
-                float tf = 0.f, te = 0.f;
-                if (EvaluatorPairTable<float>(rsq, a.d.rcutsq, tp).evalForceAndEnergy(tf, te,
-                                                                                      false))
+                float dx, dy, dz;
+                pair_vec(a.box, pi, pj[u], dx, dy, dz);
+                const float rsq = dx * dx + dy * dy + dz * dz;
+                if (!live[u] || !(rsq < a.d.rcutsq))
+                    continue;
+                const float r = sqrtf(rsq);
+                const float rinv = __builtin_amdgcn_rcpf(r);
+                float fc, dfc;
+                cutoff(a.d, r, fc, dfc);
+                const float aj[MLP_NK] = {aj4[u][0].x, aj4[u][0].y, aj4[u][0].z, aj4[u][0].w,
+                                          aj4[u][1].x, aj4[u][1].y, aj4[u][1].z, aj4[u][1].w,
+                                          aj4[u][2].x, aj4[u][2].y, aj4[u][2].z, aj4[u][2].w,
+                                          aj4[u][3].x, aj4[u][3].y, aj4[u][3].z, aj4[u][3].w};
+                float gk[MLP_NK];
+                gaussians(a.d, r, gk);
+                float sdot = 0.f;
+#pragma unroll
+                for (int k = 0; k < MLP_NK; k++)
                 {
-                    fdr += tf;
-                    ep += 0.5f * te;
+                    const float dd = r - (a.d.mu0 + (float)k * a.d.dmu);
+                    sdot += (ai[k] + aj[k]) * gk[k] * (dfc - fc * dd * a.d.invw2);
                 }
-            }
-            fx += fdr * dx;
-            fy += fdr * dy;
-            fz += fdr * dz;
-            if (a.virial)
-            {
-                const float h = 0.5f * fdr;
-                v[0] += h * dx * dx;
-                v[1] += h * dx * dy;
-                v[2] += h * dx * dz;
-                v[3] += h * dy * dy;
-                v[4] += h * dy * dz;
-                v[5] += h * dz * dz;
+                float fdr = -sdot * rinv;
+                if (tw && r >= a.ptab_rmin)
+                {
+                    const float vf = (r - a.ptab_rmin) * t_inv;
+                    unsigned int k = (unsigned int)floorf(vf);
+                    k = k < tw ? k : tw - 1;
+                    const float U0 = tU[k], F0 = tF[k];
+                    const float U1 = k + 1 < tw ? tU[k + 1] : 0.f;
+                    const float F1 = k + 1 < tw ? tF[k + 1] : 0.f;
+                    const float f = vf - (float)k;
+                    fdr += (F0 + f * (F1 - F0)) * rinv;
+                    ep += 0.5f * (U0 + f * (U1 - U0));
+                }
+                fx += fdr * dx;
+                fy += fdr * dy;
+                fz += fdr * dz;
+                if (a.virial)
+                {
+                    const float h = 0.5f * fdr;
+                    v[0] += h * dx * dx;
+                    v[1] += h * dx * dy;
+                    v[2] += h * dx * dz;
+                    v[3] += h * dy * dy;
+                    v[4] += h * dy * dz;
+                    v[5] += h * dz * dz;
+                }
             }
         }
     }
@@ -462,12 +533,13 @@ __global__ __launch_bounds__(256) void mlp_adjoint_kernel(MlpArgs a)
             cutoff(a.d, r, fc, dfc);
             const float4 ej = a.err[j];
             const float de = ((ei.x - ej.x) * dx + (ei.y - ej.y) * dy + (ei.z - ej.z) * dz) / r;
+            float gk[MLP_NK];
+            gaussians(a.d, r, gk);
 #pragma unroll
             for (int k = 0; k < MLP_NK; k++)
             {
                 const float dd = r - (a.d.mu0 + (float)k * a.d.dmu);
-                const float gk = __expf(-dd * dd * a.d.inv2w2);
-                c[k] -= gk * (dfc - fc * dd * a.d.invw2) * de;
+                c[k] -= gk[k] * (dfc - fc * dd * a.d.invw2) * de;
             }
         }
     }
@@ -566,7 +638,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpArgs a)
             f4v v;
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                v[r] = silu_d(T.h1[m][r]) * t1[m][r];
+                v[r] = silu_d_s(T.h1[m][r], T.s1[m][r]) * t1[m][r];
             ta1b[m] = pack4(v);
         }
         bf4 th2b[4];
@@ -583,8 +655,9 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpArgs a)
             {
                 const int o = 16 * m + 4 * g + r;
                 const float h = T.h2[m][r];
-                gw3[m][r] += silu_d(h) * t2[r];
-                th2[r] = S.m.w3[o] * silu_dd(h) * t2[r];
+                const float sg = sigm(h);
+                gw3[m][r] += silu_d_s(h, sg) * t2[r];
+                th2[r] = S.m.w3[o] * silu_dd_s(h, sg) * t2[r];
                 gb2[m][r] += th2[r];
             }
             th2b[m] = pack4(th2);
@@ -602,8 +675,8 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpArgs a)
 #pragma unroll
             for (int r = 0; r < 4; r++)
             {
-                const float h = T.h1[m][r];
-                th1[r] = T.u[m][r] * silu_dd(h) * t1[m][r] + vv[r] * silu_d(h);
+                const float h = T.h1[m][r], sg = T.s1[m][r];
+                th1[r] = T.u[m][r] * silu_dd_s(h, sg) * t1[m][r] + vv[r] * silu_d_s(h, sg);
                 gb1[m][r] += th1[r];
             }
             th1b[m] = pack4(th1);
@@ -913,7 +986,8 @@ __global__ __launch_bounds__(256) void frames_sub_harmonic_kernel(
 
 unsigned int site_blocks(unsigned int N)
 {
-    // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront
+    // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront (more workgroups
+    // measured slower: each stages the 20 KB of weights)
     const unsigned int tiles = (N + 15) / 16;
     const unsigned int want = (tiles + 31) / 32;
     return want < 1 ? 1 : (want > 2048 ? 2048 : want);
@@ -1056,6 +1130,16 @@ MlpDescr make_mlp_descr(const hm_mlp_cg_model& m)
         d.gmean[k] = (float)m.gmean[k];
         d.gscale[k] = (float)m.gscale[k];
     }
+    // recurrence from centre GMID: g_GMID = exp(-b (t - GMID)^2) over t = (r - mu0) / dmu for r in
+    // [0, rcut] must stay above exp(-80) (FP32 normals end near exp(-87)); the ratios then stay
+    // below exp(2 sqrt(80 b)) <= exp(80), and every product is a Gaussian value <= 1
+    const double b = m.dmu * m.dmu / (2.0 * m.width * m.width);
+    const double t0 = -m.mu0 / m.dmu, t1 = (m.r_cut - m.mu0) / m.dmu;
+    const double far = std::max(std::fabs(t0 - GMID), std::fabs(t1 - GMID));
+    d.rebase = (m.dmu > 0.0 && b * far * far <= 80.0 && 2.0 * std::sqrt(80.0 * b) <= 80.0) ? 1 : 0;
+    d.inv_dmu = m.dmu > 0.0 ? (float)(1.0 / m.dmu) : 0.f;
+    d.b = (float)b;
+    d.q = (float)std::exp(-2.0 * b);
     return d;
 }
 } // namespace hm
