@@ -50,8 +50,11 @@ class Operations:
         if isinstance(op, (ParticleSorter, NeighborListBuffer)):
             self.tuners.append(op)
         elif isinstance(op, ThermodynamicQuantities):
+            # a compute reads the state on demand (its virial included): the running system
+            # stays as it is
             self.computes.append(op)
             op._simulation = self._simulation
+            return
         else:
             raise TypeError(f"hoomd_amd.Operations cannot hold {type(op).__name__}")
         self._simulation._invalidate()
@@ -83,9 +86,11 @@ class Operations:
             self.writers.remove(op)
             op.close()
             return
-        for lst in (self.tuners, self.computes):
-            if op in lst:
-                lst.remove(op)
+        if op in self.computes:
+            self.computes.remove(op)
+            return
+        if op in self.tuners:
+            self.tuners.remove(op)
         self._simulation._invalidate()
 
 
