@@ -709,12 +709,14 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     return 0
 
 
-def cg_step(n_chains, steps, warmup, seed):
+def cg_step(n_chains, steps, warmup, seed, mlp_only=0):
     """Profiling mode of the learned-CG step alone: n_chains x 10 CG sites as random walks of
     step 1.7 at site density 0.17 (the 5:1-mapped melt's), the site energy with random
     parameters (init_params(seed) x 0.05) and a repulsive pair table in its force pass, harmonic
     bonds k 2.3 r0 1.74, Langevin kT 1 gamma 5, site mass 5, dt 0.02; relaxed for warmup steps,
-    then steps timed. Prints one JSON line (timesteps/s and the three site-energy kernels)."""
+    then steps timed. Prints one JSON line (timesteps/s and the three site-energy kernels).
+    With mlp_only > 0 it launches only the three site-energy kernels that many times each after
+    the relaxation (outside any graph, for counter passes) and exits."""
     import ctypes as Cc
 
     import hoomd_amd as hoomd
@@ -755,6 +757,13 @@ def cg_step(n_chains, steps, warmup, seed):
     mlp.params = MC.init_params(seed) * 0.05
     log(f"[cg-step] {N} sites, L {L:.3f}: relax {warmup} steps")
     cg.run(warmup)
+    if mlp_only:
+        ms3 = (Cc.c_double * 3)()
+        _lib.check(_lib.lib().hm_sys_time_mlp(cg._system, mlp._force_id, mlp_only, ms3),
+                   "time mlp")
+        log(f"[cg-step] {mlp_only} launches each: densities {ms3[0] * 1e3:.1f} us, site MLP "
+            f"{ms3[1] * 1e3:.1f} us, forces {ms3[2] * 1e3:.1f} us")
+        return 0
     _lib.check(_lib.lib().hm_sys_prime(cg._system, steps), "prime")
     _lib.check(_lib.lib().hm_synchronize())
     t0 = time.perf_counter()
@@ -798,6 +807,9 @@ def main():
     ap.add_argument("--pair-only", type=int, default=0,
                     help="profiling mode: after warmup launch only the pair kernel this many "
                          "times (for rocprofv3 counter passes) and exit")
+    ap.add_argument("--build-only", type=int, default=0,
+                    help="profiling mode: after warmup run this many full cell + list rebuilds "
+                         "outside any graph (for rocprofv3 counter passes) and exit")
     ap.add_argument("--pair-kind", type=int, default=2, choices=(0, 1, 2, 3),
                     help="pair kernel: 0 FP64 cell tiles, 1 FP64 gather, 2 FP32 cell-relative "
                          "gather, 3 FP32 cell tiles")
@@ -829,6 +841,9 @@ def main():
     ap.add_argument("--cg-step", type=int, default=0, metavar="CHAINS",
                     help="profiling mode: the learned-CG step alone on CHAINS x 10 synthetic "
                          "sites (20000 for the 200 000-site system)")
+    ap.add_argument("--mlp-only", type=int, default=0,
+                    help="with --cg-step: only launch the site-energy kernels this many times "
+                         "each after the relaxation (for rocprofv3 counter passes)")
     ap.add_argument("--replicas", type=int, default=0, metavar="R",
                     help="run R batched uneven LJ replicas instead (256 for the batched workload)")
     a = ap.parse_args()
@@ -843,7 +858,7 @@ def main():
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
             return 2
-        return cg_step(a.cg_step, a.steps, a.warmup, a.seed)
+        return cg_step(a.cg_step, a.steps, a.warmup, a.seed, a.mlp_only)
     if a.replicas:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)
@@ -897,6 +912,13 @@ def main():
         _lib.check(_lib.lib().hm_sys_time_pair(sim._system, lj._force_id, a.pair_only,
                                                C.byref(ms), None))
         log(f"[bench] {a.pair_only} pair launches, {ms.value * 1e3:.1f} us each")
+        return 0
+    if a.build_only:
+        import ctypes as C
+        t2 = (C.c_double * 2)()
+        _lib.check(_lib.lib().hm_sys_time_build(sim._system, a.build_only, t2), "time build")
+        log(f"[bench] {a.build_only} full rebuilds: cells {t2[0] * 1e3:.1f} us, list "
+            f"{t2[1] * 1e3:.1f} us each")
         return 0
     pcfg = np.zeros(5, np.uint32)
     _lib.check(_lib.lib().hm_sys_pair_config(sim._system, pcfg.ctypes.data))

@@ -953,14 +953,15 @@ __device__ void nlist_cell(const NlistKernelArgs& a, unsigned int cell,
                             const unsigned int p = cnt + lanes_below(m);
                             if (p < a.nmax)
                             {
-                                // rows are streamed out: keep them from evicting the candidates
+                                // plain stores: the hits of one sweep land in a few 128-byte
+                                // row blocks, which L2 merges into whole lines before they go
+                                // out (streaming stores wrote 3.3x the row bytes to HBM)
                                 if (!a.skip_index_rows)
                                 {
                                     if (a.offset_codes)
-                                        __builtin_nontemporal_store(S.sslot[k],
-                                                                    row + row_store_pos(p));
+                                        row[row_store_pos(p)] = S.sslot[k];
                                     else
-                                        __builtin_nontemporal_store(j, row + p);
+                                        row[p] = j;
                                 }
                                 if (lrow)
                                     lrow[p] = (unsigned short)(base_k + k);
