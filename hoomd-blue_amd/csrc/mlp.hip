@@ -311,7 +311,9 @@ __device__ __forceinline__ void site_eval(const MlpLds& S, const MlpArgs& a, con
     T.dx = dx;
 }
 
-__global__ __launch_bounds__(256) void mlp_site_kernel(MlpArgs a)
+constexpr unsigned int SITE_BLOCK = 1024;  // 16 wavefronts share one staging of the weights
+
+__global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
 {
     __shared__ MlpLds S;
     stage_weights(S, a);
@@ -986,8 +988,9 @@ __global__ __launch_bounds__(256) void frames_sub_harmonic_kernel(
 
 unsigned int site_blocks(unsigned int N)
 {
-    // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront (more workgroups
-    // measured slower: each stages the 20 KB of weights)
+    // 16 wavefronts per workgroup, two tiles of 16 sites per wavefront: 4 wavefronts per SIMD
+    // hide the MFMA and SiLU latencies, and each workgroup stages the 20 KB of weights once for
+    // 32 tiles (more, smaller workgroups measured slower)
     const unsigned int tiles = (N + 15) / 16;
     const unsigned int want = (tiles + 31) / 32;
     return want < 1 ? 1 : (want > 2048 ? 2048 : want);
@@ -1013,7 +1016,7 @@ hipError_t launch_mlp_site(const MlpArgs& a, hipStream_t s)
 {
     if (a.N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(mlp_site_kernel, dim3(site_blocks(a.N)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(mlp_site_kernel, dim3(site_blocks(a.N)), dim3(SITE_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
