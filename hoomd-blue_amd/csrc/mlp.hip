@@ -311,7 +311,7 @@ __device__ __forceinline__ void site_eval(const MlpLds& S, const MlpArgs& a, con
     T.dx = dx;
 }
 
-constexpr unsigned int SITE_BLOCK = 1024;  // 16 wavefronts share one staging of the weights
+constexpr unsigned int SITE_BLOCK = 256;  // 1024 (16 wavefronts per staging) measured 2.4x slower
 
 __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
 {
@@ -988,9 +988,8 @@ __global__ __launch_bounds__(256) void frames_sub_harmonic_kernel(
 
 unsigned int site_blocks(unsigned int N)
 {
-    // 16 wavefronts per workgroup, two tiles of 16 sites per wavefront: 4 wavefronts per SIMD
-    // hide the MFMA and SiLU latencies, and each workgroup stages the 20 KB of weights once for
-    // 32 tiles (more, smaller workgroups measured slower)
+    // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront (more workgroups, or
+    // more wavefronts per workgroup, measured slower)
     const unsigned int tiles = (N + 15) / 16;
     const unsigned int want = (tiles + 31) / 32;
     return want < 1 ? 1 : (want > 2048 ? 2048 : want);
