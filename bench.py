@@ -327,6 +327,11 @@ def kg(steps, warmup, seed, cpu_steps, pushoff=10000):
         s.set_langevin(1.0, 1.0)
         s.set_sorter(200, 256)
         s.run(1)
+        # a bounded sample of about 15 s: probe two steps, then at least cpu_steps
+        t1 = time.perf_counter()
+        s.run(2)
+        per = (time.perf_counter() - t1) / 2
+        cpu_steps = max(cpu_steps, int(15.0 / max(per, 1e-6)))
         t1 = time.perf_counter()
         s.run(cpu_steps)
         cdt = time.perf_counter() - t1
