@@ -858,9 +858,13 @@ struct hm_system
 
     void launch_step()
     {
-        hchk(launch_step_begin(st, ke_partial, n_partial, method == 1 ? 1 : 0, ndof(), dt, tau,
-                               kT, seed, stream),
-             "step begin");
+        // the Bussi factor needs the kinetic energy of every particle before step one: a
+        // step-begin kernel of its own. Without it, step one begins the step and step two ends it
+        const bool bussi = method == 1;
+        if (bussi)
+            hchk(launch_step_begin(st, ke_partial, n_partial, 1, ndof(), dt, tau, kT, seed,
+                                   stream),
+                 "step begin");
         StepOneArgs a1;
         memset(&a1, 0, sizeof(a1));
         a1.N = N;
@@ -872,7 +876,8 @@ struct hm_system
         a1.dt = dt;
         a1.alpha = 1.0;
         a1.st = st;
-        a1.use_state_alpha = method == 1 ? 1 : 0;
+        a1.use_state_alpha = bussi ? 1 : 0;
+        a1.begin = bussi ? nullptr : st;
         bool have_nl = rcut_max > 0.0;
         if (have_nl)
         {
@@ -934,6 +939,7 @@ struct hm_system
         }
         a2.st = st;
         a2.ke_partial = ke_partial;
+        a2.finish = bussi ? nullptr : st;
         hchk(launch_step_two(a2, stream, nullptr), "step two");
     }
 

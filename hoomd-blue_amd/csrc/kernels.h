@@ -419,6 +419,10 @@ struct StepOneArgs
     // fused clearing of the cell occupancy counters
     unsigned int* cell_size;
     unsigned int ncell;
+    // when set, this launch begins the step without a step-begin kernel: every workgroup takes
+    // the current step from begin->next (which no kernel of the step writes before step two)
+    // and one thread stores it in begin->timestep for the kernels that follow
+    StepState* begin;
 };
 hipError_t launch_step_one(const StepOneArgs& a, hipStream_t s);
 
@@ -441,6 +445,9 @@ struct StepTwoArgs
     const StepState* st;
     // FP64 kinetic energy partial sums per block (may be NULL)
     double* ke_partial;
+    // when set (with StepOneArgs::begin), one thread ends the step: finish->next =
+    // finish->timestep + 1; no kernel of the step reads next after step one
+    StepState* finish;
 };
 hipError_t launch_step_two(const StepTwoArgs& a, hipStream_t s, unsigned int* nblocks_out);
 

@@ -2675,6 +2675,8 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
             moved = dx * dx + dy * dy + dz * dz >= a.maxsq;
         }
     }
+    if (a.begin && gtid == 0)
+        a.begin->timestep = a.begin->next;
     if (a.flag_state)
     {
         // NeighborList::needsUpdating (NeighborList.cc:1325-1418) for step t1: no check before
@@ -2682,7 +2684,7 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
         // the buffer is zero or distance checks are off and the period is hit; else the distance
         // check. The conditions are the same on every lane, so the ballot stays uniform.
         const StepState* st = a.flag_state;
-        const unsigned long long t1 = st->timestep + 1;
+        const unsigned long long t1 = (a.begin ? a.begin->next : st->timestep) + 1;
         const bool should = !(t1 < st->last_update + a.check_delay);
         const bool always = a.maxsq < 0.0
                             || (!a.dist_check
@@ -2788,6 +2790,8 @@ __global__ __launch_bounds__(256) void step_two_kernel(StepTwoArgs a)
         if (threadIdx.x == 0)
             a.ke_partial[blockIdx.x] = t;
     }
+    if (a.finish && n == 0)
+        a.finish->next = a.finish->timestep + 1;
 }
 
 hipError_t launch_step_two(const StepTwoArgs& a, hipStream_t s, unsigned int* nblocks_out)

@@ -24,27 +24,23 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int PAIR_TPP = 4;  // lanes per site in the pair passes
 constexpr int GMID = MLP_NK / 2;  // base centre of the Gaussian recurrence
 
+typedef __bf16 hb4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> bf16 by the hardware conversion (v_cvt_pk_bf16_f32, round to nearest even)
 __device__ __forceinline__ unsigned short f2bf(float f)
 {
-    // round to nearest even (operands are finite)
-    unsigned int u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ bf4 pack4(float a, float b, float c, float d)
-{
-    bf4 v;
-    v[0] = (short)f2bf(a);
-    v[1] = (short)f2bf(b);
-    v[2] = (short)f2bf(c);
-    v[3] = (short)f2bf(d);
-    return v;
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 
 __device__ __forceinline__ bf4 pack4(const f4v& x)
 {
-    return pack4(x[0], x[1], x[2], x[3]);
+    return __builtin_bit_cast(bf4, __builtin_convertvector(x, hb4));
+}
+
+__device__ __forceinline__ bf4 pack4(float a, float b, float c, float d)
+{
+    f4v x = {a, b, c, d};
+    return pack4(x);
 }
 
 __device__ __forceinline__ f4v mfma(bf4 a, bf4 b, f4v c)
@@ -95,11 +91,12 @@ __device__ __forceinline__ void pair_vec(const DevBox& box, const double4& pi, c
     fz = (float)dz;
 }
 
-// cosine cutoff f_c(r) = (cos(pi r / rc) + 1) / 2 and its derivative
+// cosine cutoff f_c(r) = (cos(pi r / rc) + 1) / 2 and its derivative. The hardware sine and
+// cosine take their argument in turns (v_sin_f32: sin(2 pi x)); x = r / (2 rc) lies in [0, 1/2].
 __device__ __forceinline__ void cutoff(const MlpDescr& d, float r, float& fc, float& dfc)
 {
-    float sn, cs;
-    sincospif(r * d.inv_rcut, &sn, &cs);
+    const float x = 0.5f * r * d.inv_rcut;
+    const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
     fc = 0.5f * (cs + 1.0f);
     dfc = -0.5f * 3.14159265358979f * d.inv_rcut * sn;
 }
@@ -140,6 +137,54 @@ __device__ __forceinline__ void gaussians(const MlpDescr& d, float r, float (&g)
     }
 }
 
+// sum_k c_k phi_k'(r) with c_k = ai_k + aj_k, phi_k = g_k f_c, phi_k' = g_k (dfc - fc (r - mu_k)
+// / w^2). With the recurrence the Gaussians are consumed as they are formed:
+// S0 = sum c_k g_k, Sm = sum (k - GMID) c_k g_k and sum c_k g_k (r - mu_k) = dmu (s S0 - Sm),
+// s = (r - mu0) / dmu - GMID, so no array of sixteen values is held per pair.
+__device__ __forceinline__ float basis_deriv_dot(const MlpDescr& d, float r, float fc, float dfc,
+                                                 const float (&ai)[MLP_NK],
+                                                 const float (&aj)[MLP_NK])
+{
+    if (d.rebase)
+    {
+        const float s = (r - d.mu0) * d.inv_dmu - (float)GMID;
+        const float g0 = __expf(-d.b * s * s);
+        float up = __expf(2.f * d.b * s - d.b);
+        float dn = __expf(-2.f * d.b * s - d.b);
+        float S0 = (ai[GMID] + aj[GMID]) * g0, Sm = 0.f;
+        float g = g0;
+#pragma unroll
+        for (int k = GMID + 1; k < MLP_NK; k++)
+        {
+            g *= up;
+            up *= d.q;
+            const float t = (ai[k] + aj[k]) * g;
+            S0 += t;
+            Sm += (float)(k - GMID) * t;
+        }
+        g = g0;
+#pragma unroll
+        for (int k = GMID - 1; k >= 0; k--)
+        {
+            g *= dn;
+            dn *= d.q;
+            const float t = (ai[k] + aj[k]) * g;
+            S0 += t;
+            Sm += (float)(k - GMID) * t;
+        }
+        return dfc * S0 - fc * d.invw2 * d.dmu * (s * S0 - Sm);
+    }
+    float sdot = 0.f;
+#pragma unroll
+    for (int k = 0; k < MLP_NK; k++)
+    {
+        const float dd = r - (d.mu0 + (float)k * d.dmu);
+        const float g = __expf(-dd * dd * d.inv2w2);
+        sdot += (ai[k] + aj[k]) * g * (dfc - fc * dd * d.invw2);
+    }
+    return sdot;
+}
+
 template<int T> __device__ __forceinline__ float group_sum(float v)
 {
 #pragma unroll
@@ -149,7 +194,7 @@ template<int T> __device__ __forceinline__ float group_sum(float v)
 }
 
 // ------------------------------------------------------------------------------ pass 1
-__global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
+template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
 {
     const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
@@ -163,22 +208,48 @@ __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
         const double4 pi = a.pos[i];
         const unsigned int n = min(a.n_neigh[i], a.nmax);
         const unsigned int* row = a.nlist + (size_t)i * a.nmax;
-        for (unsigned int q = sub; q < n; q += PAIR_TPP)
-        {
-            const unsigned int j = row[q];
-            float dx, dy, dz;
-            pair_vec(a.box, pi, a.pos[j], dx, dy, dz);
-            const float rsq = dx * dx + dy * dy + dz * dz;
-            if (!(rsq < a.d.rcutsq))
-                continue;
-            const float r = sqrtf(rsq);
-            float fc, dfc;
-            cutoff(a.d, r, fc, dfc);
-            float gk[MLP_NK];
-            gaussians(a.d, r, gk);
+        // two neighbours per iteration, and the row entries of the next iteration loaded one
+        // iteration ahead, so the list loads and the position gathers overlap
+        unsigned int jn[U];
 #pragma unroll
-            for (int k = 0; k < MLP_NK; k++)
-                acc[k] += gk[k] * fc;
+        for (int u = 0; u < U; u++)
+        {
+            const unsigned int q = sub + u * PAIR_TPP;
+            jn[u] = q < n ? row[q] : i;
+        }
+        for (unsigned int q0 = sub; q0 < n; q0 += PAIR_TPP * U)
+        {
+            double4 pj[U];
+            bool live[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                live[u] = q0 + u * PAIR_TPP < n;
+                pj[u] = a.pos[jn[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                const unsigned int q = q0 + (U + u) * PAIR_TPP;
+                jn[u] = q < n ? row[q] : i;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                float dx, dy, dz;
+                pair_vec(a.box, pi, pj[u], dx, dy, dz);
+                const float rsq = dx * dx + dy * dy + dz * dz;
+                if (!live[u] || !(rsq < a.d.rcutsq))
+                    continue;
+                const float r = rsq * __builtin_amdgcn_rsqf(rsq);
+                float fc, dfc;
+                cutoff(a.d, r, fc, dfc);
+                float gk[MLP_NK];
+                gaussians(a.d, r, gk);
+#pragma unroll
+                for (int k = 0; k < MLP_NK; k++)
+                    acc[k] += gk[k] * fc;
+            }
         }
     }
 #pragma unroll
@@ -316,18 +387,26 @@ constexpr unsigned int SITE_BLOCK = 256;  // 1024 (16 wavefronts per staging) me
 __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
 {
     __shared__ MlpLds S;
-    stage_weights(S, a);
     const int lane = threadIdx.x & 63;
     const int s = lane & 15, g = lane >> 4;
-    const float b3 = a.master[MP_B3];
     const unsigned int ntiles = (a.N + 15) / 16;
     const unsigned int nwaves = gridDim.x * (blockDim.x / 64);
-    for (unsigned int t = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; t < ntiles;
-         t += nwaves)
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the descriptor of the wavefront's next tile is loaded one tile ahead (the first one
+    // while the weights are staged), so each tile's load latency hides behind the previous
+    // tile's matrix work
+    unsigned int t = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    float4 gnext = (t < ntiles && 16 * t + s < a.N) ? a.G[(size_t)(16 * t + s) * 4 + g] : zero;
+    stage_weights(S, a);
+    const float b3 = a.master[MP_B3];
+    for (; t < ntiles; t += nwaves)
     {
         const unsigned int site = 16 * t + s;
         const bool valid = site < a.N;
-        const float4 gv = valid ? a.G[(size_t)site * 4 + g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 gv = gnext;
+        const unsigned int tn = t + nwaves;
+        if (tn < ntiles)
+            gnext = 16 * tn + s < a.N ? a.G[(size_t)(16 * tn + s) * 4 + g] : zero;
         SiteTile T;
         site_eval(S, a, gv, s, g, b3, T);
         if (valid)
@@ -348,7 +427,7 @@ __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
 // Tables up to this many points are staged in LDS for the force pass
 constexpr unsigned int MLP_TAB_LDS = 2048;
 
-template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
+template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
 {
     __shared__ float stab[2 * MLP_TAB_LDS];
     const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,7 +465,14 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
         const unsigned int* row = a.nlist + (size_t)i * a.nmax;
         // two neighbours per iteration: both position and both dE/dG gathers are issued before
         // the arithmetic of the first
-        constexpr int U = 2;
+        // the row entries of the next iteration are loaded one iteration ahead
+        unsigned int jn[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const unsigned int q = sub + u * PAIR_TPP;
+            jn[u] = q < n ? row[q] : i;
+        }
         for (unsigned int q0 = sub; q0 < n; q0 += PAIR_TPP * U)
         {
             unsigned int jj[U];
@@ -394,9 +480,8 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
-                const unsigned int q = q0 + u * PAIR_TPP;
-                live[u] = q < n;
-                jj[u] = live[u] ? row[q] : i;
+                live[u] = q0 + u * PAIR_TPP < n;
+                jj[u] = jn[u];
             }
             double4 pj[U];
             float4 aj4[U][4];
@@ -411,28 +496,28 @@ template<bool TRAIN> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
+                const unsigned int q = q0 + (U + u) * PAIR_TPP;
+                jn[u] = q < n ? row[q] : i;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
                 float dx, dy, dz;
                 pair_vec(a.box, pi, pj[u], dx, dy, dz);
                 const float rsq = dx * dx + dy * dy + dz * dz;
                 if (!live[u] || !(rsq < a.d.rcutsq))
                     continue;
-                const float r = sqrtf(rsq);
-                const float rinv = __builtin_amdgcn_rcpf(r);
+                // r and 1/r from the hardware reciprocal square root (1 ulp): the correctly
+                // rounded sqrtf expands to a dozen instructions per pair
+                const float rinv = __builtin_amdgcn_rsqf(rsq);
+                const float r = rsq * rinv;
                 float fc, dfc;
                 cutoff(a.d, r, fc, dfc);
                 const float aj[MLP_NK] = {aj4[u][0].x, aj4[u][0].y, aj4[u][0].z, aj4[u][0].w,
                                           aj4[u][1].x, aj4[u][1].y, aj4[u][1].z, aj4[u][1].w,
                                           aj4[u][2].x, aj4[u][2].y, aj4[u][2].z, aj4[u][2].w,
                                           aj4[u][3].x, aj4[u][3].y, aj4[u][3].z, aj4[u][3].w};
-                float gk[MLP_NK];
-                gaussians(a.d, r, gk);
-                float sdot = 0.f;
-#pragma unroll
-                for (int k = 0; k < MLP_NK; k++)
-                {
-                    const float dd = r - (a.d.mu0 + (float)k * a.d.dmu);
-                    sdot += (ai[k] + aj[k]) * gk[k] * (dfc - fc * dd * a.d.invw2);
-                }
+                const float sdot = basis_deriv_dot(a.d, r, fc, dfc, ai, aj);
                 float fdr = -sdot * rinv;
                 if (tw && r >= a.ptab_rmin)
                 {
@@ -988,11 +1073,25 @@ __global__ __launch_bounds__(256) void frames_sub_harmonic_kernel(
 
 unsigned int site_blocks(unsigned int N)
 {
-    // 4 wavefronts per workgroup, up to 8 tiles of 16 sites per wavefront (more workgroups, or
-    // more wavefronts per workgroup, measured slower)
+    // one workgroup of 4 wavefronts per CU, each wavefront walking its tiles of 16 sites: the
+    // kernel's time is its matrix and SiLU work per SIMD, which this spreads evenly (a grid of
+    // 1.5 wavefronts per SIMD measured 15 % slower, and every extra workgroup stages the weights
+    // again); never more wavefronts than tiles
+    static int cus[16] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int& c = cus[dev & 15];
+    if (c <= 0)
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
+            || v <= 0)
+            v = 256;
+        c = v;
+    }
     const unsigned int tiles = (N + 15) / 16;
-    const unsigned int want = (tiles + 31) / 32;
-    return want < 1 ? 1 : (want > 2048 ? 2048 : want);
+    const unsigned int want = (tiles + 3) / 4;
+    return want < 1 ? 1 : (want > (unsigned int)c ? (unsigned int)c : want);
 }
 
 } // namespace
@@ -1006,8 +1105,10 @@ hipError_t launch_mlp_descriptor(const MlpArgs& a, hipStream_t s)
 {
     if (a.N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(mlp_descriptor_kernel, dim3(mlp_pair_blocks(a.N)), dim3(MLP_PAIR_BLOCK), 0,
-                       s, a);
+    // two neighbours per iteration in flight (one: 44 us at 200 000 sites; three or four: no
+    // faster than two)
+    hipLaunchKernelGGL((mlp_descriptor_kernel<2>), dim3(mlp_pair_blocks(a.N)),
+                       dim3(MLP_PAIR_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1023,11 +1124,12 @@ hipError_t launch_mlp_force(const MlpArgs& a, bool train, hipStream_t s)
 {
     if (a.N == 0)
         return hipSuccess;
+    // two neighbours per iteration in flight (three: 141 VGPRs, 3 wavefronts per SIMD, slower)
     if (train)
-        hipLaunchKernelGGL(mlp_force_kernel<true>, dim3(mlp_pair_blocks(a.N)),
+        hipLaunchKernelGGL((mlp_force_kernel<true, 2>), dim3(mlp_pair_blocks(a.N)),
                            dim3(MLP_PAIR_BLOCK), 0, s, a);
     else
-        hipLaunchKernelGGL(mlp_force_kernel<false>, dim3(mlp_pair_blocks(a.N)),
+        hipLaunchKernelGGL((mlp_force_kernel<false, 2>), dim3(mlp_pair_blocks(a.N)),
                            dim3(MLP_PAIR_BLOCK), 0, s, a);
     return hipGetLastError();
 }
