@@ -196,7 +196,9 @@ template<int T> __device__ __forceinline__ float group_sum(float v)
 // ------------------------------------------------------------------------------ pass 1
 template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
 {
-    const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
+    // so the neighbour gathers of one XCD stay in its own L2
+    const unsigned int gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
     const bool valid = i < a.N;
     float acc[MLP_NK];
@@ -430,7 +432,9 @@ constexpr unsigned int MLP_TAB_LDS = 2048;
 template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
 {
     __shared__ float stab[2 * MLP_TAB_LDS];
-    const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
+    // so the neighbour gathers of one XCD stay in its own L2
+    const unsigned int gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
     const bool valid = i < a.N;
     float fx = 0.f, fy = 0.f, fz = 0.f, ep = 0.f;
@@ -594,7 +598,9 @@ template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void ml
 // ------------------------------------------------------------------------------ adjoint pass
 __global__ __launch_bounds__(256) void mlp_adjoint_kernel(MlpArgs a)
 {
-    const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
+    // so the neighbour gathers of one XCD stay in its own L2
+    const unsigned int gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int i = gid / PAIR_TPP, sub = gid % PAIR_TPP;
     const bool valid = i < a.N;
     float c[MLP_NK];
@@ -1098,7 +1104,9 @@ unsigned int site_blocks(unsigned int N)
 
 unsigned int mlp_pair_blocks(unsigned int N)
 {
-    return (unsigned int)(((size_t)N * PAIR_TPP + MLP_PAIR_BLOCK - 1) / MLP_PAIR_BLOCK);
+    // a multiple of 8 for the XCD-aware block order
+    return round_up((unsigned int)(((size_t)N * PAIR_TPP + MLP_PAIR_BLOCK - 1) / MLP_PAIR_BLOCK),
+                    8u);
 }
 
 hipError_t launch_mlp_descriptor(const MlpArgs& a, hipStream_t s)
