@@ -97,6 +97,7 @@ struct ForceSlot
     float* mparams = nullptr;
     uint16_t* mbf16 = nullptr;
     float4 *mG = nullptr, *mA = nullptr;
+    uint2* mAb = nullptr;  // bf16 copy of mA for the force pass
     float* mE = nullptr;
     float4 *mT = nullptr, *mErr = nullptr, *mC = nullptr;
     float *mLoss = nullptr, *mGpart = nullptr, *mM1 = nullptr, *mM2 = nullptr, *mGrad = nullptr;
@@ -284,6 +285,7 @@ struct hm_system
             dfree(f.mbf16);
             dfree(f.mG);
             dfree(f.mA);
+            dfree(f.mAb);
             dfree(f.mE);
             dfree(f.mT);
             dfree(f.mErr);
@@ -780,6 +782,7 @@ struct hm_system
         a.wb = f.mbf16;
         a.G = f.mG;
         a.A = f.mA;
+        a.Ab = f.mAb;
         a.Esite = f.mE;
         a.force = f.f;
         a.virial = with_virial ? f.v : nullptr;
@@ -2340,6 +2343,10 @@ void mlp_frame_pass(hm_system* s, ForceSlot& f, const float* fpos, const float* 
     if (s->rcut_max > 0.0)
         s->build_nlist_now();
     MlpArgs a = s->mlp_args(f, 0);
+    // training reads the FP32 dE/dG: the loss and its gradient are those of the FP32 forces
+    // (the bf16 copy of the production force pass moves forces by about 1e-3 of their size,
+    // far below the force-matching residual)
+    a.Ab = nullptr;
     hchk(launch_mlp_descriptor(a, s->stream), "mlp descriptor");
     hchk(launch_mlp_site(a, s->stream), "mlp site energies");
     hchk(launch_mlp_force(a, true, s->stream), "mlp forces");
@@ -2366,6 +2373,7 @@ int hm_sys_add_mlp_cg(hm_system* s, const hm_mlp_cg_model* model, const float* p
         f.mbf16 = dalloc<uint16_t>(MLP_NBF16);
         f.mG = dalloc<float4>((size_t)4 * N);
         f.mA = dalloc<float4>((size_t)4 * N);
+        f.mAb = dalloc<uint2>((size_t)4 * N);
         f.mE = dalloc<float>(N);
         hchk(hipMemset(f.f, 0, sizeof(float4) * N), "clear");
         hchk(hipMemset(f.v, 0, sizeof(float) * 6 * N), "clear");

@@ -78,6 +78,9 @@ struct MlpArgs
     float* loss_partial;           // [ceil(N * tpp / MLP_PAIR_BLOCK)]
     float* gpart;                  // [MLP_GRAD_BLOCKS][MLP_NPARAM]
     float loss_scale;              // 2 / (3N)
+    // bf16 copy of A (4 values per uint2, 16 per site) written by the site kernel and gathered
+    // by the force pass for both partners, or NULL (the force pass then reads A)
+    uint2* Ab;
 };
 
 MlpDescr make_mlp_descr(const hm_mlp_cg_model& m);

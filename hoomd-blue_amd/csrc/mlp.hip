@@ -419,6 +419,11 @@ __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
             o.z = a.d.gscale[4 * g + 2] * T.dx[2];
             o.w = a.d.gscale[4 * g + 3] * T.dx[3];
             a.A[(size_t)site * 4 + g] = o;
+            if (a.Ab)
+            {
+                const bf4 ob = pack4(o.x, o.y, o.z, o.w);
+                a.Ab[(size_t)site * 4 + g] = __builtin_bit_cast(uint2, ob);
+            }
             if (g == 0)
                 a.Esite[site] = T.e;
         }
@@ -429,7 +434,52 @@ __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
 // Tables up to this many points are staged in LDS for the force pass
 constexpr unsigned int MLP_TAB_LDS = 2048;
 
-template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
+// the 16 dE/dG values of site j: FP32 (four 16-byte loads) or the bf16 copy (two)
+template<bool BF> struct AjLoad;
+template<> struct AjLoad<false>
+{
+    float4 v[4];
+    __device__ __forceinline__ void load(const MlpArgs& a, unsigned int j)
+    {
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            v[w] = a.A[(size_t)j * 4 + w];
+    }
+    __device__ __forceinline__ void get(float (&o)[MLP_NK]) const
+    {
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+        {
+            o[4 * w] = v[w].x;
+            o[4 * w + 1] = v[w].y;
+            o[4 * w + 2] = v[w].z;
+            o[4 * w + 3] = v[w].w;
+        }
+    }
+};
+template<> struct AjLoad<true>
+{
+    uint4 v[2];
+    __device__ __forceinline__ void load(const MlpArgs& a, unsigned int j)
+    {
+        const uint4* p = (const uint4*)a.Ab + (size_t)j * 2;
+        v[0] = p[0];
+        v[1] = p[1];
+    }
+    __device__ __forceinline__ void get(float (&o)[MLP_NK]) const
+    {
+        const unsigned int u[8] = {v[0].x, v[0].y, v[0].z, v[0].w,
+                                   v[1].x, v[1].y, v[1].z, v[1].w};
+#pragma unroll
+        for (int h = 0; h < 8; h++)
+        {
+            o[2 * h] = __uint_as_float(u[h] << 16);
+            o[2 * h + 1] = __uint_as_float(u[h] & 0xffff0000u);
+        }
+    }
+};
+
+template<bool TRAIN, int U, bool BF> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
 {
     __shared__ float stab[2 * MLP_TAB_LDS];
     // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
@@ -455,14 +505,10 @@ template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void ml
     if (valid)
     {
         float ai[MLP_NK];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
         {
-            const float4 t = a.A[(size_t)i * 4 + q];
-            ai[4 * q] = t.x;
-            ai[4 * q + 1] = t.y;
-            ai[4 * q + 2] = t.z;
-            ai[4 * q + 3] = t.w;
+            AjLoad<BF> li;
+            li.load(a, i);
+            li.get(ai);
         }
         const double4 pi = a.pos[i];
         const unsigned int n = min(a.n_neigh[i], a.nmax);
@@ -488,14 +534,12 @@ template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void ml
                 jj[u] = jn[u];
             }
             double4 pj[U];
-            float4 aj4[U][4];
+            AjLoad<BF> lj[U];
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
                 pj[u] = a.pos[jj[u]];
-#pragma unroll
-                for (int w = 0; w < 4; w++)
-                    aj4[u][w] = a.A[(size_t)jj[u] * 4 + w];
+                lj[u].load(a, jj[u]);
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -517,10 +561,8 @@ template<bool TRAIN, int U> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void ml
                 const float r = rsq * rinv;
                 float fc, dfc;
                 cutoff(a.d, r, fc, dfc);
-                const float aj[MLP_NK] = {aj4[u][0].x, aj4[u][0].y, aj4[u][0].z, aj4[u][0].w,
-                                          aj4[u][1].x, aj4[u][1].y, aj4[u][1].z, aj4[u][1].w,
-                                          aj4[u][2].x, aj4[u][2].y, aj4[u][2].z, aj4[u][2].w,
-                                          aj4[u][3].x, aj4[u][3].y, aj4[u][3].z, aj4[u][3].w};
+                float aj[MLP_NK];
+                lj[u].get(aj);
                 const float sdot = basis_deriv_dot(a.d, r, fc, dfc, ai, aj);
                 float fdr = -sdot * rinv;
                 if (tw && r >= a.ptab_rmin)
@@ -1133,12 +1175,15 @@ hipError_t launch_mlp_force(const MlpArgs& a, bool train, hipStream_t s)
     if (a.N == 0)
         return hipSuccess;
     // two neighbours per iteration in flight (three: 141 VGPRs, 3 wavefronts per SIMD, slower)
-    if (train)
-        hipLaunchKernelGGL((mlp_force_kernel<true, 2>), dim3(mlp_pair_blocks(a.N)),
-                           dim3(MLP_PAIR_BLOCK), 0, s, a);
+    const dim3 g(mlp_pair_blocks(a.N)), b(MLP_PAIR_BLOCK);
+    if (train && a.Ab)
+        hipLaunchKernelGGL((mlp_force_kernel<true, 2, true>), g, b, 0, s, a);
+    else if (train)
+        hipLaunchKernelGGL((mlp_force_kernel<true, 2, false>), g, b, 0, s, a);
+    else if (a.Ab)
+        hipLaunchKernelGGL((mlp_force_kernel<false, 2, true>), g, b, 0, s, a);
     else
-        hipLaunchKernelGGL((mlp_force_kernel<false, 2>), dim3(mlp_pair_blocks(a.N)),
-                           dim3(MLP_PAIR_BLOCK), 0, s, a);
+        hipLaunchKernelGGL((mlp_force_kernel<false, 2, false>), g, b, 0, s, a);
     return hipGetLastError();
 }
 
@@ -1290,6 +1335,7 @@ MlpArgs to_args(const hm_mlp_cg_args* x, const hm_mlp_cg_model* m)
     a.loss_partial = x->loss_partial;
     a.gpart = x->grad_partial;
     a.loss_scale = x->N ? (float)(2.0 / (3.0 * (double)x->N)) : 0.f;
+    a.Ab = (uint2*)x->A_bf16;
     return a;
 }
 } // namespace

@@ -25,7 +25,9 @@
  * Everything is FP64. With bf16 = 1 the operands of every matrix product are rounded to
  * bfloat16 (round to nearest even, after rounding to FP32) at the same points as the GPU
  * kernels, which multiply bf16 operands and accumulate in FP32; FP32 rounding of the other
- * intermediate values is not emulated.
+ * intermediate values is not emulated. With bf16 = 2 the force pass (and so the loss) also
+ * reads a(i) rounded to bfloat16, as the GPU force pass does when it gathers the bf16 copy
+ * of dE/dG; the a(i) returned stay unrounded.
  *
  * Pinning (tests/test_oracle_mlp.py): forces against the finite-difference gradient of the
  * total energy and against torch autograd of the same energy; parameter gradients against
@@ -224,11 +226,12 @@ double ocg_forces(const ocg_model* m, const double* params, const double* L, con
     memset(F, 0, sizeof(double) * 3 * N);
     if (virial)
         memset(virial, 0, sizeof(double) * 6 * N);
+    const int ra = m->bf16 >= 2;
     FOR_PAIRS({
         (void)phi;
         double s = 0.0;
         for (int k = 0; k < OCG_NK; k++)
-            s += (a[i * OCG_NK + k] + a[j * OCG_NK + k]) * dphi[k];
+            s += (rnd(ra, a[i * OCG_NK + k]) + rnd(ra, a[j * OCG_NK + k])) * dphi[k];
         /* f_ij = -s r_hat; the virial of i takes half of each pair, d_a f_b */
         const double fdr = -s / r;
         for (int c = 0; c < 3; c++)

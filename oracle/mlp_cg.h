@@ -26,7 +26,8 @@ typedef struct
 {
     double rcut, width, mu0, dmu;
     double gmean[OCG_NK], gscale[OCG_NK];
-    int bf16; /* 1: round matrix-product operands to bfloat16 as the GPU kernels do */
+    int bf16; /* 1: round matrix-product operands to bfloat16 as the GPU kernels do; 2: also
+                 round dE/dG where the force pass reads it (the GPU's bf16 copy) */
 } ocg_model;
 
 void ocg_basis(const ocg_model* m, double r, double* phi, double* dphi);

@@ -49,7 +49,8 @@ def _sigs():
 
 class Model:
     """The descriptor constants and normalisation: centres mu0 + k dmu, Gaussian width, cosine
-    cutoff r_cut; x = (G - gmean) * gscale. bf16 rounds matrix-product operands as the GPU."""
+    cutoff r_cut; x = (G - gmean) * gscale. bf16 = 1 (or True) rounds matrix-product operands as
+    the GPU; bf16 = 2 also rounds dE/dG where the force pass reads it (the GPU's bf16 copy)."""
 
     def __init__(self, rcut=3.0, width=0.2, mu0=0.0, dmu=0.2, gmean=None, gscale=None,
                  bf16=False):
@@ -67,8 +68,9 @@ class Model:
         return C.cast(C.pointer(self.s), C.c_void_p)
 
     def with_bf16(self, on=True):
+        """The same model with bf16 mode on (1 or True, or 2: see the class) or off."""
         m = Model(self.s.rcut, self.s.width, self.s.mu0, self.s.dmu, list(self.s.gmean),
-                  list(self.s.gscale), on)
+                  list(self.s.gscale), int(on))
         return m
 
 

@@ -58,7 +58,9 @@ def cg_sim(hoomd, pos, L, bonds=None, params=None, gmean=None, gscale=None, excl
 def stats_model(pos, L, nl, nn):
     G = M.descriptor(M.Model(), L, pos, nl, nn)
     gmean, gscale = G.mean(axis=0), 1.0 / (G.std(axis=0) + 1e-3)
-    return M.Model(gmean=gmean, gscale=gscale, bf16=True), gmean, gscale
+    # the engine's production force pass gathers the bf16 copy of dE/dG (oracle bf16 mode 2);
+    # its training reads the FP32 dE/dG (mode 1)
+    return M.Model(gmean=gmean, gscale=gscale, bf16=2), gmean, gscale
 
 
 def test_engine_forces_match_oracle(hoomd):
@@ -143,7 +145,8 @@ def test_training_teacher_student(hoomd):
     p2 = confs[2][0]
     nl, nn = O.nlist_binned(O.Box(L), p2, 3.0, 0.0, full=True)
     T = fs.forces(2).astype(np.float64)
-    l_or, g_or, _ = M.loss_grad(om, pt, L, fs.positions(2).astype(np.float64), nl, nn, T)
+    l_or, g_or, _ = M.loss_grad(om.with_bf16(1), pt, L, fs.positions(2).astype(np.float64), nl,
+                                nn, T)
     print(f"frame 2: loss {l_gpu:.6e} vs oracle {l_or:.6e}, grad rel RMS "
           f"{rel_rms(g_gpu, g_or):.2e}", flush=True)
     # after training the residual F - T is ~4 % of F, so the kernels' force differences from

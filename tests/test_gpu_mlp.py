@@ -67,7 +67,7 @@ def cg_config(N, rho=0.17, seed=1, rmin=0.8):
 class Setup:
     """Buffers of hm_mlp_cg_args for one configuration, its oracle list and a model."""
 
-    def __init__(self, H, N, seed, params_seed):
+    def __init__(self, H, N, seed, params_seed, bf16_a=False):
         self.H = H
         pos, L = cg_config(N, seed=seed)
         self.pos, self.L, self.N = pos, L, N
@@ -109,6 +109,9 @@ class Setup:
         a.G, a.A, a.Esite, a.force, a.virial = (self.d_G.ptr, self.d_A.ptr, self.d_E.ptr,
                                                 self.d_F.ptr, self.d_V.ptr)
         a.target, a.err, a.C = self.d_T.ptr, self.d_err.ptr, self.d_C.ptr
+        if bf16_a:
+            self.d_Ab = H.DeviceArray((16 * N,), np.uint16, fill=0)
+            a.A_bf16 = self.d_Ab.ptr
         a.loss_partial, a.grad_partial = self.d_lp.ptr, self.d_gp.ptr
         self.args = a
         H.check(H.lib().hm_mlp_cg_pack(self.d_par.ptr, self.d_bf.ptr, None), "pack")
@@ -145,6 +148,41 @@ def test_mlp_forward_matches_oracle(H, N, seed):
         assert errs[k] < 3e-2, k
     np.testing.assert_array_equal(F[:, 3], E)
     # third law: the forces sum to zero
+    assert np.abs(F[:, :3].astype(np.float64).sum(axis=0)).max() < 1e-3 * np.abs(F[:, :3]).max()
+
+
+def bf16_rne(x):
+    """FP32 -> bf16 bits, round to nearest even (finite values)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("N,seed", [(3000, 4), (333, 5)])
+def test_mlp_bf16_dEdG_forces_match_oracle(H, N, seed):
+    """The force pass on the bf16 copy of dE/dG: the copy is A rounded to nearest even, bit for
+    bit, and the forces follow the oracle that rounds a(i) in pass 2 (bf16 mode 2)."""
+    print(f"mlp forward, bf16 dE/dG: N={N} seed={seed}", flush=True)
+    S = Setup(H, N, seed, params_seed=seed + 10, bf16_a=True)
+    S.forward()
+    A = S.d_A.numpy()
+    np.testing.assert_array_equal(S.d_Ab.numpy(), bf16_rne(A))
+    F = S.d_F.numpy().reshape(N, 4)
+    E = S.d_E.numpy()
+    V = S.d_V.numpy().reshape(6, N)
+    om2 = S.om.with_bf16(2)
+    _, oF, oE, oV, oA = M.forces(om2, S.p64, S.L, S.pos, S.nl, S.nn)
+    _, fF, _, _, _ = M.forces(S.om, S.p64, S.L, S.pos, S.nl, S.nn)
+    _, bF, _, _, _ = M.forces(S.omb, S.p64, S.L, S.pos, S.nl, S.nn)
+    errs = dict(F=rel_rms(F[:, :3], oF), V=rel_rms(V, oV), E=rel_rms(E, oE),
+                A=rel_rms(A.reshape(N, 16), oA), F64=rel_rms(F[:, :3], fF),
+                F_vs_mode1=rel_rms(F[:, :3], bF))
+    print("rel RMS:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k in ("F", "V", "E", "A"):
+        assert errs[k] < 5e-4, k
+    assert errs["F64"] < 3e-2
+    # the oracle's mode 2 is what the kernel computes: closer than mode 1 (FP32 dE/dG)
+    assert errs["F"] < errs["F_vs_mode1"]
+    # pairs use the same rounded values on both sides: the forces still sum to zero
     assert np.abs(F[:, :3].astype(np.float64).sum(axis=0)).max() < 1e-3 * np.abs(F[:, :3]).max()
 
 

@@ -145,6 +145,39 @@ def test_two_site_closed_form():
         [rhat[0] ** 2, rhat[0] * rhat[1], 0, rhat[1] ** 2, 0, 0]) * r * r, atol=1e-14)
 
 
+def test_bf16_mode2_rounds_dEdG_in_the_force_pass():
+    """bf16 mode 2 (the GPU force pass on the bf16 copy of dE/dG): on the two-site closed form
+    the pair force uses bf16(a) of both sites; site energies and the returned dE/dG are those
+    of mode 1; the forces move away from mode 1 by the rounding and stay close to FP64."""
+    L = 20.0
+    r = 1.7
+    pos = np.array([[0.0, 0.0, 0.0], [r * 0.6, r * 0.8, 0.0]])
+    m = M.Model(gmean=np.full(16, 0.1), gscale=np.full(16, 2.0))
+    p = M.init_params(5)
+    nl = np.array([[1], [0]], np.uint32)
+    nn = np.array([1, 1], np.uint32)
+    _, F1, E1, _, A1 = M.forces(m.with_bf16(1), p, L, pos, nl, nn)
+    Et, F2, E2, _, A2 = M.forces(m.with_bf16(2), p, L, pos, nl, nn)
+    np.testing.assert_array_equal(E2, E1)
+    np.testing.assert_array_equal(A2, A1)
+    _, dphi = M.basis(m, r)
+    ab = np.array([M.bf16_round(v) for v in A1[0]])
+    assert not np.array_equal(ab, A1[0])  # the rounding is visible at this size
+    rhat = (pos[0] - pos[1]) / r
+    np.testing.assert_allclose(F2[0], -2.0 * (ab @ dphi) * rhat, rtol=1e-12)
+    np.testing.assert_allclose(F2[1], -F2[0], rtol=1e-14)
+    # a liquid-like configuration: within bf16 rounding of mode 1 and within 3e-2 of FP64
+    pos, L = cg_config(140, seed=8)
+    m, nl, nn = model_for(pos, L)
+    pr = M.init_params(6)
+    _, F, _, _, _ = M.forces(m, pr, L, pos, nl, nn)
+    _, Fb, _, _, _ = M.forces(m.with_bf16(1), pr, L, pos, nl, nn)
+    _, Fb2, _, _, _ = M.forces(m.with_bf16(2), pr, L, pos, nl, nn)
+    d12 = np.sqrt(((Fb2 - Fb) ** 2).mean() / (Fb ** 2).mean())
+    d64 = np.sqrt(((Fb2 - F) ** 2).mean() / (F ** 2).mean())
+    assert 1e-5 < d12 < 1e-2 and d64 < 3e-2
+
+
 def test_forces_are_minus_energy_gradient():
     pos, L = cg_config(60, seed=3)
     m, nl, nn = model_for(pos, L)
