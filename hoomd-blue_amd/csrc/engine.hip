@@ -223,6 +223,10 @@ struct hm_system
     uint64_t timestep = 0;
     bool prepared = false;
     int compute_virial = 0;
+    // FP32 copy of the positions for the learned-CG pair passes (allocated with the first such
+    // force); written by step one, so inside a step the forces read it as it is
+    float4* pos32 = nullptr;
+    bool in_step = false;
     PairLaunch pair_cfg{4, 256};      // gather kernels: lanes per particle, block size
     PairLaunch pair_cell_cfg{8, 0};   // cell-tiled kernel: lanes per particle, workgroups
     PairLaunch pair_tile_cfg{8, 2048, 8};  // FP32 cell tiles: lanes, workgroups, row batch
@@ -264,6 +268,8 @@ struct hm_system
     {
         clear_graphs();
         dfree(pos);
+        dfree(pos32);
+        pos32 = nullptr;
         dfree(vel);
         dfree(accel);
         dfree(image);
@@ -800,7 +806,15 @@ struct hm_system
 
     void launch_mlp(ForceSlot& f, int with_virial)
     {
-        const MlpArgs a = mlp_args(f, with_virial);
+        MlpArgs a = mlp_args(f, with_virial);
+        if (pos32)
+        {
+            // step one of this step wrote the copy; elsewhere (set-up, thermo, after a sort)
+            // it is refreshed first
+            if (!in_step)
+                hchk(launch_pos_f32(N, pos, pos32, stream), "fp32 positions");
+            a.pos32 = pos32;
+        }
         hchk(launch_mlp_descriptor(a, stream), "mlp descriptor");
         hchk(launch_mlp_site(a, stream), "mlp site energies");
         hchk(launch_mlp_force(a, false, stream), "mlp forces");
@@ -879,6 +893,7 @@ struct hm_system
         a1.dt = dt;
         a1.alpha = 1.0;
         a1.st = st;
+        a1.pos32 = pos32;
         a1.use_state_alpha = bussi ? 1 : 0;
         a1.begin = bussi ? nullptr : st;
         bool have_nl = rcut_max > 0.0;
@@ -923,7 +938,15 @@ struct hm_system
             fill_nlist_args(k, true);
             hchk(launch_nlist_build(k, stream), "nlist build");
         }
-        launch_forces(compute_virial);
+        {
+            struct InStep
+            {
+                bool& f;
+                explicit InStep(bool& x) : f(x) { f = true; }
+                ~InStep() { f = false; }
+            } guard(in_step);
+            launch_forces(compute_virial);
+        }
         StepTwoArgs a2;
         memset(&a2, 0, sizeof(a2));
         a2.N = N;
@@ -2374,6 +2397,8 @@ int hm_sys_add_mlp_cg(hm_system* s, const hm_mlp_cg_model* model, const float* p
         f.mG = dalloc<float4>((size_t)4 * N);
         f.mA = dalloc<float4>((size_t)4 * N);
         f.mAb = dalloc<uint2>((size_t)4 * N);
+        if (!s->pos32)
+            s->pos32 = dalloc<float4>(N);
         f.mE = dalloc<float>(N);
         hchk(hipMemset(f.f, 0, sizeof(float4) * N), "clear");
         hchk(hipMemset(f.v, 0, sizeof(float) * 6 * N), "clear");
@@ -2781,7 +2806,12 @@ int hm_sys_time_mlp(hm_system* s, int force_id, unsigned int iters, double* avg_
     return guarded([&]() {
         if (!s->prepared)
             s->prep();
-        const MlpArgs a = s->mlp_args(s->forces[force_id], s->compute_virial);
+        MlpArgs a = s->mlp_args(s->forces[force_id], s->compute_virial);
+        if (s->pos32)
+        {
+            hchk(launch_pos_f32(s->N, s->pos, s->pos32, s->stream), "fp32 positions");
+            a.pos32 = s->pos32;
+        }
         hipEvent_t e[4];
         for (auto& x : e)
             hchk(hipEventCreate(&x), "event");

@@ -419,6 +419,8 @@ struct StepOneArgs
     // fused clearing of the cell occupancy counters
     unsigned int* cell_size;
     unsigned int ncell;
+    // FP32 copy of the new positions (x, y, z, 0) for the learned-CG pair passes, or NULL
+    float4* pos32;
     // when set, this launch begins the step without a step-begin kernel: every workgroup takes
     // the current step from begin->next (which no kernel of the step writes before step two)
     // and one thread stores it in begin->timestep for the kernels that follow

@@ -2662,6 +2662,8 @@ __global__ __launch_bounds__(256) void step_one_kernel(StepOneArgs a)
         a.vel[n] = v;
         a.pos[n] = p;
         a.image[n] = img;
+        if (a.pos32)
+            a.pos32[n] = make_float4((float)px, (float)py, (float)pz, 0.f);
         if (a.rel.rel)
         {
             const float4 r = rel_of(px, py, pz, a.rel.pcell[n], a.box, a.rel);

@@ -81,7 +81,12 @@ struct MlpArgs
     // bf16 copy of A (4 values per uint2, 16 per site) written by the site kernel and gathered
     // by the force pass for both partners, or NULL (the force pass then reads A)
     uint2* Ab;
+    // FP32 copy of the positions (x, y, z) gathered by the density and force passes, or NULL
+    // (they then read the FP64 positions)
+    const float4* pos32;
 };
+// FP32 copy of the FP64 positions (x, y, z, 0), round to nearest
+hipError_t launch_pos_f32(unsigned int N, const double4* pos, float4* pos32, hipStream_t s);
 
 MlpDescr make_mlp_descr(const hm_mlp_cg_model& m);
 unsigned int mlp_pair_blocks(unsigned int N);

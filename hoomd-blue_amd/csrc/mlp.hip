@@ -91,6 +91,41 @@ __device__ __forceinline__ void pair_vec(const DevBox& box, const double4& pi, c
     fz = (float)dz;
 }
 
+// Positions of a pair pass: the FP64 positions with the FP64 minimum image, or the FP32 copy.
+// With FP32 positions the difference of two FP32 values is exact or within half an ulp of the
+// difference, and the image shift subtracts the box length in two FP32 parts (L = Lh + Ll, the
+// shift by Lh exact by Sterbenz's lemma), so the pair vector keeps the FP32 positions' accuracy.
+template<bool P32> struct PairPos;
+template<> struct PairPos<false>
+{
+    double4 p;
+    __device__ __forceinline__ void load(const MlpArgs& a, unsigned int j) { p = a.pos[j]; }
+    __device__ __forceinline__ void vec(const MlpArgs& a, const PairPos& pj, float& dx, float& dy,
+                                        float& dz) const
+    {
+        pair_vec(a.box, p, pj.p, dx, dy, dz);
+    }
+};
+template<> struct PairPos<true>
+{
+    float4 p;
+    __device__ __forceinline__ void load(const MlpArgs& a, unsigned int j) { p = a.pos32[j]; }
+    __device__ __forceinline__ static float axis(float d, double L, double Linv)
+    {
+        const float Lh = (float)L, Ll = (float)(L - (double)Lh);
+        const float s = rintf(d * (float)Linv);
+        d = __builtin_fmaf(-s, Lh, d);
+        return __builtin_fmaf(-s, Ll, d);
+    }
+    __device__ __forceinline__ void vec(const MlpArgs& a, const PairPos& pj, float& dx, float& dy,
+                                        float& dz) const
+    {
+        dx = axis(p.x - pj.p.x, a.box.L[0], a.box.Linv[0]);
+        dy = axis(p.y - pj.p.y, a.box.L[1], a.box.Linv[1]);
+        dz = axis(p.z - pj.p.z, a.box.L[2], a.box.Linv[2]);
+    }
+};
+
 // cosine cutoff f_c(r) = (cos(pi r / rc) + 1) / 2 and its derivative. The hardware sine and
 // cosine take their argument in turns (v_sin_f32: sin(2 pi x)); x = r / (2 rc) lies in [0, 1/2].
 __device__ __forceinline__ void cutoff(const MlpDescr& d, float r, float& fc, float& dfc)
@@ -194,7 +229,7 @@ template<int T> __device__ __forceinline__ float group_sum(float v)
 }
 
 // ------------------------------------------------------------------------------ pass 1
-template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
+template<int U, bool P32> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(MlpArgs a)
 {
     // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
     // so the neighbour gathers of one XCD stay in its own L2
@@ -207,7 +242,8 @@ template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(Mlp
         acc[k] = 0.f;
     if (valid)
     {
-        const double4 pi = a.pos[i];
+        PairPos<P32> pi;
+        pi.load(a, i);
         const unsigned int n = min(a.n_neigh[i], a.nmax);
         const unsigned int* row = a.nlist + (size_t)i * a.nmax;
         // two neighbours per iteration, and the row entries of the next iteration loaded one
@@ -221,13 +257,13 @@ template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(Mlp
         }
         for (unsigned int q0 = sub; q0 < n; q0 += PAIR_TPP * U)
         {
-            double4 pj[U];
+            PairPos<P32> pj[U];
             bool live[U];
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
                 live[u] = q0 + u * PAIR_TPP < n;
-                pj[u] = a.pos[jn[u]];
+                pj[u].load(a, jn[u]);
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -239,7 +275,7 @@ template<int U> __global__ __launch_bounds__(256) void mlp_descriptor_kernel(Mlp
             for (int u = 0; u < U; u++)
             {
                 float dx, dy, dz;
-                pair_vec(a.box, pi, pj[u], dx, dy, dz);
+                pi.vec(a, pj[u], dx, dy, dz);
                 const float rsq = dx * dx + dy * dy + dz * dz;
                 if (!live[u] || !(rsq < a.d.rcutsq))
                     continue;
@@ -479,7 +515,7 @@ template<> struct AjLoad<true>
     }
 };
 
-template<bool TRAIN, int U, bool BF> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
+template<bool TRAIN, int U, bool BF, bool P32> __global__ __launch_bounds__(MLP_PAIR_BLOCK) void mlp_force_kernel(MlpArgs a)
 {
     __shared__ float stab[2 * MLP_TAB_LDS];
     // XCD-aware: each XCD's workgroups take a contiguous eighth of the (Hilbert-ordered) sites,
@@ -510,7 +546,8 @@ template<bool TRAIN, int U, bool BF> __global__ __launch_bounds__(MLP_PAIR_BLOCK
             li.load(a, i);
             li.get(ai);
         }
-        const double4 pi = a.pos[i];
+        PairPos<P32> pi;
+        pi.load(a, i);
         const unsigned int n = min(a.n_neigh[i], a.nmax);
         const unsigned int* row = a.nlist + (size_t)i * a.nmax;
         // two neighbours per iteration: both position and both dE/dG gathers are issued before
@@ -533,12 +570,12 @@ template<bool TRAIN, int U, bool BF> __global__ __launch_bounds__(MLP_PAIR_BLOCK
                 live[u] = q0 + u * PAIR_TPP < n;
                 jj[u] = jn[u];
             }
-            double4 pj[U];
+            PairPos<P32> pj[U];
             AjLoad<BF> lj[U];
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
-                pj[u] = a.pos[jj[u]];
+                pj[u].load(a, jj[u]);
                 lj[u].load(a, jj[u]);
             }
 #pragma unroll
@@ -551,7 +588,7 @@ template<bool TRAIN, int U, bool BF> __global__ __launch_bounds__(MLP_PAIR_BLOCK
             for (int u = 0; u < U; u++)
             {
                 float dx, dy, dz;
-                pair_vec(a.box, pi, pj[u], dx, dy, dz);
+                pi.vec(a, pj[u], dx, dy, dz);
                 const float rsq = dx * dx + dy * dy + dz * dz;
                 if (!live[u] || !(rsq < a.d.rcutsq))
                     continue;
@@ -1151,14 +1188,37 @@ unsigned int mlp_pair_blocks(unsigned int N)
                     8u);
 }
 
+__global__ __launch_bounds__(256) void pos_f32_kernel(unsigned int N, const double4* pos,
+                                                      float4* pos32)
+{
+    const unsigned int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N)
+    {
+        const double4 p = pos[n];
+        pos32[n] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+    }
+}
+
+hipError_t launch_pos_f32(unsigned int N, const double4* pos, float4* pos32, hipStream_t s)
+{
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(pos_f32_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, pos, pos32);
+    return hipGetLastError();
+}
+
 hipError_t launch_mlp_descriptor(const MlpArgs& a, hipStream_t s)
 {
     if (a.N == 0)
         return hipSuccess;
     // two neighbours per iteration in flight (one: 44 us at 200 000 sites; three or four: no
     // faster than two)
-    hipLaunchKernelGGL((mlp_descriptor_kernel<2>), dim3(mlp_pair_blocks(a.N)),
-                       dim3(MLP_PAIR_BLOCK), 0, s, a);
+    if (a.pos32)
+        hipLaunchKernelGGL((mlp_descriptor_kernel<2, true>), dim3(mlp_pair_blocks(a.N)),
+                           dim3(MLP_PAIR_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((mlp_descriptor_kernel<2, false>), dim3(mlp_pair_blocks(a.N)),
+                           dim3(MLP_PAIR_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1176,14 +1236,18 @@ hipError_t launch_mlp_force(const MlpArgs& a, bool train, hipStream_t s)
         return hipSuccess;
     // two neighbours per iteration in flight (three: 141 VGPRs, 3 wavefronts per SIMD, slower)
     const dim3 g(mlp_pair_blocks(a.N)), b(MLP_PAIR_BLOCK);
-    if (train && a.Ab)
-        hipLaunchKernelGGL((mlp_force_kernel<true, 2, true>), g, b, 0, s, a);
-    else if (train)
-        hipLaunchKernelGGL((mlp_force_kernel<true, 2, false>), g, b, 0, s, a);
-    else if (a.Ab)
-        hipLaunchKernelGGL((mlp_force_kernel<false, 2, true>), g, b, 0, s, a);
-    else
-        hipLaunchKernelGGL((mlp_force_kernel<false, 2, false>), g, b, 0, s, a);
+    const int v = (train ? 4 : 0) | (a.Ab ? 2 : 0) | (a.pos32 ? 1 : 0);
+    switch (v)
+    {
+    case 0: hipLaunchKernelGGL((mlp_force_kernel<false, 2, false, false>), g, b, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((mlp_force_kernel<false, 2, false, true>), g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((mlp_force_kernel<false, 2, true, false>), g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((mlp_force_kernel<false, 2, true, true>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((mlp_force_kernel<true, 2, false, false>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((mlp_force_kernel<true, 2, false, true>), g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((mlp_force_kernel<true, 2, true, false>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((mlp_force_kernel<true, 2, true, true>), g, b, 0, s, a); break;
+    }
     return hipGetLastError();
 }
 
@@ -1336,6 +1400,7 @@ MlpArgs to_args(const hm_mlp_cg_args* x, const hm_mlp_cg_model* m)
     a.gpart = x->grad_partial;
     a.loss_scale = x->N ? (float)(2.0 / (3.0 * (double)x->N)) : 0.f;
     a.Ab = (uint2*)x->A_bf16;
+    a.pos32 = (const float4*)x->pos32;
     return a;
 }
 } // namespace

@@ -89,7 +89,8 @@ class HmMlpArgs(C.Structure):
     _fields_ = [("N", _u), ("pos", _p), ("box", HmBox), ("nlist", _p), ("nmax", _u),
                 ("n_neigh", _p), ("params", _p), ("params_bf16", _p), ("G", _p), ("A", _p),
                 ("Esite", _p), ("force", _p), ("virial", _p), ("target", _p), ("err", _p),
-                ("C", _p), ("loss_partial", _p), ("grad_partial", _p), ("A_bf16", _p)]
+                ("C", _p), ("loss_partial", _p), ("grad_partial", _p), ("A_bf16", _p),
+                ("pos32", _p)]
 
 
 class HmMlpTrainArgs(C.Structure):

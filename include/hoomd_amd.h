@@ -379,6 +379,10 @@ typedef struct hm_mlp_cg_args
      * hm_mlp_cg_forward writes it beside A and hm_mlp_cg_forces reads it for both partners of
      * every pair (32 instead of 64 gathered bytes per pair). */
     uint16_t* A_bf16;              /* [16N] */
+    /* optional FP32 positions (x, y, z, unused; 16-byte aligned), or NULL. When set, the
+     * density and force passes gather these (16 instead of 24 bytes per pair) and take the
+     * minimum image in FP32 with the box length split into two FP32 parts. */
+    const float* pos32;            /* [4N] */
 } hm_mlp_cg_args;
 
 /* Entries of loss_partial for N sites. */

@@ -67,7 +67,7 @@ def cg_config(N, rho=0.17, seed=1, rmin=0.8):
 class Setup:
     """Buffers of hm_mlp_cg_args for one configuration, its oracle list and a model."""
 
-    def __init__(self, H, N, seed, params_seed, bf16_a=False):
+    def __init__(self, H, N, seed, params_seed, bf16_a=False, pos32=False):
         self.H = H
         pos, L = cg_config(N, seed=seed)
         self.pos, self.L, self.N = pos, L, N
@@ -112,6 +112,11 @@ class Setup:
         if bf16_a:
             self.d_Ab = H.DeviceArray((16 * N,), np.uint16, fill=0)
             a.A_bf16 = self.d_Ab.ptr
+        if pos32:
+            p32 = np.zeros((N, 4), np.float32)
+            p32[:, :3] = pos.astype(np.float32)
+            self.d_p32 = dev(p32)
+            a.pos32 = self.d_p32.ptr
         a.loss_partial, a.grad_partial = self.d_lp.ptr, self.d_gp.ptr
         self.args = a
         H.check(H.lib().hm_mlp_cg_pack(self.d_par.ptr, self.d_bf.ptr, None), "pack")
@@ -157,13 +162,20 @@ def bf16_rne(x):
     return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 
-@pytest.mark.parametrize("N,seed", [(3000, 4), (333, 5)])
-def test_mlp_bf16_dEdG_forces_match_oracle(H, N, seed):
+@pytest.mark.parametrize("N,seed,pos32", [(3000, 4, False), (333, 5, False), (3000, 6, True),
+                                          (517, 7, True)])
+def test_mlp_bf16_dEdG_forces_match_oracle(H, N, seed, pos32):
     """The force pass on the bf16 copy of dE/dG: the copy is A rounded to nearest even, bit for
-    bit, and the forces follow the oracle that rounds a(i) in pass 2 (bf16 mode 2)."""
-    print(f"mlp forward, bf16 dE/dG: N={N} seed={seed}", flush=True)
-    S = Setup(H, N, seed, params_seed=seed + 10, bf16_a=True)
+    bit, and the forces follow the oracle that rounds a(i) in pass 2 (bf16 mode 2). With the
+    FP32 position copy both pair passes match the oracle evaluated at the FP32-rounded
+    positions (the densities to the same 1e-5 as from FP64 positions)."""
+    print(f"mlp forward, bf16 dE/dG: N={N} seed={seed} pos32={pos32}", flush=True)
+    S = Setup(H, N, seed, params_seed=seed + 10, bf16_a=True, pos32=pos32)
+    if pos32:
+        S.pos = S.pos.astype(np.float32).astype(np.float64)
     S.forward()
+    G = S.d_G.numpy().reshape(N, 16)
+    assert rel_rms(G, M.descriptor(S.om, S.L, S.pos, S.nl, S.nn)) < 1e-5
     A = S.d_A.numpy()
     np.testing.assert_array_equal(S.d_Ab.numpy(), bf16_rne(A))
     F = S.d_F.numpy().reshape(N, 4)
