@@ -447,7 +447,7 @@ MLP_SITE_FLOP = 2 * (16 * 64 + 64 * 64) + 2 * 64 + 2 * (64 * 64 + 64 * 16)  # fo
 
 
 def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pushoff, lr=2e-3,
-         frames_per_step=1, ibi_iters=12, cg_dt=0.02, pressure_match=False):
+         frames_per_step=1, ibi_iters=12, cg_dt=0.02, pressure_match=False, cg_buffer=0.4):
     """Learned CG of the Kremer-Grest melt, end to end on the GPU, against the plain run:
 
     1. fine-grained melt: n_chains x 50 random-walk chains (bond 0.97, rho 0.85) pushed off with
@@ -527,7 +527,7 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     snap.bonds.group[:] = cgb
     cg = hoomd.Simulation(device=hoomd.device.GPU(), seed=seed + 1)
     cg.create_state_from_snapshot(snap)
-    nl = hoomd.md.nlist.Cell(buffer=0.4, exclusions=("bond",))
+    nl = hoomd.md.nlist.Cell(buffer=cg_buffer, exclusions=("bond",))
     mlp = hoomd.md.mlcg.SiteEnergyMLP(nlist=nl, seed=seed)
     harm = hoomd.md.bond.Harmonic()
     harm.params["cg"] = dict(k=kb, r0=r0)
@@ -629,6 +629,9 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     cg.run(steps)
     _lib.check(_lib.lib().hm_synchronize())
     tps_cg_ibi = steps / (time.perf_counter() - t0)
+    builds_cg = nl.num_builds  # builds of the timed run
+    log(f"[mlcg] final model timed: {tps_cg_ibi:.1f} steps/s, {builds_cg} list builds in {steps} "
+        f"steps (buffer {cg_buffer})")
     trained_params = mlp.params
     mlp.params = np.zeros(MC.NPARAM, np.float32)
     mlp.pair_table = prior_params
@@ -669,6 +672,7 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
         "plain": {"timesteps_per_s": tps_fg, "particles": N, "dt": 0.01,
                   "tau_per_s": tps_fg * 0.01, "pressure": float(np.mean(p_fg))},
         "cg": {"timesteps_per_s": tps_cg_ibi, "sites": smap.nsites, "dt": cg_dt,
+               "nlist_buffer": cg_buffer, "list_builds_in_timed_steps": builds_cg,
                "tau_per_s": tps_cg_ibi * cg_dt, "pressure": float(p_ibi),
                "model": "trained site energy + refined pair prior (in the force pass) + "
                         "refined harmonic bonds",
@@ -845,6 +849,8 @@ def main():
     ap.add_argument("--cg-dt", type=float, default=0.02,
                     help="timestep of the CG run (the learned-CG configuration's 0.02; 0.04 "
                          "gives the same structure and temperature)")
+    ap.add_argument("--cg-buffer", type=float, default=0.4,
+                    help="neighbour-list buffer of the CG run (site cut-off 3.0)")
     ap.add_argument("--pressure-match", action="store_true",
                     help="add the linear pressure correction to each refinement step")
     ap.add_argument("--cg-step", type=int, default=0, metavar="CHAINS",
@@ -862,7 +868,7 @@ def main():
             return 2
         return mlcg(a.mlcg, a.steps, a.warmup, a.seed, a.frames, a.frame_every, a.train_steps,
                     a.pushoff, a.lr, a.frames_per_step, a.ibi_iters, a.cg_dt,
-                    a.pressure_match)
+                    a.pressure_match, a.cg_buffer)
     if a.cg_step:
         if a.gpus != 1:
             print("bench.py runs on exactly one GPU", file=sys.stderr)

@@ -807,6 +807,7 @@ struct hm_system
     void launch_mlp(ForceSlot& f, int with_virial)
     {
         MlpArgs a = mlp_args(f, with_virial);
+        a.A = nullptr;  // the force pass reads the bf16 copy; the FP32 dE/dG is for training
         if (pos32)
         {
             // step one of this step wrote the copy; elsewhere (set-up, thermo, after a sort)
@@ -2807,6 +2808,7 @@ int hm_sys_time_mlp(hm_system* s, int force_id, unsigned int iters, double* avg_
         if (!s->prepared)
             s->prep();
         MlpArgs a = s->mlp_args(s->forces[force_id], s->compute_virial);
+        a.A = nullptr;  // as in a step: the bf16 copy only
         if (s->pos32)
         {
             hchk(launch_pos_f32(s->N, s->pos, s->pos32, s->stream), "fp32 positions");

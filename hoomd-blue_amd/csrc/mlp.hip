@@ -454,7 +454,8 @@ __global__ __launch_bounds__(SITE_BLOCK) void mlp_site_kernel(MlpArgs a)
             o.y = a.d.gscale[4 * g + 1] * T.dx[1];
             o.z = a.d.gscale[4 * g + 2] * T.dx[2];
             o.w = a.d.gscale[4 * g + 3] * T.dx[3];
-            a.A[(size_t)site * 4 + g] = o;
+            if (a.A)
+                a.A[(size_t)site * 4 + g] = o;
             if (a.Ab)
             {
                 const bf4 ob = pack4(o.x, o.y, o.z, o.w);
@@ -1431,7 +1432,8 @@ int hm_mlp_cg_descriptor(const hm_mlp_cg_args* x, const hm_mlp_cg_model* m, hm_s
 int hm_mlp_cg_forward(const hm_mlp_cg_args* x, const hm_mlp_cg_model* m, hm_stream_t stream)
 {
     if (!x || !model_ok(m)
-        || (x->N && (!x->G || !x->A || !x->Esite || !x->params || !x->params_bf16)))
+        || (x->N && (!x->G || (!x->A && !x->A_bf16) || !x->Esite || !x->params
+                      || !x->params_bf16)))
         return fail(HM_ERR_ARGS, "hm_mlp_cg_forward: invalid arguments");
     return check(launch_mlp_site(to_args(x, m), (hipStream_t)stream), "hm_mlp_cg_forward");
 }
@@ -1440,7 +1442,9 @@ int hm_mlp_cg_forces(const hm_mlp_cg_args* x, const hm_mlp_cg_model* m, int trai
                      hm_stream_t stream)
 {
     if (!x || !model_ok(m)
-        || (x->N && (!x->pos || !x->nlist || !x->n_neigh || !x->A || !x->Esite || !x->force))
+        || (x->N
+            && (!x->pos || !x->nlist || !x->n_neigh || (!x->A && !x->A_bf16) || !x->Esite
+                || !x->force))
         || (train && x->N && (!x->target || !x->err || !x->loss_partial)))
         return fail(HM_ERR_ARGS, "hm_mlp_cg_forces: invalid arguments");
     return check(launch_mlp_force(to_args(x, m), train != 0, (hipStream_t)stream),

@@ -376,8 +376,9 @@ typedef struct hm_mlp_cg_args
     float* loss_partial;           /* [hm_mlp_cg_loss_blocks(N)] */
     float* grad_partial;           /* [HM_MLP_GRAD_BLOCKS * HM_MLP_NPARAM] */
     /* optional bf16 copy of A (round to nearest even; 16-byte aligned), or NULL. When set,
-     * hm_mlp_cg_forward writes it beside A and hm_mlp_cg_forces reads it for both partners of
-     * every pair (32 instead of 64 gathered bytes per pair). */
+     * hm_mlp_cg_forward writes it beside A (A may then be NULL: only the copy is written) and
+     * hm_mlp_cg_forces reads it for both partners of every pair (32 instead of 64 gathered
+     * bytes per pair). */
     uint16_t* A_bf16;              /* [16N] */
     /* optional FP32 positions (x, y, z, unused; 16-byte aligned), or NULL. When set, the
      * density and force passes gather these (16 instead of 24 bytes per pair) and take the

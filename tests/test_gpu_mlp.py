@@ -196,6 +196,14 @@ def test_mlp_bf16_dEdG_forces_match_oracle(H, N, seed, pos32):
     assert errs["F"] < errs["F_vs_mode1"]
     # pairs use the same rounded values on both sides: the forces still sum to zero
     assert np.abs(F[:, :3].astype(np.float64).sum(axis=0)).max() < 1e-3 * np.abs(F[:, :3]).max()
+    # without A the site kernel writes only the copy, and the forces are unchanged
+    Ab = S.d_Ab.numpy()
+    H.check(H.lib().hm_memset(S.d_Ab.ptr, 0, Ab.nbytes))
+    H.check(H.lib().hm_memset(S.d_F.ptr, 0, 16 * N))
+    S.args.A = None
+    S.forward()
+    np.testing.assert_array_equal(S.d_Ab.numpy(), Ab)
+    np.testing.assert_array_equal(S.d_F.numpy().reshape(N, 4), F)
 
 
 def test_mlp_gradient_and_adam_match_oracle(H):
