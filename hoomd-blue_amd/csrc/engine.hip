@@ -1047,8 +1047,13 @@ struct hm_system
         for (int sh : shapes)
         {
             const bool one_wave = sh == NL_SHAPE_TINY || sh == NL_SHAPE_SMALL;
-            for (unsigned int g : one_wave ? std::vector<unsigned int>{2048u, 4096u, 8192u, 16384u}
-                                           : std::vector<unsigned int>{1024u, 2048u, 4096u, 8192u})
+            std::vector<unsigned int> grids =
+                one_wave ? std::vector<unsigned int>{2048u, 4096u, 8192u, 16384u}
+                         : std::vector<unsigned int>{1024u, 2048u, 4096u, 8192u};
+            // one-wavefront shapes also try one workgroup per cell (no persistent loop)
+            if (one_wave && round_up(ncell, 8) > 16384u)
+                grids.push_back(round_up(ncell, 8));
+            for (unsigned int g : grids)
             {
                 if (g > round_up(ncell, 8) && g != 1024u && g != 2048u)
                     continue;
