@@ -632,6 +632,12 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
     builds_cg = nl.num_builds  # builds of the timed run
     log(f"[mlcg] final model timed: {tps_cg_ibi:.1f} steps/s, {builds_cg} list builds in {steps} "
         f"steps (buffer {cg_buffer})")
+    nl_out = np.zeros(3, np.int32)
+    nl_log = np.zeros((32, 3), np.float32)
+    rows = _lib.lib().hm_sys_nlist_launch(cg._system, nl_out.ctypes.data, nl_log.ctypes.data, 32)
+    for r in nl_log[:max(rows, 0)]:
+        log(f"[mlcg] CG list-build tuner shape={int(r[0])} grid={int(r[1])} {r[2] * 1e3:.1f} us")
+    cg_nl_launch = {"shape": int(nl_out[0]), "grid": int(nl_out[1])}
     trained_params = mlp.params
     mlp.params = np.zeros(MC.NPARAM, np.float32)
     mlp.pair_table = prior_params
@@ -673,6 +679,7 @@ def mlcg(n_chains, steps, warmup, seed, nframes, frame_every, train_steps, pusho
                   "tau_per_s": tps_fg * 0.01, "pressure": float(np.mean(p_fg))},
         "cg": {"timesteps_per_s": tps_cg_ibi, "sites": smap.nsites, "dt": cg_dt,
                "nlist_buffer": cg_buffer, "list_builds_in_timed_steps": builds_cg,
+               "nlist_launch": cg_nl_launch,
                "tau_per_s": tps_cg_ibi * cg_dt, "pressure": float(p_ibi),
                "model": "trained site energy + refined pair prior (in the force pass) + "
                         "refined harmonic bonds",
